@@ -1,0 +1,1424 @@
+// MI355X FDTD engine: device-resident worlds and the half-step programs that the host
+// library (alis.h) drives.  One world = one HIP stream; every launch is asynchronous.
+//
+// Layout in HBM.  Every field component is its own array over
+// [iMIN-1..iMAX+1] x [jMIN-1..jMAX+1] x [kMIN-1..kMAX+1] (ghost layer included), z
+// contiguous, each (i,j) row padded to a multiple of 128 bytes and shifted so that k = kMIN
+// starts on a 128-byte boundary: a 64-lane wavefront reading 64 consecutive k touches
+// exactly two cache lines.  CPML slabs normal to x and y use the same padded rows; slabs
+// normal to z, ADE currents and fill factors are compact over their boxes.
+//
+// Arithmetic.  Kernels keep the reference's operation order per cell (curl, then the CPML
+// slabs in face order xMin,yMin,zMin,xMax,yMax,zMax; ADE scaling, then -J per pole; strips
+// and sources in chain order) and are compiled without contraction, so FP32 results are
+// bitwise those of the reference's float code.  FP64 worlds run the same programs on
+// doubles (coefficients widened from the float setup).
+#include <hip/hip_runtime.h>
+#include <vector>
+#include <string>
+#include <cstring>
+#include <cstdio>
+#include <cstdint>
+#include <cmath>
+#include <algorithm>
+
+#include "alis_dev.h"
+
+namespace {
+
+// Boundary codes of the reference (src/alis_c.h:15-21).
+enum { SUR_SYM = 1, SUR_PEC = 2, SUR_PML = 3, SUR_PBC = 4, SUR_HBC = 5, SUR_BBC = 6 };
+
+std::string g_error;
+
+int fail(const char *what, hipError_t e)
+{
+	g_error = std::string(what) + ": " + hipGetErrorString(e);
+	return -1;
+}
+int fail(const std::string &what)
+{
+	g_error = what;
+	return -1;
+}
+
+#define HIPCHECK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(#call, e_); } while (0)
+
+constexpr int ROW_ALIGN_BYTES = 128;
+constexpr int MAXPOLE = ALIS_DEV_MAXPOLES;
+constexpr int MAXF = 64;          // phaser frequencies per update launch
+
+// Index map of one device array: (i - i0)*si + (j - j0)*sj + (k - k0) + koff.
+struct Lay {
+	int i0, j0, k0, koff;
+	long si, sj;
+	__host__ __device__ long at(int i, int j, int k) const { return (long)(i - i0) * si + (long)(j - j0) * sj + (k - k0) + koff; }
+};
+
+struct Box {
+	int i0, i1, j0, j1, k0, k1;
+	long count() const { return (i1 < i0 || j1 < j0 || k1 < k0) ? 0 : (long)(i1-i0+1)*(j1-j0+1)*(k1-k0+1); }
+};
+
+// ---------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------
+
+template<class T> struct CurlFace {
+	T *pa, *pb;                 // the two auxiliary components of this face
+	const T *b, *ca, *cb;       // recursion coefficients along the face axis (index - MIN)
+	Lay L;
+	int lo, hi, on;
+};
+
+template<class T> struct CurlArgs {
+	T *F0, *F1, *F2;
+	const T *G0, *G1, *G2;
+	const T *C0, *C1, *C2;
+	const T *dtx, *dty, *dtz;   // dt/d along each axis, index - MIN
+	Lay L;
+	int iMIN, jMIN, kMIN, ni, nj, nk;
+	int alt;                    // +1: E from H, -1: H from E
+	CurlFace<T> face[6];        // xMin yMin zMin xMax yMax zMax
+};
+
+// Curl update of one half step plus the CPML slabs (src/update.c:48-125).  One thread per
+// cell; 64 lanes along z, 4 rows along y per workgroup, one x plane per grid z index.
+template<class T, bool HASC>
+__global__ __launch_bounds__(256) void k_curl(CurlArgs<T> a)
+{
+	const int k = a.kMIN + (int)blockIdx.x * 64 + (int)threadIdx.x;
+	const int j = a.jMIN + (int)blockIdx.y * 4 + (int)threadIdx.y;
+	const int i = a.iMIN + (int)blockIdx.z;
+	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
+
+	const long id = a.L.at(i, j, k);
+	const long oI = a.alt * a.L.si, oJ = a.alt * a.L.sj, oK = a.alt;
+	const T gx = a.G0[id], gy = a.G1[id], gz = a.G2[id];
+	const T dx = a.dtx[i - a.iMIN], dy = a.dty[j - a.jMIN], dz = a.dtz[k - a.kMIN];
+	T fx = a.F0[id], fy = a.F1[id], fz = a.F2[id];
+	T cx = 1, cy = 1, cz = 1;
+
+	T ux = dz * (gy - a.G1[id + oK]) + dy * (a.G2[id + oJ] - gz);
+	T uy = dx * (gz - a.G2[id + oI]) + dz * (a.G0[id + oK] - gx);
+	T uz = dy * (gx - a.G0[id + oJ]) + dx * (a.G1[id + oI] - gy);
+	if (HASC) {
+		cx = a.C0[id], cy = a.C1[id], cz = a.C2[id];
+		fx = fx + ux * cx, fy = fy + uy * cy, fz = fz + uz * cz;
+	}
+	else {
+		fx = fx + ux, fy = fy + uy, fz = fz + uz;
+	}
+
+#pragma unroll
+	for (int f = 0; f < 6; f++) {
+		const CurlFace<T> &P = a.face[f];
+		if (!P.on) continue;
+		const int ax = f % 3;
+		const int c = ax == 0 ? i : ax == 1 ? j : k;
+		if (c < P.lo || c > P.hi) continue;
+		const long p = P.L.at(i, j, k);
+		const int q = c - (ax == 0 ? a.iMIN : ax == 1 ? a.jMIN : a.kMIN);
+		const T b = P.b[q], ca = P.ca[q], cb = P.cb[q];
+		if (ax == 0) {
+			T pa = b * P.pa[p] + ca * (gz - a.G2[id + oI]);
+			T pb = b * P.pb[p] + cb * (gy - a.G1[id + oI]);
+			P.pa[p] = pa, P.pb[p] = pb;
+			fy = fy + (HASC ? pa * cy : pa);
+			fz = fz + (HASC ? pb * cz : pb);
+		}
+		else if (ax == 1) {
+			T pa = b * P.pa[p] + ca * (gx - a.G0[id + oJ]);
+			T pb = b * P.pb[p] + cb * (gz - a.G2[id + oJ]);
+			P.pa[p] = pa, P.pb[p] = pb;
+			fz = fz + (HASC ? pa * cz : pa);
+			fx = fx + (HASC ? pb * cx : pb);
+		}
+		else {
+			T pa = b * P.pa[p] + ca * (gy - a.G1[id + oK]);
+			T pb = b * P.pb[p] + cb * (gx - a.G0[id + oK]);
+			P.pa[p] = pa, P.pb[p] = pb;
+			fx = fx + (HASC ? pa * cx : pa);
+			fy = fy + (HASC ? pb * cy : pb);
+		}
+	}
+	a.F0[id] = fx, a.F1[id] = fy, a.F2[id] = fz;
+}
+
+// ADE: per dispersive object, over its box clipped to the grid.
+template<class T> struct AdeArgs {
+	T *F[3];                    // field components
+	Lay L;                      // field layout
+	const T *fill[3];           // fill factors (compact over the object box)
+	T *J[3][MAXPOLE], *K[3][MAXPOLE];
+	Lay LO;                     // compact object-box layout
+	int i0, j0, k0, ni, nj, nk; // clipped box
+	T a[3];                     // conductivity factors
+	T JJ[3][MAXPOLE], JK[3][MAXPOLE], JE[3][MAXPOLE], KJ[3][MAXPOLE], KK[3][MAXPOLE], KE[3][MAXPOLE];
+	int NJ, NK, mode;           // mode 0: scale, 1: subtract J, 2: update J/K
+};
+
+__device__ inline void boxCell(long t, int ni, int nj, int nk, int &di, int &dj, int &dk)
+{
+	dk = (int)(t % nk);
+	long r = t / nk;
+	dj = (int)(r % nj);
+	di = (int)(r / nj);
+	(void)ni;
+}
+
+// Conductivity scaling, current subtraction and current update of one object
+// (src/update.c:89-108, 128-158).
+template<class T>
+__global__ __launch_bounds__(256) void k_ade(AdeArgs<T> a)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long)a.ni * a.nj * a.nk) return;
+	int di, dj, dk;
+	boxCell(t, a.ni, a.nj, a.nk, di, dj, dk);
+	const int i = a.i0 + di, j = a.j0 + dj, k = a.k0 + dk;
+	const long id = a.L.at(i, j, k), o = a.LO.at(i, j, k);
+	if (a.mode == 0) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			T fl = a.fill[c][o];
+			a.F[c][id] *= (T(1) - a.a[c] * fl) / (T(1) + a.a[c] * fl);
+		}
+	}
+	else if (a.mode == 1) {
+		T f0 = a.F[0][id], f1 = a.F[1][id], f2 = a.F[2][id];
+		for (int p = 0; p < a.NK; p++) {
+			f0 -= a.J[0][p][o];
+			f1 -= a.J[1][p][o];
+			f2 -= a.J[2][p][o];
+		}
+		a.F[0][id] = f0, a.F[1][id] = f1, a.F[2][id] = f2;
+	}
+	else {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			const T e = a.F[c][id], fl = a.fill[c][o];
+			for (int p = 0; p < a.NJ; p++)
+				a.J[c][p][o] = a.JJ[c][p] * a.J[c][p][o] + a.JE[c][p] * fl * e;
+			for (int p = a.NJ; p < a.NK; p++) {
+				T Jn = a.JJ[c][p] * a.J[c][p][o] + a.JK[c][p] * a.K[c][p][o] + a.JE[c][p] * fl * e;
+				T Kn = a.KJ[c][p] * Jn + a.KK[c][p] * a.K[c][p][o] + a.KE[c][p] * fl * e;
+				a.J[c][p][o] = Jn, a.K[c][p][o] = Kn;
+			}
+		}
+	}
+}
+
+// Small patch operations on boxes: ghost copies, TF/SF strips and source injection.
+enum { OP_COPY = 0, OP_STRIP = 1, OP_SOURCE = 2 };
+
+template<class T> struct Op {
+	int kind;
+	T *F, *G;                   // COPY: re / im (G may be null); STRIP: target / aux source
+	const T *C;                 // STRIP: inverse permittivity or null; SOURCE: amplitude array or null
+	const T *W;                 // SOURCE: per-cell wave or null
+	Lay LF, LG, LC;             // layouts of F, G (aux for STRIP), C/W (compact box for SOURCE)
+	Box b;
+	int di, dj, dk;
+	T c, s;                     // COPY: cos, sin; STRIP: dt/ds; SOURCE: amplitude, wave
+	int exX, exY, auxIMAX, auxJMAX, altX, altY;
+};
+
+constexpr int MAXOPS = 12;
+template<class T> struct OpGroup {
+	int n;
+	Op<T> op[MAXOPS];
+};
+
+template<class T>
+__global__ __launch_bounds__(256) void k_ops(OpGroup<T> g)
+{
+	const Op<T> &o = g.op[blockIdx.y];
+	const int ni = o.b.i1 - o.b.i0 + 1, nj = o.b.j1 - o.b.j0 + 1, nk = o.b.k1 - o.b.k0 + 1;
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long)ni * nj * nk) return;
+	int di, dj, dk;
+	boxCell(t, ni, nj, nk, di, dj, dk);
+	const int i = o.b.i0 + di, j = o.b.j0 + dj, k = o.b.k0 + dk;
+	const long d = o.LF.at(i, j, k);
+	if (o.kind == OP_COPY) {
+		const long s = o.LF.at(i + o.di, j + o.dj, k + o.dk);
+		if (o.G) {
+			const T fs = o.F[s], gs = o.G[s];
+			o.F[d] = o.c * fs + o.s * gs;
+			o.G[d] = o.c * gs - o.s * fs;
+		}
+		else o.F[d] = o.c * o.F[s];
+	}
+	else if (o.kind == OP_STRIP) {
+		int ia = i + o.di, ja = j + o.dj;
+		if (o.exX && ia > o.auxIMAX) ia = o.auxIMAX + o.altX;
+		if (o.exY && ja > o.auxJMAX) ja = o.auxJMAX + o.altY;
+		const T g = o.G[o.LG.at(ia, ja, k + o.dk)];
+		if (o.C) o.F[d] = o.F[d] + o.c * g * o.C[o.LC.at(i, j, k)];
+		else o.F[d] = o.F[d] + o.c * g;
+	}
+	else {
+		const long c = o.LC.at(i, j, k);
+		const T amp = o.C ? o.C[c] : o.c;
+		const T w = o.W ? o.W[c] : o.s;
+		o.F[d] = o.F[d] + amp * w;
+	}
+}
+
+// Phaser running DFT (src/phaser.c:83-110): one thread per face cell, the four face
+// components are read once and accumulated into every frequency of the launch.
+struct PhsFaceGeom {
+	int kind;                   // 0 xMin 1 xMax 2 yMin 3 yMax 4 zMin 5 zMax
+	int fixed;                  // the face's fixed index
+	int n2;                     // cells along the fast face axis
+	long off;                   // first cell of this face in the concatenated cell list
+};
+
+template<class T> struct PhsArgs {
+	const T *E[3], *H[3];
+	Lay L;
+	int iMin, jMin, kMin, nface;
+	PhsFaceGeom face[6];
+	long ncell;
+	void *acc;                  // [freq][comp 4][cell] complex (TA pairs)
+	int q0, nq;
+	double eE[MAXF][2], eH[MAXF][2];
+};
+
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= a.ncell) return;
+	int f = 0;
+	while (f + 1 < a.nface && t >= a.face[f + 1].off) f++;
+	const PhsFaceGeom &g = a.face[f];
+	long local = t - g.off;
+	int u = (int)(local / g.n2), v = (int)(local % g.n2);
+	int i, j, k;
+	T e1, e2, h1, h2;
+	if (g.kind < 2) {
+		i = g.fixed, j = a.jMin + u, k = a.kMin + v;
+		long id = a.L.at(i, j, k), s = a.L.si;
+		e1 = a.E[1][id], e2 = a.E[2][id];
+		h1 = a.H[1][id] + a.H[1][id + s], h2 = a.H[2][id] + a.H[2][id + s];
+	}
+	else if (g.kind < 4) {
+		i = a.iMin + u, j = g.fixed, k = a.kMin + v;
+		long id = a.L.at(i, j, k), s = a.L.sj;
+		e1 = a.E[2][id], e2 = a.E[0][id];
+		h1 = a.H[2][id] + a.H[2][id + s], h2 = a.H[0][id] + a.H[0][id + s];
+	}
+	else {
+		i = a.iMin + u, j = a.jMin + v, k = g.fixed;
+		long id = a.L.at(i, j, k);
+		e1 = a.E[0][id], e2 = a.E[1][id];
+		h1 = a.H[0][id] + a.H[0][id + 1], h2 = a.H[1][id] + a.H[1][id + 1];
+	}
+	TA *acc = (TA *)a.acc;
+	const TA half = TA(0.5);
+	for (int q = 0; q < a.nq; q++) {
+		const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
+		const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
+		const long base = (long)(a.q0 + q) * 4 * a.ncell;
+		TA *p0 = acc + 2 * (base + t);
+		TA *p1 = acc + 2 * (base + a.ncell + t);
+		TA *p2 = acc + 2 * (base + 2 * a.ncell + t);
+		TA *p3 = acc + 2 * (base + 3 * a.ncell + t);
+		p0[0] = p0[0] + er * TA(e1), p0[1] = p0[1] + ei * TA(e1);
+		p1[0] = p1[0] + er * TA(e2), p1[1] = p1[1] + ei * TA(e2);
+		p2[0] = p2[0] + hr * TA(h1) * half, p2[1] = p2[1] + hi * TA(h1) * half;
+		p3[0] = p3[0] + hr * TA(h2) * half, p3[1] = p3[1] + hi * TA(h2) * half;
+	}
+}
+
+// Pack a box of a device array into a compact buffer (float or double), and back.
+template<class T, class U>
+__global__ __launch_bounds__(256) void k_pack(const T *src, Lay L, Box b, U *dst)
+{
+	const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long)ni * nj * nk) return;
+	int di, dj, dk;
+	boxCell(t, ni, nj, nk, di, dj, dk);
+	dst[t] = (U)src[L.at(b.i0 + di, b.j0 + dj, b.k0 + dk)];
+}
+
+template<class T, class U>
+__global__ __launch_bounds__(256) void k_unpack(const U *src, Lay L, Box b, T *dst)
+{
+	const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long)ni * nj * nk) return;
+	int di, dj, dk;
+	boxCell(t, ni, nj, nk, di, dj, dk);
+	dst[L.at(b.i0 + di, b.j0 + dj, b.k0 + dk)] = (T)src[t];
+}
+
+template<class TA>
+__global__ __launch_bounds__(256) void k_acc_to_float(const TA *src, float *dst, long n)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) dst[t] = (float)src[t];
+}
+
+template<class TA>
+__global__ __launch_bounds__(256) void k_acc_to_double(const TA *src, double *dst, long n)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) dst[t] = (double)src[t];
+}
+
+inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Device world
+// ---------------------------------------------------------------------------------------
+
+struct DevSlab {
+	bool on = false;
+	int lo = 0, hi = -1;
+	Lay L{};
+	Box b{};                    // host extents of the slab array
+	void *p[2][2] = {};         // [m][a/b]
+};
+
+struct DevObj {
+	Box box{};                  // object box (C->iMin[n]..C->iMax[n] ...)
+	Box clip{};                 // box clipped to the grid [MIN..MAX]
+	Lay LO{};
+	int NJ = 0, NK = 0;
+	float a[3] = {};
+	float JJ[3][MAXPOLE] = {}, JK[3][MAXPOLE] = {}, JE[3][MAXPOLE] = {}, KJ[3][MAXPOLE] = {}, KK[3][MAXPOLE] = {}, KE[3][MAXPOLE] = {};
+	void *fill[3] = {};
+	void *J[2][3][MAXPOLE] = {}, *K[2][3][MAXPOLE] = {};
+};
+
+struct DevSrc {
+	int h = 0, m = 0, axis = 0;
+	Box b{};
+	Lay LB{};                   // compact box layout
+	float amp = 0;
+	void *ampArr = nullptr;     // per-cell amplitude (T) or null
+	bool cellWave = false;
+	void *waveDev = nullptr;    // per-cell wave buffer (T)
+	float *wavePinned = nullptr;
+	hipEvent_t waveDone = nullptr;
+	long ncell = 0;
+};
+
+struct DevTfsf {
+	alis_dev_tfsf S{};
+	alis_dev_world *aux = nullptr;
+};
+
+struct DevPhaser {
+	bool live = false;
+	std::vector<float *> cf;    // host cfield data starts [q][face][comp]
+	int nf = 0, nface = 0;
+	int iMin = 0, iMax = 0, jMin = 0, jMax = 0, kMin = 0, kMax = 0;
+	PhsFaceGeom face[6]{};
+	long ncell = 0;
+	void *acc = nullptr;        // TA
+	float *stage = nullptr;     // device float buffer for host syncs
+};
+
+struct alis_dev_world {
+	alis_dev_world_desc W{};    // copy of the descriptor (arrays below keep its tables alive)
+	std::vector<alis_dev_ade> objDesc;
+	std::vector<alis_dev_src> srcDesc[2];
+	std::vector<alis_dev_tfsf> tfDesc;
+	int fp64 = 0, dft64 = 0, cplx = 0;
+	size_t esz = 4;
+	int iMIN, iMAX, jMIN, jMAX, kMIN, kMAX, iMin, iMax, jMin, jMax, kMin, kMax, iNUM, jNUM, kNUM;
+	Lay L{};
+	long fsize = 0;             // elements per field array
+	void *F[2][2][3] = {};      // [h][m][axis]
+	void *C[3] = {};            // inverse permittivity of the E grid, or null
+	void *dt[2][3] = {};        // [h][axis] dt/d arrays (T), index - MIN
+	void *cp[2][3][3] = {};     // [h][axis][b, ca, cb] CPML arrays or null
+	DevSlab slab[2][6];
+	std::vector<DevObj> obj;
+	std::vector<DevSrc> src[2];
+	std::vector<DevTfsf> tf;
+	std::vector<std::vector<Op<float>>> ghostF[2];  // float programs (FP32 worlds)
+	std::vector<std::vector<Op<double>>> ghostD[2]; // double programs (FP64 worlds)
+	std::vector<DevPhaser> phs;
+	hipStream_t st = nullptr;
+	void *scratch = nullptr;    // pack/unpack buffer (double-sized)
+	long scratchElems = 0;
+	std::vector<void *> allocs;
+	long bytes = 0;
+	bool timing = false;
+	std::vector<hipEvent_t> evPairs[2];
+	double evMs[2] = {0, 0};
+	long evCount[2] = {0, 0};
+};
+
+namespace {
+
+int devAlloc(alis_dev_world *D, void **p, size_t bytes)
+{
+	if (!bytes) bytes = 16;
+	HIPCHECK(hipMalloc(p, bytes));
+	HIPCHECK(hipMemsetAsync(*p, 0, bytes, D->st));
+	D->allocs.push_back(*p);
+	D->bytes += bytes;
+	return 0;
+}
+
+// Upload a 1-D float host array of n entries (widened to T).
+template<class T>
+int upload1(alis_dev_world *D, void **dst, const float *src, int n)
+{
+	if (devAlloc(D, dst, n * sizeof(T))) return -1;
+	std::vector<T> tmp(n);
+	for (int a = 0; a < n; a++) tmp[a] = (T)src[a];
+	HIPCHECK(hipMemcpyAsync(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+int ensureScratch(alis_dev_world *D, long elems)
+{
+	if (elems <= D->scratchElems) return 0;
+	if (D->scratch) {
+		HIPCHECK(hipStreamSynchronize(D->st));
+		HIPCHECK(hipFree(D->scratch));
+	}
+	HIPCHECK(hipMalloc(&D->scratch, elems * sizeof(double)));
+	D->scratchElems = elems;
+	return 0;
+}
+
+// Copy the host block of a box (float, k fastest) into a device array of layout L.
+template<class T>
+int uploadBox(alis_dev_world *D, void *dst, const Lay &L, const Box &b, const float *host)
+{
+	long n = b.count();
+	if (!n) return 0;
+	if (ensureScratch(D, n)) return -1;
+	HIPCHECK(hipMemcpyAsync(D->scratch, host, n * sizeof(float), hipMemcpyHostToDevice, D->st));
+	k_unpack<T, float><<<blocksFor(n), 256, 0, D->st>>>((const float *)D->scratch, L, b, (T *)dst);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+// Copy a box of a device array into a host float block (k fastest).
+template<class T>
+int downloadBox(alis_dev_world *D, const void *src, const Lay &L, const Box &b, float *host)
+{
+	long n = b.count();
+	if (!n) return 0;
+	if (ensureScratch(D, n)) return -1;
+	k_pack<T, float><<<blocksFor(n), 256, 0, D->st>>>((const T *)src, L, b, (float *)D->scratch);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(host, D->scratch, n * sizeof(float), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+// Box sync into a host float*** array with arbitrary extents: pack, download, scatter rows.
+template<class T>
+int downloadSubBox(alis_dev_world *D, const void *src, const Lay &L, const Box &b, float ***host)
+{
+	long n = b.count();
+	if (!n) return 0;
+	std::vector<float> buf(n);
+	if (downloadBox<T>(D, src, L, b, buf.data())) return -1;
+	long c = 0;
+	int nk = b.k1 - b.k0 + 1;
+	for (int i = b.i0; i <= b.i1; i++)
+		for (int j = b.j0; j <= b.j1; j++, c += nk)
+			memcpy(&host[i][j][b.k0], &buf[c], nk * sizeof(float));
+	return 0;
+}
+
+Lay fieldLayout(const alis_dev_world *D)
+{
+	const int koff = ROW_ALIGN_BYTES / (int)D->esz;
+	const int nkTot = koff + D->kNUM + 1;                      // offsets 0..koff+kNUM used
+	const int align = ROW_ALIGN_BYTES / (int)D->esz;
+	const long SK = ((nkTot + align - 1) / align) * align;
+	Lay L;
+	L.i0 = D->iMIN - 1, L.j0 = D->jMIN - 1, L.k0 = D->kMIN, L.koff = koff;
+	L.sj = SK;
+	L.si = (long)(D->jNUM + 2) * SK;
+	return L;
+}
+
+Box fieldBox(const alis_dev_world *D)
+{
+	return Box{D->iMIN - 1, D->iMAX + 1, D->jMIN - 1, D->jMAX + 1, D->kMIN - 1, D->kMAX + 1};
+}
+
+Lay compactLayout(const Box &b)
+{
+	Lay L;
+	L.i0 = b.i0, L.j0 = b.j0, L.k0 = b.k0, L.koff = 0;
+	L.sj = b.k1 - b.k0 + 1;
+	L.si = (long)(b.j1 - b.j0 + 1) * L.sj;
+	return L;
+}
+
+float *hostBlock(float ***f, const Box &b) { return &f[b.i0][b.j0][b.k0]; }
+
+// Ghost-copy schedule of the reference (src/update.c:284-297 for E, :311-339 for H) as a
+// list of copy operations in call order.
+struct CopySpec { int axis; float c, s; Box b; int di, dj, dk; };
+
+std::vector<CopySpec> ghostSchedule(const alis_dev_world_desc *W, int h)
+{
+	std::vector<CopySpec> v;
+	const int iAlt = (W->xMinSur==SUR_PBC || W->xMinSur==SUR_BBC) ? W->iNUM-1 : 1;
+	const int jAlt = (W->yMinSur==SUR_PBC || W->yMinSur==SUR_BBC) ? W->jNUM-1 : 1;
+	const int kAlt = (W->zMinSur==SUR_PBC || W->zMinSur==SUR_BBC) ? W->kNUM-1 : 1;
+	auto add = [&](int axis, float c, float s, int i0, int i1, int j0, int j1, int k0, int k1, int di, int dj, int dk) {
+		v.push_back(CopySpec{axis, c, s, Box{i0, i1, j0, j1, k0, k1}, di, dj, dk});
+	};
+	if (h == 0) {
+		add(0, W->coskx, -W->sinkx, W->iMAX+1, W->iMAX+1, W->jMIN, W->jMAX, W->kMIN, W->kMAX, -iAlt, 0, 0);
+		add(2, W->coskz, -W->sinkz, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMAX+1, W->kMAX+1, 0, 0, -kAlt);
+		if (W->yMinSur==SUR_HBC) {
+			add(1, W->coskym, -W->sinkym, W->iMIN, 0, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMAX, 1-W->jNUM, 0);
+			add(1, W->coskyp, -W->sinkyp, 1, W->iMAX+1, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMIN, 1-W->jNUM, 0);
+		}
+		else add(1, W->cosky, -W->sinky, W->iMIN, W->iMAX, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, 0, -jAlt, 0);
+		return v;
+	}
+	const float cxm = W->xMaxSur==SUR_PEC ? 1 : W->coskx, czm = W->zMaxSur==SUR_PEC ? 1 : W->coskz, cym = W->yMaxSur==SUR_PEC ? 1 : W->cosky;
+	add(1, cxm, -W->sinkx, W->iMAX+1, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX+1, -iAlt, 0, 0);
+	add(2, cxm, -W->sinkx, W->iMAX+1, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX+1, -iAlt, 0, 0);
+	add(0, czm, -W->sinkz, W->iMIN, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMAX+1, W->kMAX+1, 0, 0, -kAlt);
+	add(1, czm, -W->sinkz, W->iMIN, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMAX+1, W->kMAX+1, 0, 0, -kAlt);
+	add(1, W->coskx, W->sinkx, W->iMIN, W->iMIN, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX+1, iAlt, 0, 0);
+	add(2, W->coskx, W->sinkx, W->iMIN, W->iMIN, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX+1, iAlt, 0, 0);
+	add(0, W->coskz, W->sinkz, W->iMIN, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMIN, W->kMIN, 0, 0, kAlt);
+	add(1, W->coskz, W->sinkz, W->iMIN, W->iMAX+1, W->jMIN, W->jMAX+1, W->kMIN, W->kMIN, 0, 0, kAlt);
+	if (W->yMinSur==SUR_HBC) {
+		add(0, W->coskyp, W->sinkyp, W->iMIN, 0, W->jMIN, W->jMIN, W->kMIN, W->kMAX, W->iMAX, W->jNUM-1, 0);
+		add(2, W->coskyp, W->sinkyp, W->iMIN, 0, W->jMIN, W->jMIN, W->kMIN, W->kMAX, W->iMAX, W->jNUM-1, 0);
+		add(0, W->coskym, -W->sinkym, W->iMIN, 0, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMAX, 1-W->jNUM, 0);
+		add(2, W->coskym, -W->sinkym, W->iMIN, 0, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMAX, 1-W->jNUM, 0);
+		add(0, W->coskym, W->sinkym, 1, W->iMAX+1, W->jMIN, W->jMIN, W->kMIN, W->kMAX, W->iMIN, W->jNUM-1, 0);
+		add(2, W->coskym, W->sinkym, 1, W->iMAX+1, W->jMIN, W->jMIN, W->kMIN, W->kMAX, W->iMIN, W->jNUM-1, 0);
+		add(0, W->coskyp, -W->sinkyp, 1, W->iMAX+1, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMIN, 1-W->jNUM, 0);
+		add(2, W->coskyp, -W->sinkyp, 1, W->iMAX+1, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX, W->iMIN, 1-W->jNUM, 0);
+	}
+	else {
+		add(2, cym, -W->sinky, W->iMIN, W->iMAX+1, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX+1, 0, -jAlt, 0);
+		add(0, cym, -W->sinky, W->iMIN, W->iMAX+1, W->jMAX+1, W->jMAX+1, W->kMIN, W->kMAX+1, 0, -jAlt, 0);
+		add(2, W->cosky, W->sinky, W->iMIN, W->iMAX+1, W->jMIN, W->jMIN, W->kMIN, W->kMAX+1, 0, jAlt, 0);
+		add(0, W->cosky, W->sinky, W->iMIN, W->iMAX+1, W->jMIN, W->jMIN, W->kMIN, W->kMAX+1, 0, jAlt, 0);
+	}
+	return v;
+}
+
+bool boxesMeet(const Box &a, const Box &b)
+{
+	return !(a.i1 < b.i0 || b.i1 < a.i0 || a.j1 < b.j0 || b.j1 < a.j0 || a.k1 < b.k0 || b.k1 < a.k0);
+}
+
+Box shifted(const Box &a, int di, int dj, int dk)
+{
+	return Box{a.i0 + di, a.i1 + di, a.j0 + dj, a.j1 + dj, a.k0 + dk, a.k1 + dk};
+}
+
+// Two operations may run in the same launch when neither writes what the other reads or
+// writes (the launch has no ordering between its operations).  Strips and sources only
+// write their target F (their other arrays are read-only); copies read and write both the
+// real and imaginary arrays.
+template<class T>
+bool independent(const Op<T> &a, const Op<T> &b)
+{
+	if (a.kind != OP_COPY && b.kind != OP_COPY) return a.F != b.F || !boxesMeet(a.b, b.b);
+	bool shared = a.F == b.F || (a.G && (a.G == b.F || a.G == b.G)) || (b.G && b.G == a.F);
+	if (!shared) return true;
+	Box ra = a.kind == OP_COPY ? shifted(a.b, a.di, a.dj, a.dk) : a.b;
+	Box rb = b.kind == OP_COPY ? shifted(b.b, b.di, b.dj, b.dk) : b.b;
+	return !boxesMeet(a.b, b.b) && !boxesMeet(a.b, rb) && !boxesMeet(ra, b.b);
+}
+
+template<class T>
+std::vector<std::vector<Op<T>>> groupOps(const std::vector<Op<T>> &ops)
+{
+	std::vector<std::vector<Op<T>>> groups;
+	for (const Op<T> &o : ops) {
+		if (o.b.count() == 0) continue;
+		bool fits = !groups.empty() && (int)groups.back().size() < MAXOPS;
+		if (fits) for (const Op<T> &p : groups.back()) if (!independent(o, p)) { fits = false; break; }
+		if (fits) groups.back().push_back(o);
+		else groups.push_back({o});
+	}
+	return groups;
+}
+
+template<class T>
+int launchGroups(alis_dev_world *D, const std::vector<std::vector<Op<T>>> &groups)
+{
+	for (const auto &g : groups) {
+		OpGroup<T> G;
+		G.n = (int)g.size();
+		long most = 0;
+		for (int a = 0; a < G.n; a++) G.op[a] = g[a], most = std::max(most, g[a].b.count());
+		dim3 grid(blocksFor(most), G.n);
+		k_ops<T><<<grid, 256, 0, D->st>>>(G);
+		HIPCHECK(hipGetLastError());
+	}
+	return 0;
+}
+
+template<class T>
+std::vector<std::vector<Op<T>>> &ghostProgram(alis_dev_world *D, int h);
+template<> std::vector<std::vector<Op<float>>> &ghostProgram<float>(alis_dev_world *D, int h) { return D->ghostF[h]; }
+template<> std::vector<std::vector<Op<double>>> &ghostProgram<double>(alis_dev_world *D, int h) { return D->ghostD[h]; }
+
+template<class T>
+void buildGhosts(alis_dev_world *D)
+{
+	for (int h = 0; h < 2; h++) {
+		std::vector<Op<T>> ops;
+		for (const CopySpec &c : ghostSchedule(&D->W, h)) {
+			Op<T> o{};
+			o.kind = OP_COPY;
+			o.F = (T *)D->F[h][0][c.axis];
+			o.G = D->cplx ? (T *)D->F[h][1][c.axis] : nullptr;
+			o.LF = D->L;
+			o.b = c.b;
+			o.di = c.di, o.dj = c.dj, o.dk = c.dk;
+			o.c = (T)c.c, o.s = (T)c.s;
+			ops.push_back(o);
+		}
+		ghostProgram<T>(D, h) = groupOps(ops);
+	}
+}
+
+// Commit: allocate and upload everything of the descriptor.
+template<class T>
+int commitWorld(alis_dev_world *D)
+{
+	const alis_dev_world_desc *W = &D->W;
+	D->L = fieldLayout(D);
+	D->fsize = (long)(D->iNUM + 2) * D->L.si;
+	const Box fb = fieldBox(D);
+
+	for (int h = 0; h < 2; h++)
+		for (int m = 0; m <= D->cplx; m++)
+			for (int a = 0; a < 3; a++) {
+				if (!W->field[h][m][a]) return fail("field array missing in the descriptor");
+				if (devAlloc(D, &D->F[h][m][a], D->fsize * sizeof(T))) return -1;
+				if (uploadBox<T>(D, D->F[h][m][a], D->L, fb, hostBlock(W->field[h][m][a], fb))) return -1;
+			}
+
+	// inverse permittivity (reference extents: one extra node along the component axis)
+	if (W->eps[0] && W->eps[1] && W->eps[2]) {
+		for (int a = 0; a < 3; a++) {
+			Box b{D->iMIN, D->iMAX + (a==0), D->jMIN, D->jMAX + (a==1), D->kMIN, D->kMAX + (a==2)};
+			if (devAlloc(D, &D->C[a], D->fsize * sizeof(T))) return -1;
+			if (uploadBox<T>(D, D->C[a], D->L, b, hostBlock(W->eps[a], b))) return -1;
+		}
+	}
+
+	// per-axis step ratios and CPML coefficients
+	const int NUMs[3] = { D->iNUM, D->jNUM, D->kNUM };
+	for (int h = 0; h < 2; h++)
+		for (int a = 0; a < 3; a++) {
+			if (upload1<T>(D, &D->dt[h][a], W->dt[h][a], NUMs[a])) return -1;
+			if (W->cpml[h][a][0])
+				for (int c = 0; c < 3; c++) if (upload1<T>(D, &D->cp[h][a][c], W->cpml[h][a][c], NUMs[a])) return -1;
+		}
+
+	// CPML slabs: ranges per face for E (Alt=+1, Offset 0) and H (Alt=-1, Offset 1)
+	const int MINa[3] = { D->iMIN, D->jMIN, D->kMIN }, MAXa[3] = { D->iMAX, D->jMAX, D->kMAX };
+	const int Mina[3] = { D->iMin, D->jMin, D->kMin }, Maxa[3] = { D->iMax, D->jMax, D->kMax };
+	for (int h = 0; h < 2; h++) {
+		const int alt = h ? -1 : 1;
+		for (int f = 0; f < 6; f++) {
+			const int ax = f % 3;
+			DevSlab &S = D->slab[h][f];
+			const bool minFace = f < 3;
+			S.on = minFace ? MINa[ax] < Mina[ax] : MAXa[ax] > Maxa[ax];
+			if (!S.on) continue;
+			if (!D->cp[h][ax][0]) return fail("CPML face without a CPML profile");
+			S.lo = minFace ? MINa[ax] - (alt - 1) / 2 : Maxa[ax] + 2;
+			S.hi = minFace ? Mina[ax] - (alt + 3) / 2 : MAXa[ax];
+			Box b{D->iMIN, D->iMAX, D->jMIN, D->jMAX, D->kMIN, D->kMAX};
+			if (ax == 0) b.i0 = S.lo, b.i1 = S.hi;
+			if (ax == 1) b.j0 = S.lo, b.j1 = S.hi;
+			if (ax == 2) b.k0 = S.lo, b.k1 = S.hi;
+			S.b = b;
+			if (ax == 2) S.L = compactLayout(b);
+			else {
+				S.L = D->L;
+				S.L.i0 = b.i0, S.L.j0 = b.j0;
+				S.L.si = (long)(b.j1 - b.j0 + 1) * D->L.sj;
+			}
+			const long n = (long)(b.i1 - b.i0 + 1) * S.L.si;
+			for (int m = 0; m <= D->cplx; m++)
+				for (int s2 = 0; s2 < 2; s2++) {
+					float ***hostp = W->slab[h][m][f][s2];
+					if (!hostp) return fail("CPML slab missing on the host");
+					if (devAlloc(D, &S.p[m][s2], n * sizeof(T))) return -1;
+					if (uploadBox<T>(D, S.p[m][s2], S.L, b, hostBlock(hostp, b))) return -1;
+				}
+		}
+	}
+
+	// dispersive objects
+	for (int n = 0; n < W->nobj; n++) {
+		const alis_dev_ade &A = W->obj[n];
+		DevObj O;
+		O.box = Box{A.i0, A.i1, A.j0, A.j1, A.k0, A.k1};
+		O.clip = Box{std::max(D->iMIN, A.i0), std::min(D->iMAX, A.i1),
+		             std::max(D->jMIN, A.j0), std::min(D->jMAX, A.j1),
+		             std::max(D->kMIN, A.k0), std::min(D->kMAX, A.k1)};
+		O.LO = compactLayout(O.box);
+		O.NJ = A.nj, O.NK = A.nk;
+		if (O.NK > MAXPOLE || O.NJ > O.NK) return fail("bad pole counts");
+		for (int c = 0; c < 3; c++) {
+			O.a[c] = A.a[c];
+			for (int p = 0; p < O.NK; p++) {
+				O.JJ[c][p] = A.JJ[c][p], O.JE[c][p] = A.JE[c][p];
+				if (p >= O.NJ) O.JK[c][p] = A.JK[c][p], O.KJ[c][p] = A.KJ[c][p], O.KK[c][p] = A.KK[c][p], O.KE[c][p] = A.KE[c][p];
+			}
+		}
+		const long n3 = O.box.count();
+		for (int c = 0; c < 3; c++) {
+			if (devAlloc(D, &O.fill[c], n3 * sizeof(T))) return -1;
+			if (uploadBox<T>(D, O.fill[c], O.LO, O.box, hostBlock(A.fill[c], O.box))) return -1;
+		}
+		for (int m = 0; m <= D->cplx; m++)
+			for (int c = 0; c < 3; c++)
+				for (int p = 0; p < O.NK; p++) {
+					if (devAlloc(D, &O.J[m][c][p], n3 * sizeof(T))) return -1;
+					if (uploadBox<T>(D, O.J[m][c][p], O.LO, O.box, hostBlock(A.J[m][c][p], O.box))) return -1;
+					if (p >= O.NJ) {
+						if (devAlloc(D, &O.K[m][c][p], n3 * sizeof(T))) return -1;
+						if (uploadBox<T>(D, O.K[m][c][p], O.LO, O.box, hostBlock(A.K[m][c][p], O.box))) return -1;
+					}
+				}
+		D->obj.push_back(O);
+	}
+
+	// sources
+	for (int h = 0; h < 2; h++)
+		for (int q = 0; q < W->nsrc[h]; q++) {
+			const alis_dev_src &S = W->src[h][q];
+			DevSrc s;
+			s.h = h, s.m = S.m, s.axis = S.axis;
+			if (S.h != h || S.m < 0 || S.m > D->cplx || S.axis < 0 || S.axis > 2) return fail("source targets an array outside this world");
+			s.b = Box{S.i0, S.i1, S.j0, S.j1, S.k0, S.k1};
+			const Box fbx = fieldBox(D);
+			if (s.b.i0 < fbx.i0 || s.b.i1 > fbx.i1 || s.b.j0 < fbx.j0 || s.b.j1 > fbx.j1 || s.b.k0 < fbx.k0 || s.b.k1 > fbx.k1)
+				return fail("source box outside the grid");
+			s.LB = compactLayout(s.b);
+			s.ncell = s.b.count();
+			s.amp = S.amplitude;
+			if (S.Amplitude) {
+				if (devAlloc(D, &s.ampArr, s.ncell * sizeof(T))) return -1;
+				if (uploadBox<T>(D, s.ampArr, s.LB, s.b, hostBlock(S.Amplitude, s.b))) return -1;
+			}
+			if (S.cellPhase) {
+				s.cellWave = true;
+				if (devAlloc(D, &s.waveDev, s.ncell * sizeof(T))) return -1;
+				HIPCHECK(hipHostMalloc((void **)&s.wavePinned, s.ncell * sizeof(T), hipHostMallocDefault));
+				HIPCHECK(hipEventCreateWithFlags(&s.waveDone, hipEventDisableTiming));
+				HIPCHECK(hipEventRecord(s.waveDone, D->st));
+			}
+			D->src[h].push_back(s);
+		}
+
+	buildGhosts<T>(D);
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+template<class T>
+CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
+{
+	CurlArgs<T> a{};
+	a.F0 = (T *)D->F[h][m][0], a.F1 = (T *)D->F[h][m][1], a.F2 = (T *)D->F[h][m][2];
+	a.G0 = (const T *)D->F[1-h][m][0], a.G1 = (const T *)D->F[1-h][m][1], a.G2 = (const T *)D->F[1-h][m][2];
+	if (h == 0) a.C0 = (const T *)D->C[0], a.C1 = (const T *)D->C[1], a.C2 = (const T *)D->C[2];
+	a.dtx = (const T *)D->dt[h][0], a.dty = (const T *)D->dt[h][1], a.dtz = (const T *)D->dt[h][2];
+	a.L = D->L;
+	a.iMIN = D->iMIN, a.jMIN = D->jMIN, a.kMIN = D->kMIN;
+	a.ni = D->iNUM, a.nj = D->jNUM, a.nk = D->kNUM;
+	a.alt = h ? -1 : 1;
+	for (int f = 0; f < 6; f++) {
+		const DevSlab &S = D->slab[h][f];
+		CurlFace<T> &P = a.face[f];
+		P.on = S.on;
+		if (!S.on) continue;
+		const int ax = f % 3;
+		P.pa = (T *)S.p[m][0], P.pb = (T *)S.p[m][1];
+		P.b = (const T *)D->cp[h][ax][0], P.ca = (const T *)D->cp[h][ax][1], P.cb = (const T *)D->cp[h][ax][2];
+		P.L = S.L, P.lo = S.lo, P.hi = S.hi;
+	}
+	return a;
+}
+
+template<class T>
+AdeArgs<T> adeArgs(alis_dev_world *D, const DevObj &O, int m, int mode)
+{
+	AdeArgs<T> a{};
+	for (int c = 0; c < 3; c++) {
+		a.F[c] = (T *)D->F[0][m][c];
+		a.fill[c] = (const T *)O.fill[c];
+		a.a[c] = (T)O.a[c];
+		for (int p = 0; p < O.NK; p++) {
+			a.J[c][p] = (T *)O.J[m][c][p];
+			a.K[c][p] = (T *)O.K[m][c][p];
+			a.JJ[c][p] = (T)O.JJ[c][p], a.JK[c][p] = (T)O.JK[c][p], a.JE[c][p] = (T)O.JE[c][p];
+			a.KJ[c][p] = (T)O.KJ[c][p], a.KK[c][p] = (T)O.KK[c][p], a.KE[c][p] = (T)O.KE[c][p];
+		}
+	}
+	a.L = D->L, a.LO = O.LO;
+	a.i0 = O.clip.i0, a.j0 = O.clip.j0, a.k0 = O.clip.k0;
+	a.ni = O.clip.i1 - O.clip.i0 + 1, a.nj = O.clip.j1 - O.clip.j0 + 1, a.nk = O.clip.k1 - O.clip.k0 + 1;
+	a.NJ = O.NJ, a.NK = O.NK, a.mode = mode;
+	return a;
+}
+
+template<class T>
+int launchAde(alis_dev_world *D, int mode)
+{
+	for (int m = 0; m <= D->cplx; m++)
+		for (const DevObj &O : D->obj) {
+			if (mode == 0 && !(O.a[0] || O.a[1] || O.a[2])) continue;
+			long n = O.clip.count();
+			if (!n) continue;
+			k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, mode));
+			HIPCHECK(hipGetLastError());
+		}
+	return 0;
+}
+
+template<class T>
+int coreHalf(alis_dev_world *D, int h)
+{
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (D->timing) {
+		HIPCHECK(hipEventCreate(&e0));
+		HIPCHECK(hipEventCreate(&e1));
+	}
+	if (h == 0) {
+		// reference order: all conductivity scalings, then all -J subtractions, per part m
+		for (int m = 0; m <= D->cplx; m++) {
+			for (const DevObj &O : D->obj) {
+				if (!(O.a[0] || O.a[1] || O.a[2])) continue;
+				long n = O.clip.count();
+				if (n) k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, 0));
+			}
+			for (const DevObj &O : D->obj) {
+				long n = O.clip.count();
+				if (n) k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, 1));
+			}
+		}
+	}
+	dim3 grid((D->kNUM + 63) / 64, (D->jNUM + 3) / 4, D->iNUM), block(64, 4);
+	if (e0) HIPCHECK(hipEventRecord(e0, D->st));
+	for (int m = 0; m <= D->cplx; m++) {
+		CurlArgs<T> a = curlArgs<T>(D, h, m);
+		if (h == 0 && D->C[0]) k_curl<T, true><<<grid, block, 0, D->st>>>(a);
+		else k_curl<T, false><<<grid, block, 0, D->st>>>(a);
+	}
+	if (e1) {
+		HIPCHECK(hipEventRecord(e1, D->st));
+		D->evPairs[h].push_back(e0);
+		D->evPairs[h].push_back(e1);
+	}
+	HIPCHECK(hipGetLastError());
+	if (h == 1) return launchAde<T>(D, 2);
+	return 0;
+}
+
+// TF/SF strips of box s (src/update.c:205-272): F += dt/ds * G_aux * (1/eps) on the 12 face
+// strips, with auxiliary indices beyond its last row aliased as in the reference.
+template<class T>
+int tfsfHalf(alis_dev_world *D, int s, int h)
+{
+	if (s >= (int)D->tf.size() || !D->tf[s].aux) return fail("TF/SF box not linked");
+	const alis_dev_world_desc *W = &D->W;
+	const alis_dev_tfsf *S = &D->tf[s].S;
+	alis_dev_world *A = D->tf[s].aux;
+	const alis_dev_world_desc *AW = &A->W;
+	static const int altY[6] = { 0, 1, 0, 1, 0, 1 }, altX[6] = { 1, 0, 0, 0, 1, 1 };
+	std::vector<Op<T>> ops;
+	const float *dtx = W->dt[h][0] - W->iMIN, *dty = W->dt[h][1] - W->jMIN, *dtz = W->dt[h][2] - W->kMIN;
+	auto strip = [&](int m, int fa, int ga, float dtds, int i0, int i1, int j0, int j1, int k0, int k1, int di, int dj, int dk) {
+		Op<T> o{};
+		o.kind = OP_STRIP;
+		o.F = (T *)D->F[h][m][fa];
+		o.G = (T *)A->F[1-h][m][ga];
+		o.C = h == 0 ? (const T *)D->C[fa] : nullptr;
+		o.LF = D->L, o.LG = A->L, o.LC = D->L;
+		o.b = Box{i0, i1, j0, j1, k0, k1};
+		o.di = di, o.dj = dj, o.dk = dk;
+		o.c = (T)dtds;
+		const int comp = (1-h) * 3 + ga;   // aux component index E.x..H.z
+		o.exX = AW->expandX, o.exY = AW->expandY;
+		o.auxIMAX = AW->iMAX, o.auxJMAX = AW->jMAX;
+		o.altX = altX[comp], o.altY = altY[comp];
+		ops.push_back(o);
+	};
+	for (int m = 0; m <= W->complexField; m++) {
+		if (m > AW->complexField) return fail("complex main world with a real auxiliary world");
+		if (h == 0) {
+			if (S->iMin > W->iMIN) {
+				strip(m, 2, 1, -dtx[S->iMin], S->iMin, S->iMin, S->jMin, S->jMax, S->kMIN, S->kMAX, 0, 0, 0);
+				strip(m, 1, 2, +dtx[S->iMin], S->iMin, S->iMin, S->jMIN, S->jMAX, S->kMin, S->kMax, 0, 0, 0);
+			}
+			if (S->jMin > W->jMIN) {
+				strip(m, 0, 2, -dty[S->jMin], S->iMIN, S->iMAX, S->jMin, S->jMin, S->kMin, S->kMax, 0, 0, 0);
+				strip(m, 2, 0, +dty[S->jMin], S->iMin, S->iMax, S->jMin, S->jMin, S->kMIN, S->kMAX, 0, 0, 0);
+			}
+			if (S->kMin > W->kMIN) {
+				strip(m, 1, 0, -dtz[S->kMin], S->iMin, S->iMax, S->jMIN, S->jMAX, S->kMin, S->kMin, 0, 0, 0);
+				strip(m, 0, 1, +dtz[S->kMin], S->iMIN, S->iMAX, S->jMin, S->jMax, S->kMin, S->kMin, 0, 0, 0);
+			}
+			if (S->iMax < W->iMAX) {
+				strip(m, 2, 1, +dtx[S->iMax], S->iMax, S->iMax, S->jMin, S->jMax, S->kMIN, S->kMAX, 1, 0, 0);
+				strip(m, 1, 2, -dtx[S->iMax], S->iMax, S->iMax, S->jMIN, S->jMAX, S->kMin, S->kMax, 1, 0, 0);
+			}
+			if (S->jMax < W->jMAX) {
+				strip(m, 0, 2, +dty[S->jMax], S->iMIN, S->iMAX, S->jMax, S->jMax, S->kMin, S->kMax, 0, 1, 0);
+				strip(m, 2, 0, -dty[S->jMax], S->iMin, S->iMax, S->jMax, S->jMax, S->kMIN, S->kMAX, 0, 1, 0);
+			}
+			if (S->kMax < W->kMAX) {
+				strip(m, 1, 0, +dtz[S->kMax], S->iMin, S->iMax, S->jMIN, S->jMAX, S->kMax, S->kMax, 0, 0, 1);
+				strip(m, 0, 1, -dtz[S->kMax], S->iMIN, S->iMAX, S->jMin, S->jMax, S->kMax, S->kMax, 0, 0, 1);
+			}
+		}
+		else {
+			if (S->iMin > W->iMIN) {
+				strip(m, 2, 1, +dtx[S->iMin], S->iMin, S->iMin, S->jMIN, S->jMAX, S->kMin, S->kMax, 0, 0, 0);
+				strip(m, 1, 2, -dtx[S->iMin], S->iMin, S->iMin, S->jMin, S->jMax, S->kMIN, S->kMAX, 0, 0, 0);
+			}
+			if (S->jMin > W->jMIN) {
+				strip(m, 0, 2, +dty[S->jMin], S->iMin, S->iMax, S->jMin, S->jMin, S->kMIN, S->kMAX, 0, 0, 0);
+				strip(m, 2, 0, -dty[S->jMin], S->iMIN, S->iMAX, S->jMin, S->jMin, S->kMin, S->kMax, 0, 0, 0);
+			}
+			if (S->kMin > W->kMIN) {
+				strip(m, 1, 0, +dtz[S->kMin], S->iMIN, S->iMAX, S->jMin, S->jMax, S->kMin, S->kMin, 0, 0, 0);
+				strip(m, 0, 1, -dtz[S->kMin], S->iMin, S->iMax, S->jMIN, S->jMAX, S->kMin, S->kMin, 0, 0, 0);
+			}
+			if (S->iMax < W->iMAX) {
+				strip(m, 2, 1, -dtx[S->iMax+1], S->iMax+1, S->iMax+1, S->jMIN, S->jMAX, S->kMin, S->kMax, -1, 0, 0);
+				strip(m, 1, 2, +dtx[S->iMax+1], S->iMax+1, S->iMax+1, S->jMin, S->jMax, S->kMIN, S->kMAX, -1, 0, 0);
+			}
+			if (S->jMax < W->jMAX) {
+				strip(m, 0, 2, -dty[S->jMax+1], S->iMin, S->iMax, S->jMax+1, S->jMax+1, S->kMIN, S->kMAX, 0, -1, 0);
+				strip(m, 2, 0, +dty[S->jMax+1], S->iMIN, S->iMAX, S->jMax+1, S->jMax+1, S->kMin, S->kMax, 0, -1, 0);
+			}
+			if (S->kMax < W->kMAX) {
+				strip(m, 1, 0, -dtz[S->kMax+1], S->iMIN, S->iMAX, S->jMin, S->jMax, S->kMax+1, S->kMax+1, 0, 0, -1);
+				strip(m, 0, 1, +dtz[S->kMax+1], S->iMin, S->iMax, S->jMIN, S->jMAX, S->kMax+1, S->kMax+1, 0, 0, -1);
+			}
+		}
+	}
+	// every strip must stay inside both grids
+	const Box fb = fieldBox(D);
+	for (const Op<T> &o : ops) {
+		if (o.b.count() == 0) continue;
+		if (o.b.i0 < fb.i0 || o.b.i1 > fb.i1 || o.b.j0 < fb.j0 || o.b.j1 > fb.j1 || o.b.k0 < fb.k0 || o.b.k1 > fb.k1)
+			return fail("TF/SF strip outside the grid");
+		const int ia0 = o.b.i0 + o.di, ja0 = o.b.j0 + o.dj, ka0 = o.b.k0 + o.dk, ka1 = o.b.k1 + o.dk;
+		const int ia1 = AW->expandX ? std::min(o.b.i1 + o.di, AW->iMAX + 1) : o.b.i1 + o.di;
+		const int ja1 = AW->expandY ? std::min(o.b.j1 + o.dj, AW->jMAX + 1) : o.b.j1 + o.dj;
+		if (ia0 < AW->iMIN - 1 || ia1 > AW->iMAX + 1 || ja0 < AW->jMIN - 1 || ja1 > AW->jMAX + 1 || ka0 < AW->kMIN - 1 || ka1 > AW->kMAX + 1)
+			return fail("TF/SF box reaches outside its auxiliary grid");
+	}
+	return launchGroups<T>(D, groupOps(ops));
+}
+
+template<class T>
+int injectHalf(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell)
+{
+	if (index >= (int)D->src[h].size()) return fail("source index out of range");
+	DevSrc &s = D->src[h][index];
+	Op<T> o{};
+	o.kind = OP_SOURCE;
+	o.F = (T *)D->F[h][s.m][s.axis];
+	o.LF = D->L;
+	o.C = (const T *)s.ampArr;
+	o.LC = s.LB;
+	o.b = s.b;
+	o.c = (T)s.amp;
+	o.s = (T)wave;
+	if (cellWave) {
+		if (!s.cellWave || ncell != s.ncell) return fail("per-cell wave does not match the source box");
+		HIPCHECK(hipEventSynchronize(s.waveDone));
+		T *pin = (T *)s.wavePinned;
+		for (long c = 0; c < ncell; c++) pin[c] = (T)cellWave[c];
+		HIPCHECK(hipMemcpyAsync(s.waveDev, pin, ncell * sizeof(T), hipMemcpyHostToDevice, D->st));
+		HIPCHECK(hipEventRecord(s.waveDone, D->st));
+		o.W = (const T *)s.waveDev;
+	}
+	std::vector<std::vector<Op<T>>> g{{o}};
+	return launchGroups<T>(D, g);
+}
+
+template<class T>
+int syncHost(alis_dev_world *D, Box box)
+{
+	const alis_dev_world_desc *W = &D->W;
+	const bool whole = box.i0 > box.i1;
+	const Box fb = fieldBox(D);
+	for (int h = 0; h < 2; h++)
+		for (int m = 0; m <= D->cplx; m++)
+			for (int a = 0; a < 3; a++) {
+				float ***hf = W->field[h][m][a];
+				Box b{std::max(box.i0, fb.i0), std::min(box.i1, fb.i1), std::max(box.j0, fb.j0), std::min(box.j1, fb.j1),
+				      std::max(box.k0, fb.k0), std::min(box.k1, fb.k1)};
+				// whole arrays go straight into the host block (also right for aliased rows)
+				if (whole || W->expandX || W->expandY) {
+					if (downloadBox<T>(D, D->F[h][m][a], D->L, fb, hostBlock(hf, fb))) return -1;
+				}
+				else if (b.count() && downloadSubBox<T>(D, D->F[h][m][a], D->L, b, hf)) return -1;
+			}
+	for (size_t n = 0; n < D->obj.size(); n++) {
+		const DevObj &O = D->obj[n];
+		const alis_dev_ade &A = W->obj[n];
+		if (!whole && !boxesMeet(O.box, box)) continue;
+		for (int m = 0; m <= D->cplx; m++)
+			for (int c = 0; c < 3; c++)
+				for (int p = 0; p < O.NK; p++) {
+					if (downloadBox<T>(D, O.J[m][c][p], O.LO, O.box, hostBlock(A.J[m][c][p], O.box))) return -1;
+					if (whole && p >= O.NJ && downloadBox<T>(D, O.K[m][c][p], O.LO, O.box, hostBlock(A.K[m][c][p], O.box))) return -1;
+				}
+	}
+	if (!whole) return 0;
+	for (int h = 0; h < 2; h++)
+		for (int f = 0; f < 6; f++) {
+			const DevSlab &S = D->slab[h][f];
+			if (!S.on) continue;
+			for (int m = 0; m <= D->cplx; m++)
+				for (int s2 = 0; s2 < 2; s2++)
+					if (downloadBox<T>(D, S.p[m][s2], S.L, S.b, hostBlock(W->slab[h][m][f][s2], S.b))) return -1;
+		}
+	return 0;
+}
+
+template<class T>
+int readField(alis_dev_world *D, int h, int m, int axis, double *out)
+{
+	const Box fb = fieldBox(D);
+	long n = fb.count();
+	if (ensureScratch(D, n)) return -1;
+	k_pack<T, double><<<blocksFor(n), 256, 0, D->st>>>((const T *)D->F[h][m][axis], D->L, fb, (double *)D->scratch);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, D->scratch, n * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+template<class T>
+int writeField(alis_dev_world *D, int h, int m, int axis, const double *in)
+{
+	const Box fb = fieldBox(D);
+	long n = fb.count();
+	if (ensureScratch(D, n)) return -1;
+	HIPCHECK(hipMemcpyAsync(D->scratch, in, n * sizeof(double), hipMemcpyHostToDevice, D->st));
+	k_unpack<T, double><<<blocksFor(n), 256, 0, D->st>>>((const double *)D->scratch, D->L, fb, (T *)D->F[h][m][axis]);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+// ---- phasers ----
+
+template<class T, class TA>
+int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH)
+{
+	PhsArgs<T> a{};
+	for (int c = 0; c < 3; c++) a.E[c] = (const T *)D->F[0][0][c], a.H[c] = (const T *)D->F[1][0][c];
+	a.L = D->L;
+	a.iMin = P.iMin, a.jMin = P.jMin, a.kMin = P.kMin, a.nface = P.nface;
+	for (int f = 0; f < P.nface; f++) a.face[f] = P.face[f];
+	a.ncell = P.ncell;
+	a.acc = P.acc;
+	for (int q0 = 0; q0 < qn; q0 += MAXF) {
+		a.q0 = qa + q0;
+		a.nq = std::min(MAXF, qn - q0);
+		for (int q = 0; q < a.nq; q++) {
+			a.eE[q][0] = expE[2 * (q0 + q)], a.eE[q][1] = expE[2 * (q0 + q) + 1];
+			a.eH[q][0] = expH[2 * (q0 + q)], a.eH[q][1] = expH[2 * (q0 + q) + 1];
+		}
+		k_phaser<T, TA><<<blocksFor(P.ncell), 256, 0, D->st>>>(a);
+		HIPCHECK(hipGetLastError());
+	}
+	return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char *alis_dev_error(void) { return g_error.c_str(); }
+
+int alis_dev_commit(const alis_dev_world_desc *W, int fp64, int dft64, alis_dev_world **out)
+{
+	*out = nullptr;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no GPU is visible to the engine");
+	alis_dev_world *D = new alis_dev_world;
+	D->W = *W;
+	D->objDesc.assign(W->obj, W->obj + W->nobj);
+	D->W.obj = D->objDesc.data();
+	for (int h = 0; h < 2; h++) {
+		D->srcDesc[h].assign(W->src[h], W->src[h] + W->nsrc[h]);
+		D->W.src[h] = D->srcDesc[h].data();
+	}
+	D->tfDesc.assign(W->tfsf, W->tfsf + W->ntfsf);
+	D->W.tfsf = D->tfDesc.data();
+	D->fp64 = fp64 ? 1 : 0, D->dft64 = dft64 ? 1 : 0;
+	D->cplx = W->complexField ? 1 : 0;
+	D->esz = D->fp64 ? 8 : 4;
+	D->iMIN = W->iMIN, D->iMAX = W->iMAX, D->jMIN = W->jMIN, D->jMAX = W->jMAX, D->kMIN = W->kMIN, D->kMAX = W->kMAX;
+	D->iMin = W->iMin, D->iMax = W->iMax, D->jMin = W->jMin, D->jMax = W->jMax, D->kMin = W->kMin, D->kMax = W->kMax;
+	D->iNUM = W->iNUM, D->jNUM = W->jNUM, D->kNUM = W->kNUM;
+	if (hipStreamCreateWithFlags(&D->st, hipStreamNonBlocking) != hipSuccess) {
+		delete D;
+		return fail("hipStreamCreate failed");
+	}
+	int rc = D->fp64 ? commitWorld<double>(D) : commitWorld<float>(D);
+	if (rc) {
+		std::string keep = g_error;
+		alis_dev_release(D);
+		g_error = keep;
+		return rc;
+	}
+	for (int s = 0; s < W->ntfsf; s++) {
+		DevTfsf t;
+		t.S = W->tfsf[s];
+		D->tf.push_back(t);
+	}
+	*out = D;
+	return 0;
+}
+
+int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
+{
+	if (s < 0 || s >= (int)D->tf.size()) return fail("TF/SF index out of range");
+	if (aux->fp64 != D->fp64) return fail("TF/SF auxiliary world precision differs");
+	D->tf[s].aux = aux;
+	return 0;
+}
+
+void alis_dev_release(alis_dev_world *D)
+{
+	if (!D) return;
+	if (D->st) hipStreamSynchronize(D->st);
+	for (void *p : D->allocs) hipFree(p);
+	for (int h = 0; h < 2; h++)
+		for (DevSrc &s : D->src[h]) {
+			if (s.wavePinned) hipHostFree(s.wavePinned);
+			if (s.waveDone) hipEventDestroy(s.waveDone);
+		}
+	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
+	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
+	if (D->scratch) hipFree(D->scratch);
+	if (D->st) hipStreamDestroy(D->st);
+	delete D;
+}
+
+int alis_dev_core(alis_dev_world *D, int h)
+{
+	return D->fp64 ? coreHalf<double>(D, h) : coreHalf<float>(D, h);
+}
+
+int alis_dev_strips(alis_dev_world *D, int s, int h)
+{
+	return D->fp64 ? tfsfHalf<double>(D, s, h) : tfsfHalf<float>(D, s, h);
+}
+
+int alis_dev_inject(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell)
+{
+	return D->fp64 ? injectHalf<double>(D, h, index, wave, cellWave, ncell) : injectHalf<float>(D, h, index, wave, cellWave, ncell);
+}
+
+int alis_dev_ghosts(alis_dev_world *D, int h)
+{
+	return D->fp64 ? launchGroups<double>(D, D->ghostD[h]) : launchGroups<float>(D, D->ghostF[h]);
+}
+
+int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1)
+{
+	Box b{i0, i1, j0, j1, k0, k1};
+	return D->fp64 ? syncHost<double>(D, b) : syncHost<float>(D, b);
+}
+
+int alis_dev_read_field(alis_dev_world *D, int h, int m, int axis, double *out)
+{
+	if (m > D->cplx || h < 0 || h > 1 || axis < 0 || axis > 2) return fail("no such field array");
+	return D->fp64 ? readField<double>(D, h, m, axis, out) : readField<float>(D, h, m, axis, out);
+}
+
+int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double *in)
+{
+	if (m > D->cplx || h < 0 || h > 1 || axis < 0 || axis > 2) return fail("no such field array");
+	return D->fp64 ? writeField<double>(D, h, m, axis, in) : writeField<float>(D, h, m, axis, in);
+}
+
+int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *id)
+{
+	DevPhaser Q;
+	Q.nf = P->nf;
+	Q.iMin = P->iMin, Q.iMax = P->iMax, Q.jMin = P->jMin, Q.jMax = P->jMax, Q.kMin = P->kMin, Q.kMax = P->kMax;
+	Q.cf.assign(P->cf, P->cf + (size_t)Q.nf * 24);
+	const int ni = Q.iMax - Q.iMin + 1, nj = Q.jMax - Q.jMin + 1, nk = Q.kMax - Q.kMin + 1;
+	const int fixed[6] = { Q.iMin, Q.iMax, Q.jMin, Q.jMax, Q.kMin, Q.kMax };
+	const int lo[3] = { D->iMIN, D->jMIN, D->kMIN }, hi[3] = { D->iMAX, D->jMAX, D->kMAX };
+	if (Q.iMin < lo[0] || Q.iMax > hi[0] || Q.jMin < lo[1] || Q.jMax > hi[1] || Q.kMin < lo[2] || Q.kMax > hi[2] || ni < 1 || nj < 1 || nk < 1)
+		return fail("phaser box outside the grid");
+	long off = 0;
+	for (int kind = 0; kind < 6; kind++) {
+		if (!Q.cf[kind * 4]) continue;
+		PhsFaceGeom g;
+		g.kind = kind;
+		g.fixed = fixed[kind];
+		long cells = kind < 2 ? (long)nj * nk : kind < 4 ? (long)ni * nk : (long)ni * nj;
+		g.n2 = kind < 4 ? nk : nj;
+		g.off = off;
+		off += cells;
+		Q.face[Q.nface++] = g;
+	}
+	Q.ncell = off;
+	const long n = (long)Q.nf * 4 * Q.ncell * 2;
+	const size_t accBytes = (size_t)std::max(n, 1L) * (D->dft64 ? 8 : 4);
+	HIPCHECK(hipMalloc(&Q.acc, accBytes));
+	HIPCHECK(hipMalloc((void **)&Q.stage, (size_t)std::max(n, 1L) * sizeof(float)));
+	// start from the host accumulators (normally zero)
+	std::vector<float> host(std::max(n, 1L));
+	for (int q = 0; q < Q.nf; q++)
+		for (int f = 0; f < Q.nface; f++) {
+			const long cells = (f + 1 < Q.nface ? Q.face[f + 1].off : Q.ncell) - Q.face[f].off;
+			for (int comp = 0; comp < 4; comp++)
+				memcpy(&host[2 * (((size_t)q * 4 + comp) * Q.ncell + Q.face[f].off)], Q.cf[((size_t)q * 6 + Q.face[f].kind) * 4 + comp], cells * 2 * sizeof(float));
+		}
+	if (D->dft64) {
+		std::vector<double> hd(host.begin(), host.end());
+		HIPCHECK(hipMemcpyAsync(Q.acc, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, D->st));
+		HIPCHECK(hipStreamSynchronize(D->st));
+	}
+	else {
+		HIPCHECK(hipMemcpyAsync(Q.acc, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, D->st));
+		HIPCHECK(hipStreamSynchronize(D->st));
+	}
+	Q.live = true;
+	D->phs.push_back(Q);
+	*id = (int)D->phs.size() - 1;
+	return 0;
+}
+
+int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH)
+{
+	if (id < 0 || id >= (int)D->phs.size() || !D->phs[id].live) return fail("no such phaser");
+	DevPhaser &P = D->phs[id];
+	if (q0 < 0 || nq < 0 || q0 + nq > P.nf) return fail("phaser frequency range out of bounds");
+	if (!P.ncell || !nq) return 0;
+	if (D->fp64) return D->dft64 ? phaserUpdate<double, double>(D, P, q0, nq, expE, expH) : phaserUpdate<double, float>(D, P, q0, nq, expE, expH);
+	return D->dft64 ? phaserUpdate<float, double>(D, P, q0, nq, expE, expH) : phaserUpdate<float, float>(D, P, q0, nq, expE, expH);
+}
+
+long alis_dev_phaser_size(alis_dev_world *D, int id)
+{
+	if (id < 0 || id >= (int)D->phs.size() || !D->phs[id].live) return -1;
+	return (long)D->phs[id].nf * 4 * D->phs[id].ncell * 2;
+}
+
+int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
+{
+	long need = alis_dev_phaser_size(D, id);
+	if (need < 0 || n < need) return fail("phaser read buffer too small");
+	DevPhaser &P = D->phs[id];
+	if (!need) return 0;
+	if (ensureScratch(D, need)) return -1;
+	if (D->dft64) k_acc_to_double<double><<<blocksFor(need), 256, 0, D->st>>>((const double *)P.acc, (double *)D->scratch, need);
+	else k_acc_to_double<float><<<blocksFor(need), 256, 0, D->st>>>((const float *)P.acc, (double *)D->scratch, need);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, D->scratch, need * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+int alis_dev_phaser_sync(alis_dev_world *D, int id)
+{
+	long need = alis_dev_phaser_size(D, id);
+	if (need < 0) return fail("no such phaser");
+	DevPhaser &Q = D->phs[id];
+	if (!need) return 0;
+	if (D->dft64) k_acc_to_float<double><<<blocksFor(need), 256, 0, D->st>>>((const double *)Q.acc, Q.stage, need);
+	else HIPCHECK(hipMemcpyAsync(Q.stage, Q.acc, need * sizeof(float), hipMemcpyDeviceToDevice, D->st));
+	HIPCHECK(hipGetLastError());
+	std::vector<float> host(need);
+	HIPCHECK(hipMemcpyAsync(host.data(), Q.stage, need * sizeof(float), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	for (int q = 0; q < Q.nf; q++)
+		for (int f = 0; f < Q.nface; f++) {
+			const long cells = (f + 1 < Q.nface ? Q.face[f + 1].off : Q.ncell) - Q.face[f].off;
+			for (int comp = 0; comp < 4; comp++)
+				memcpy(Q.cf[((size_t)q * 6 + Q.face[f].kind) * 4 + comp], &host[2 * (((size_t)q * 4 + comp) * Q.ncell + Q.face[f].off)], cells * 2 * sizeof(float));
+		}
+	return 0;
+}
+
+int alis_dev_phaser_remove(alis_dev_world *D, int id)
+{
+	if (id < 0 || id >= (int)D->phs.size() || !D->phs[id].live) return fail("no such phaser");
+	DevPhaser &P = D->phs[id];
+	HIPCHECK(hipStreamSynchronize(D->st));
+	HIPCHECK(hipFree(P.acc));
+	HIPCHECK(hipFree(P.stage));
+	P.live = false;
+	return 0;
+}
+
+int alis_dev_synchronize(alis_dev_world *D)
+{
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+int alis_dev_timing(alis_dev_world *D, int enable)
+{
+	HIPCHECK(hipStreamSynchronize(D->st));
+	for (int h = 0; h < 2; h++) {
+		for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
+		D->evPairs[h].clear();
+		D->evMs[h] = 0, D->evCount[h] = 0;
+	}
+	D->timing = enable != 0;
+	return 0;
+}
+
+int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches)
+{
+	HIPCHECK(hipStreamSynchronize(D->st));
+	auto &v = D->evPairs[h];
+	for (size_t a = 0; a + 1 < v.size(); a += 2) {
+		float t = 0;
+		HIPCHECK(hipEventElapsedTime(&t, v[a], v[a + 1]));
+		D->evMs[h] += t;
+		D->evCount[h]++;
+		hipEventDestroy(v[a]);
+		hipEventDestroy(v[a + 1]);
+	}
+	v.clear();
+	*ms = D->evMs[h];
+	*launches = D->evCount[h];
+	return 0;
+}
+
+long alis_dev_bytes(alis_dev_world *D) { return D->bytes; }
+
+}  // extern "C"

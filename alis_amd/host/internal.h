@@ -1,0 +1,46 @@
+/*
+ * Internal declarations shared by the host library sources (not installed).
+ */
+#ifndef ALIS_INTERNAL_H
+#define ALIS_INTERNAL_H
+
+#define ALIS_LIBRARY
+#include "alis.h"
+#include "alis_dev.h"
+
+/* grid.c */
+float ***makeFieldFilled(int iMin, int iMax, int jMin, int jMax, int kMin, int kMax, float v);
+void resetField3(float ***f, int iMin, int iMax, int jMin, int jMax, int kMin, int kMax);
+void makeAuxFieldArrays(world W, vfield *F, coeffs *C);
+
+/* media.c */
+int dispersiveMatter(matter M);   /* 2: nonlinear tensor, 1: poles, 0: plain dielectric */
+float complex polePermittivity(matter M, float w, int p);
+
+/* engine.c: device residency of a world */
+int alisPrecisionDefault(int *fieldBits, int *dftBits);
+void alisCommit(world W);          /* create the device world if absent                */
+void alisInvalidate(world W);      /* setup changed: bring state home, drop the device */
+void alisDropDevice(world W);      /* host arrays are newer: drop the device copy      */
+void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1);
+void alisSyncPhaser(phaser P);
+void alisSyncAll(void);            /* every live world and phaser (timer() hook)        */
+void alisRegisterWorld(world W);
+void alisForgetWorld(world W);
+void alisRegisterPhaser(phaser P);
+void alisForgetPhaser(phaser P);
+void alisFatal(const char *what, int rc);
+phaser phaserBase(phaser P);     /* first element of the array holding P, or 0 */
+
+/* describe.c */
+void describeWorld(world W, alis_dev_world_desc *d);
+void freeWorldDesc(alis_dev_world_desc *d);
+void describePhaser(phaser B, alis_dev_phaser_desc *d);
+
+/* sources.c */
+void sourceHalfStep(world W, source S, float t, int h);  /* evaluate and inject a chain */
+
+/* fields.c: field identity helpers used by sources and probes */
+int fieldComponent(field *F, int *isH, int *m, int *axis);
+
+#endif

@@ -1,0 +1,367 @@
+/*
+ * Phasers: surface running DFTs on the faces of a box (src/phaser.c:21-295).  The
+ * accumulators live on the device; flux integrals read them in double precision and sum
+ * with Neumaier compensation.
+ */
+#include <string.h>
+#include "internal.h"
+#include "phaserview.h"
+
+#define NEWFACE(P,S,E1,E2,H1,H2,i0,i1,j0,j1,k0,k1) do { \
+	(P).S.E1 = makeComplexField(i0, i1, j0, j1, k0, k1); \
+	(P).S.E2 = makeComplexField(i0, i1, j0, j1, k0, k1); \
+	(P).S.H1 = makeComplexField(i0, i1, j0, j1, k0, k1); \
+	(P).S.H2 = makeComplexField(i0, i1, j0, j1, k0, k1); } while (0)
+
+/* Source used for spectral normalisation: the one whose duration set W->N, looked up in W
+ * and then in the first TF/SF auxiliary world (src/phaser.c:25-26). */
+source normalisingSource(world W)
+{
+	source S = W->SE && W->N==W->SE->N ? W->SE : W->SH && W->N==W->SH->N ? W->SH : 0;
+	for (world w = W; !S && w->S && w->S->W; S = w->SE && W->N==w->SE->N ? w->SE : w->SH && W->N==w->SH->N ? w->SH : 0) w = W->S->W;
+	return S;
+}
+
+/* Phasers at wavelengths LMin..LMax (frequencies equally spaced in 1/L by dL) on the faces
+ * of a box; the array ends with an f = 0 entry.  A is the inverse source spectrum at f
+ * (per unit wavelength in 3-D), so accumulated fields are normalised to a unit source
+ * (src/phaser.c:21-66). */
+phaser (createPhasers)(world W, float LMin, float LMax, float dL, float xMin, float xMax, float yMin, float yMax, float zMin, float zMax, ...)
+{
+	int nMin = floor(LMin/dL), nMax = ceil(LMax/dL);
+	source S = normalisingSource(W);
+	phaser P;
+	MALLOC(P, 0, 2-nMin+nMax);
+
+	for (int n = 0; n <= nMax-nMin; n++) {
+		P[n].W = W;
+		P[n].N = nMin * nMax * dL / W->dt;
+		P[n].f = (nMax-n) / (dL * nMin * nMax);
+		P[n].A = 2.0 / (P->W->N ? P->W->N : P->N);
+		P[n].iMin = MAX(W->iMin, W->xtoi(W, xMin)), P[n].iMax = MIN(W->iMax, W->xtoi(W, xMax));
+		P[n].jMin = MAX(W->jMin, W->ytoj(W, yMin)), P[n].jMax = MIN(W->jMax, W->ytoj(W, yMax));
+		P[n].kMin = MAX(W->kMin, W->ztok(W, zMin)), P[n].kMax = MIN(W->kMax, W->ztok(W, zMax));
+		P[n].devId = -1;
+
+		if (S && W->N) {
+			float complex c = 0;
+			float complex iwdt = 2 * PI * I * P[n].f * W->dt;
+			for (int s = 1; s < W->N; s++) c += S->waveForm(W, S, W->dt*s, S->phase) * cexpf(iwdt*s);
+			P[n].A = 1 / cabsf(c);
+			if (W->sourceType == 2) P[n].A /= sqrtf(P[n].f * S->wavelength);
+			if (W->sourceType == 3) P[n].A /= P[n].f * S->wavelength;
+		}
+		if ((xMax-xMin < INF && xMin < W->xMin && W->xMax < xMax) && (W->xMaxSur==SYM || W->xMaxSur==PBC || W->xMaxSur==BBC)) {
+			P[n].xNUM = floor((xMax - xMin) / (W->xMax - W->xMin) + 0.5);
+			P[n].expikx = (W->xMaxSur==BBC) ? (W->coskx+I*W->sinkx) : 1;
+		}
+		if ((yMax-yMin < INF && yMin < W->yMin && W->yMax < yMax) && (W->yMaxSur==SYM || W->yMaxSur==PBC || W->yMaxSur==BBC)) {
+			P[n].yNUM = floor((yMax - yMin) / (W->yMax - W->yMin) + 0.5);
+			P[n].expiky = (W->yMaxSur==BBC) ? (W->cosky+I*W->sinky) : 1;
+		}
+		phaser Q = &P[n];
+		if (W->xMinSur==PML && Q->iMin != Q->iMax) NEWFACE(*Q, xMin, Ey, Ez, Hy, Hz, Q->iMin, Q->iMin, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
+		if (W->yMinSur==PML && Q->jMin != Q->jMax) NEWFACE(*Q, yMin, Ez, Ex, Hz, Hx, Q->iMin, Q->iMax, Q->jMin, Q->jMin, Q->kMin, Q->kMax);
+		if (W->zMinSur==PML && Q->kMin != Q->kMax) NEWFACE(*Q, zMin, Ex, Ey, Hx, Hy, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMin);
+		if (W->xMaxSur==PML) NEWFACE(*Q, xMax, Ey, Ez, Hy, Hz, Q->iMax, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
+		if (W->yMaxSur==PML) NEWFACE(*Q, yMax, Ez, Ex, Hz, Hx, Q->iMin, Q->iMax, Q->jMax, Q->jMax, Q->kMin, Q->kMax);
+		if (W->zMaxSur==PML) NEWFACE(*Q, zMax, Ex, Ey, Hx, Hy, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMax, Q->kMax);
+	}
+	P[0].count = nMax - nMin + 1;
+	P[nMax-nMin+1].devId = -1;
+	alisRegisterPhaser(P);
+	return P;
+}
+
+phaser (createPhaser)(world W, float L, float xMin, float xMax, float yMin, float yMax, float zMin, float zMax, ...)
+{
+	return (createPhasers)(W, L, L, L, xMin, xMax, yMin, yMax, zMin, zMax);
+}
+
+/* Phaser of the array nearest to `wavelength` in frequency (src/phaser.c:75-80). */
+phaser choosePhaser(phaser P, float wavelength)
+{
+	do P++;
+	while (1/wavelength < P->f);
+	return (1/wavelength - P->f < (P-1)->f - 1/wavelength) ? P : P-1;
+}
+
+/* Accumulate E at W->E->t and the face-averaged H at W->H->t into P and every later
+ * phaser of its array (src/phaser.c:83-110).  The phase factors are computed here: in the
+ * reference's float arithmetic for FP32 accumulators, in double (with the double pi) for
+ * FP64 accumulators. */
+void updatePhaser(world W, phaser P)
+{
+	phaser B = phaserBase(P);
+	if (!B) printf("updatePhaser: unknown phaser\n"), exit(1);
+	if (B->W != W) printf("updatePhaser: the phaser belongs to another world\n"), exit(1);
+	alisCommit(W);
+	if (B->devId < 0) {
+		alis_dev_phaser_desc d;
+		describePhaser(B, &d);
+		int rc = alis_dev_phaser_add(W->dev, &d, &B->devId);
+		free(d.cf);
+		alisFatal("phaser registration", rc);
+	}
+	int q0 = (int)(P - B), nq = B->count - q0;
+	static double *eE, *eH;
+	static int cap;
+	if (nq > cap) cap = nq, eE = realloc(eE, 2*cap*sizeof(double)), eH = realloc(eH, 2*cap*sizeof(double));
+	for (int q = 0; q < nq; q++) {
+		if (W->devDftPrec) {
+			double aE = 2 * M_PI * (double)P[q].f * (double)W->E->t, aH = 2 * M_PI * (double)P[q].f * (double)W->H->t;
+			eE[2*q] = cos(aE), eE[2*q+1] = sin(aE);
+			eH[2*q] = cos(aH), eH[2*q+1] = sin(aH);
+		}
+		else {
+			float complex e = cexpf(2 * PI * I * P[q].f * W->E->t);
+			float complex h = cexpf(2 * PI * I * P[q].f * W->H->t);
+			eE[2*q] = crealf(e), eE[2*q+1] = cimagf(e);
+			eH[2*q] = crealf(h), eH[2*q+1] = cimagf(h);
+		}
+	}
+	alisFatal("phaser update", alis_dev_phaser_update(W->dev, B->devId, q0, nq, eE, eH));
+	B->hostStale = 1;
+	B->viewValid = 0;
+}
+
+void deletePhaser(phaser P)
+{
+	phaser B = P;
+	if (B->devId >= 0 && B->W->dev) alis_dev_phaser_remove(B->W->dev, B->devId);
+	alisForgetPhaser(B);
+	free(B->view);
+	for (; P->f; P++) {
+		cfield *F[6] = { &P->xMin, &P->xMax, &P->yMin, &P->yMax, &P->zMin, &P->zMax };
+		for (int a = 0; a < 6; a++) {
+			if (F[a]->Ex) removeField(F[a]->Ex, P->iMin, a == 3 ? P->jMax : P->jMin, a == 5 ? P->kMax : P->kMin);
+			if (F[a]->Ey) removeField(F[a]->Ey, a == 1 ? P->iMax : P->iMin, a == 3 ? P->jMax : P->jMin, a == 5 ? P->kMax : P->kMin);
+			if (F[a]->Ez) removeField(F[a]->Ez, a == 1 ? P->iMax : P->iMin, a == 3 ? P->jMax : P->jMin, P->kMin);
+			if (F[a]->Hx) removeField(F[a]->Hx, P->iMin, a == 3 ? P->jMax : P->jMin, a == 5 ? P->kMax : P->kMin);
+			if (F[a]->Hy) removeField(F[a]->Hy, a == 1 ? P->iMax : P->iMin, a == 3 ? P->jMax : P->jMin, a == 5 ? P->kMax : P->kMin);
+			if (F[a]->Hz) removeField(F[a]->Hz, a == 1 ? P->iMax : P->iMin, a == 3 ? P->jMax : P->jMin, P->kMin);
+		}
+		if (P->dE) free(&P->dE[P->kMIN]), free(&P->dH[P->kMIN]), free(&P->ex[P->kMIN]), free(&P->ez[P->kMIN]);
+	}
+	free(B);
+}
+
+
+/* ---- double-precision view of the accumulators ---- */
+
+static cfield *faceOf(phaser P, int kind)
+{
+	cfield *F[6] = { &P->xMin, &P->xMax, &P->yMin, &P->yMax, &P->zMin, &P->zMax };
+	return F[kind];
+}
+
+void phaserFaces(phaser B, phaserGeom *g)
+{
+	int ni = B->iMax - B->iMin + 1, nj = B->jMax - B->jMin + 1, nk = B->kMax - B->kMin + 1;
+	long off = 0;
+	for (int kind = 0; kind < 6; kind++) {
+		cfield *F = faceOf(B, kind);
+		g->on[kind] = F->Ex || F->Ey || F->Ez;
+		g->off[kind] = off;
+		g->n2[kind] = kind < 4 ? nk : nj;
+		if (g->on[kind]) off += kind < 2 ? (long)nj * nk : kind < 4 ? (long)ni * nk : (long)ni * nj;
+	}
+	g->ncell = off;
+}
+
+/* Accumulators of the whole array of B as doubles in the device read layout. */
+const double *phaserDoubles(phaser B, phaserGeom *g)
+{
+	phaserFaces(B, g);
+	long need = (long)B->count * 4 * g->ncell * 2;
+	if (B->viewValid && B->view) return B->view;
+	if (!B->view) B->view = calloc(need ? need : 1, sizeof(double));
+	if (B->devId >= 0 && B->W->dev) {
+		alisFatal("phaser read", alis_dev_phaser_read(B->W->dev, B->devId, B->view, need));
+	}
+	else {
+		for (int q = 0; q < B->count; q++) {
+			phaser P = B + q;
+			for (int kind = 0; kind < 6; kind++) {
+				if (!g->on[kind]) continue;
+				cfield *F = faceOf(P, kind);
+				float complex ***c[4];
+				if (kind < 2) c[0] = F->Ey, c[1] = F->Ez, c[2] = F->Hy, c[3] = F->Hz;
+				else if (kind < 4) c[0] = F->Ez, c[1] = F->Ex, c[2] = F->Hz, c[3] = F->Hx;
+				else c[0] = F->Ex, c[1] = F->Ey, c[2] = F->Hx, c[3] = F->Hy;
+				int i = kind == 1 ? P->iMax : P->iMin, j = kind == 3 ? P->jMax : P->jMin, k = kind == 5 ? P->kMax : P->kMin;
+				long cells = kind < 2 ? (long)(P->jMax-P->jMin+1)*(P->kMax-P->kMin+1) : kind < 4 ? (long)(P->iMax-P->iMin+1)*(P->kMax-P->kMin+1) : (long)(P->iMax-P->iMin+1)*(P->jMax-P->jMin+1);
+				for (int comp = 0; comp < 4; comp++) {
+					float complex *src = &c[comp][i][j][k];
+					double *dst = B->view + 2 * (((long)q * 4 + comp) * g->ncell + g->off[kind]);
+					for (long a = 0; a < cells; a++) dst[2*a] = crealf(src[a]), dst[2*a+1] = cimagf(src[a]);
+				}
+			}
+		}
+	}
+	B->viewValid = 1;
+	return B->view;
+}
+
+/* Component comp (E1 E2 H1 H2) of face `kind` at face indices (a, b) of frequency q. */
+double complex faceValue(const double *v, const phaserGeom *g, int q, int kind, int comp, int a, int b)
+{
+	const double *p = v + 2 * (((long)q * 4 + comp) * g->ncell + g->off[kind] + (long)a * g->n2[kind] + b);
+	return p[0] + I * p[1];
+}
+
+void neumaierAdd(double *sum, double *comp, double x)
+{
+	double t = *sum + x;
+	if (fabs(*sum) >= fabs(x)) *comp += (*sum - t) + x;
+	else *comp += (x - t) + *sum;
+	*sum = t;
+}
+
+
+/* Outward flux Re(E x H*) through one face (src/phaser.c:142-223): trapezoid over the
+ * face's E nodes weighted by the E and H cell widths. */
+static double faceFlux(phaser P, const double *v, const phaserGeom *g, int q, int kind)
+{
+	if (!g->on[kind]) return 0;
+	world W = P->W;
+	double s = 0, c = 0;
+	int ni = P->iMax, nj = P->jMax, nk = P->kMax;
+	if (kind < 2) {          /* x face: E1 = Ey, E2 = Ez, H1 = Hy, H2 = Hz */
+		for (int j = P->jMin+1; j <= nj; j++) {
+			double r = 0;
+			for (int k = P->kMin; k <= nk; k++) {
+				double w = (k == P->kMin || k == nk) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
+				r += w * creal(faceValue(v, g, q, kind, 0, j-P->jMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 3, j-P->jMin, k-P->kMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dy[j] * r);
+		}
+		for (int k = P->kMin+1; k <= nk; k++) {
+			double r = 0;
+			for (int j = P->jMin; j <= nj; j++) {
+				double w = (j == P->jMin || j == nj) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
+				r -= w * creal(faceValue(v, g, q, kind, 1, j-P->jMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 2, j-P->jMin, k-P->kMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dz[k] * r);
+		}
+	}
+	else if (kind < 4) {     /* y face: E1 = Ez, E2 = Ex, H1 = Hz, H2 = Hx */
+		for (int k = P->kMin+1; k <= nk; k++) {
+			double r = 0;
+			for (int i = P->iMin; i <= ni; i++) {
+				double w = (i == P->iMin || i == ni) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
+				r += w * creal(faceValue(v, g, q, kind, 0, i-P->iMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 3, i-P->iMin, k-P->kMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dz[k] * r);
+		}
+		for (int i = P->iMin+1; i <= ni; i++) {
+			double r = 0;
+			for (int k = P->kMin; k <= nk; k++) {
+				double w = (k == P->kMin || k == nk) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
+				r -= w * creal(faceValue(v, g, q, kind, 1, i-P->iMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 2, i-P->iMin, k-P->kMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dx[i] * r);
+		}
+	}
+	else {                   /* z face: E1 = Ex, E2 = Ey, H1 = Hx, H2 = Hy */
+		for (int i = P->iMin+1; i <= ni; i++) {
+			double r = 0;
+			for (int j = P->jMin; j <= nj; j++) {
+				double w = (j == P->jMin || j == nj) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
+				r += w * creal(faceValue(v, g, q, kind, 0, i-P->iMin, j-P->jMin) * conj(faceValue(v, g, q, kind, 3, i-P->iMin, j-P->jMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dx[i] * r);
+		}
+		for (int j = P->jMin+1; j <= nj; j++) {
+			double r = 0;
+			for (int i = P->iMin; i <= ni; i++) {
+				double w = (i == P->iMin || i == ni) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
+				r -= w * creal(faceValue(v, g, q, kind, 1, i-P->iMin, j-P->jMin) * conj(faceValue(v, g, q, kind, 2, i-P->iMin, j-P->jMin)));
+			}
+			neumaierAdd(&s, &c, W->CH->dy[j] * r);
+		}
+	}
+	return s + c;
+}
+
+/* Net outward flux of phaser P normalised by A^2; 2-D worlds divide by the unit length of
+ * the flat axis, symmetry planes double the result (src/phaser.c:228-245). */
+double phaserFluxD(phaser P)
+{
+	phaser B = phaserBase(P);
+	if (!B) return 0;
+	phaserGeom g;
+	const double *v = phaserDoubles(B, &g);
+	int q = (int)(P - B);
+	world W = P->W;
+	double s = 0, c = 0;
+	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 1));
+	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 0));
+	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 3));
+	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 2));
+	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 5));
+	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 4));
+	double Sum = s + c;
+	if (W->Dom.x.Min==W->Dom.x.Max) Sum /= W->CH->dx[P->iMax]+W->CH->dx[P->iMax-1];
+	if (W->Dom.y.Min==W->Dom.y.Max) Sum /= W->CH->dy[P->jMax]+W->CH->dy[P->jMax-1];
+	if (W->Dom.z.Min==W->Dom.z.Max) Sum /= W->CH->dz[P->kMax]+W->CH->dz[P->kMax-1];
+	if (W->xMinSur==SYM) Sum *= 2;
+	if (W->yMinSur==SYM) Sum *= 2;
+	if (W->zMinSur==SYM) Sum *= 2;
+	return Sum * (double)P->A * (double)P->A;
+}
+
+float phaserPoyntingOut(phaser P) { return phaserFluxD(P); }
+float phaserPoynting(phaser P) { return fabsf(phaserPoyntingOut(P)); }
+float phaserPoyntingIn(phaser P) { return -phaserPoyntingOut(P); }
+
+
+/* Text spectra over the frequencies of the first array: wavelength, then one column per
+ * phaser array (src/phaser.c:248-295). */
+void (writePoyntingSpectrum)(world W, char *format, ...)
+{
+	char str[4096], filename[1024];
+	va_list ap;
+	int M = 0;
+	va_start(ap, format);
+	vsnprintf(filename, sizeof filename, format, ap);
+	for (phaser p = va_arg(ap, phaser); p; M++) p = va_arg(ap, phaser);
+	va_end(ap);
+	phaser *P;
+	MALLOC(P, 0, M ? M : 1);
+	va_start(ap, format);
+	vsnprintf(filename, sizeof filename, format, ap);
+	for (int m = 0; m < M; m++) P[m] = va_arg(ap, phaser);
+	va_end(ap);
+	for (int n = 0; M && P[0][n].f; n++) {
+		int s = sprintf(str, "%g", 1/P[0][n].f);
+		for (int m = 0; m < M; m++) s += sprintf(str+s, "\t%g", phaserPoynting(P[m]+n));
+		sprintf(str+s, "\n");
+		(writeTxt)(W, filename, "%s", str, "");
+	}
+	free(P);
+}
+
+void (writeFarFieldSpectrum)(world W, tfield pol(phaser,float,float), float theta, float phi, char *format, ...)
+{
+	char str[4096], filename[1024];
+	va_list ap;
+	int M = 0;
+	va_start(ap, format);
+	vsnprintf(filename, sizeof filename, format, ap);
+	for (phaser p = va_arg(ap, phaser); p; M++) p = va_arg(ap, phaser);
+	va_end(ap);
+	phaser *P;
+	MALLOC(P, 0, M ? M : 1);
+	va_start(ap, format);
+	vsnprintf(filename, sizeof filename, format, ap);
+	for (int m = 0; m < M; m++) P[m] = va_arg(ap, phaser);
+	va_end(ap);
+	for (int n = 0; M && P[0][n].f; n++) {
+		int s = sprintf(str, "%g", 1/P[0][n].f);
+		for (int m = 0; m < M; m++) s += sprintf(str+s, "\t%g", farFieldI(P[m]+n, pol, theta, phi));
+		sprintf(str+s, "\n");
+		(writeTxt)(W, filename, "%s", str, "");
+	}
+	free(P);
+}

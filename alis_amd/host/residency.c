@@ -1,0 +1,182 @@
+/*
+ * Device residency of worlds and phasers.  A world is committed to the GPU at its first
+ * time step (after setup); setup calls made later bring the state home and drop the device
+ * copy, which is rebuilt at the next step.  Host arrays of the reference layout are kept as
+ * mirrors and refreshed on demand: a small box for get(), everything for the host
+ * observers, and everything when timer() reports the end of a loop.
+ */
+#include <string.h>
+#include "internal.h"
+
+static world *liveWorlds;
+static int nWorlds, capWorlds;
+static phaser *livePhasers;
+static int nPhasers, capPhasers;
+
+void alisFatal(const char *what, int rc)
+{
+	if (!rc) return;
+	fprintf(stderr, "alis: %s failed: %s\n", what, alis_dev_error());
+	exit(1);
+}
+
+int alisPrecisionDefault(int *fieldBits, int *dftBits)
+{
+	const char *p = getenv("ALIS_PRECISION");
+	*fieldBits = 0, *dftBits = 1;
+	if (p && !strcmp(p, "fp64")) *fieldBits = 1, *dftBits = 1;
+	else if (p && !strcmp(p, "ref")) *fieldBits = 0, *dftBits = 0;
+	return 0;
+}
+
+static void setPrecisionTree(world W, int f64, int d64)
+{
+	W->devPrec = f64, W->devDftPrec = d64;
+	for (tfsf S = W->S; S; S = S->S) setPrecisionTree(S->W, f64, d64);
+}
+
+void worldPrecision(world W, int fieldBits, int dftBits)
+{
+	alisInvalidate(W);
+	setPrecisionTree(W, fieldBits == 64, dftBits == 64);
+}
+
+void alisRegisterWorld(world W)
+{
+	if (nWorlds == capWorlds) capWorlds = capWorlds ? 2*capWorlds : 16, liveWorlds = realloc(liveWorlds, capWorlds*sizeof(world));
+	liveWorlds[nWorlds++] = W;
+}
+
+void alisForgetWorld(world W)
+{
+	if (W->dev) {
+		alis_dev_release(W->dev);
+		W->dev = 0;
+	}
+	for (int a = 0; a < nWorlds; a++) if (liveWorlds[a] == W) { liveWorlds[a] = liveWorlds[--nWorlds]; break; }
+	for (int a = 0; a < nPhasers; a++) if (livePhasers[a]->W == W) livePhasers[a]->devId = -1;
+}
+
+void alisRegisterPhaser(phaser P)
+{
+	if (nPhasers == capPhasers) capPhasers = capPhasers ? 2*capPhasers : 16, livePhasers = realloc(livePhasers, capPhasers*sizeof(phaser));
+	livePhasers[nPhasers++] = P;
+}
+
+void alisForgetPhaser(phaser P)
+{
+	for (int a = 0; a < nPhasers; a++) if (livePhasers[a] == P) { livePhasers[a] = livePhasers[--nPhasers]; break; }
+}
+
+/* First element of the phaser array that contains P (arrays end with an f == 0 entry). */
+phaser phaserBase(phaser P)
+{
+	for (int a = 0; a < nPhasers; a++) {
+		phaser B = livePhasers[a];
+		if (P >= B && P < B + B->count) return B;
+	}
+	return 0;
+}
+
+static void commitTree(world W)
+{
+	if (W->dev) return;
+	for (tfsf S = W->S; S; S = S->S) commitTree(S->W);
+	alis_dev_world *D;
+	alis_dev_world_desc desc;
+	describeWorld(W, &desc);
+	int rc = alis_dev_commit(&desc, W->devPrec, W->devDftPrec, &D);
+	freeWorldDesc(&desc);
+	alisFatal("commit", rc);
+	int s = 0;
+	for (tfsf S = W->S; S; S = S->S, s++) alisFatal("TF/SF link", alis_dev_link_tfsf(D, s, S->W->dev));
+	W->dev = D;
+	W->hostStale = 0;
+}
+
+void alisCommit(world W)
+{
+	if (W->dev) return;
+	if (W->isAux && W->parent) {
+		alisCommit(W->parent);
+		return;
+	}
+	commitTree(W);
+}
+
+static void syncTree(world W)
+{
+	if (W->dev && W->hostStale) {
+		alisFatal("sync", alis_dev_sync(W->dev, 0, -1, 0, -1, 0, -1));
+		W->hostStale = 0;
+	}
+	for (tfsf S = W->S; S; S = S->S) syncTree(S->W);
+}
+
+void alisSyncPhaser(phaser P);
+
+/* Bring fields, CPML and ADE state of W (and its TF/SF auxiliary worlds) and the
+ * accumulators of phasers recording W to the host arrays. */
+void syncWorld(world W)
+{
+	syncTree(W);
+	for (int a = 0; a < nPhasers; a++) if (livePhasers[a]->W == W) alisSyncPhaser(livePhasers[a]);
+}
+
+void alisSyncPhaser(phaser P)
+{
+	phaser B = phaserBase(P);
+	if (!B || B->devId < 0 || !B->hostStale || !B->W->dev) return;
+	alisFatal("phaser sync", alis_dev_phaser_sync(B->W->dev, B->devId));
+	B->hostStale = 0;
+}
+
+static void releaseTree(world W)
+{
+	for (tfsf S = W->S; S; S = S->S) releaseTree(S->W);
+	if (W->dev) {
+		alis_dev_release(W->dev);
+		W->dev = 0;
+	}
+	W->hostStale = 0;
+}
+
+void alisInvalidate(world W)
+{
+	if (W->isAux && W->parent && W->parent->dev) {
+		alisInvalidate(W->parent);
+		return;
+	}
+	if (!W->dev) return;
+	for (int a = 0; a < nPhasers; a++) {
+		phaser B = livePhasers[a];
+		if (B->W != W) continue;
+		alisSyncPhaser(B);
+		B->devId = -1;
+	}
+	syncTree(W);
+	releaseTree(W);
+}
+
+/* Host arrays were modified after a sync: drop the device copy without reading it back. */
+void alisDropDevice(world W)
+{
+	if (W->isAux && W->parent && W->parent->dev) W = W->parent;
+	for (int a = 0; a < nPhasers; a++) {
+		phaser B = livePhasers[a];
+		if (B->W == W && B->devId >= 0) alisSyncPhaser(B), B->devId = -1;
+	}
+	releaseTree(W);
+}
+
+void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1)
+{
+	if (!W->dev || !W->hostStale) return;
+	alisFatal("box sync", alis_dev_sync(W->dev, i0, i1, j0, j1, k0, k1));
+}
+
+void alisSyncAll(void)
+{
+	for (int a = 0; a < nWorlds; a++) if (!liveWorlds[a]->isAux) syncWorld(liveWorlds[a]);
+	for (int a = 0; a < nPhasers; a++) alisSyncPhaser(livePhasers[a]);
+}

@@ -1,0 +1,144 @@
+/*
+ * Time stepping: updateE / updateH drive the device half-step programs in the reference's
+ * order (src/update.c:275-341), the loop timer, and the host-array forms of copyBoundary
+ * and the nonlinear couplings.
+ */
+#include <string.h>
+#include <sys/time.h>
+#include "internal.h"
+
+/* Progress line "[elapsed/estimated] [n-1/N: %]" at most once a second; returns n <= N
+ * (src/update.c:14-45).  When the loop ends, device state is copied to the host arrays so
+ * that code after the loop may read them directly. */
+int (timer)(long int n, long int N, char *str, ...)
+{
+	static long int initialStep, previousStep;
+	static unsigned long long int initialTime, previousTime;
+	unsigned long long int currentTime, elapsedTime, estimatedTime;
+	struct timeval now;
+	gettimeofday(&now, 0);
+	currentTime = now.tv_sec * 1000000LL + now.tv_usec;
+
+	if (!n || !initialStep) {
+		initialStep = previousStep = n;
+		initialTime = previousTime = currentTime;
+	}
+	else if (n > N || currentTime/1000000LL > previousTime/1000000LL) {
+		elapsedTime = currentTime - initialTime;
+		estimatedTime = elapsedTime + (N-n+1) * (currentTime-previousTime) / (n-previousStep);
+		if (estimatedTime - elapsedTime > 3600000000LL) estimatedTime -= estimatedTime%60000000LL;
+		previousStep = n;
+		previousTime = currentTime;
+		if (str[0]) printf("\r%s ", str);
+		else printf("\r");
+		printf("[%02lld:%02lld:%02lld/%02lld:%02lld:%02lld] [%ld/%ld: %.1f%%]",
+			elapsedTime/3600000000LL, elapsedTime%3600000000LL/60000000LL, elapsedTime%60000000LL/1000000LL,
+			estimatedTime/3600000000LL, estimatedTime%3600000000LL/60000000LL, estimatedTime%60000000LL/1000000LL,
+			(n-1), N, 100.0*(n-1)/N);
+		fflush(stdout);
+		if (n > N) printf("\n");
+	}
+	if (n > N) alisSyncAll();
+	return n <= N;
+}
+
+
+/* E half step: dispersive currents and curl of H with CPML on the device, then the TF/SF
+ * auxiliary worlds and their strips, the E sources at the H time, the ghost layers, and the
+ * clock (src/update.c:275-299). */
+void updateE(world W)
+{
+	alisCommit(W);
+	alisFatal("E core", alis_dev_core(W->dev, 0));
+	int s = 0;
+	for (tfsf S = W->S; S; S = S->S, s++) {
+		updateE(S->W);
+		alisFatal("E strips", alis_dev_strips(W->dev, s, 0));
+	}
+	if (W->SE) sourceHalfStep(W, W->SE, W->H->t, 0);
+	alisFatal("E ghosts", alis_dev_ghosts(W->dev, 0));
+	W->t = W->E->t = ++W->n * W->dt;
+	W->hostStale = 1;
+}
+
+/* H half step: curl of E with CPML and the ADE current update on the new E, TF/SF,
+ * H sources at the E time, ghost layers (src/update.c:302-341). */
+void updateH(world W)
+{
+	alisCommit(W);
+	alisFatal("H core", alis_dev_core(W->dev, 1));
+	int s = 0;
+	for (tfsf S = W->S; S; S = S->S, s++) {
+		updateH(S->W);
+		alisFatal("H strips", alis_dev_strips(W->dev, s, 1));
+	}
+	if (W->SH) sourceHalfStep(W, W->SH, W->E->t, 1);
+	alisFatal("H ghosts", alis_dev_ghosts(W->dev, 1));
+	W->t = W->H->t = W->E->t + 0.5 * W->dt;
+	W->hostStale = 1;
+}
+
+
+/* Host-array ghost copy (src/update.c:182-191): F = c F' + s G', G = c G' - s F'. */
+void (copyBoundary)(float ***F, float ***G, float cosk, float sink, int iMin, int iMax, int jMin, int jMax, int kMin, int kMax, int iAlt, int jAlt, int kAlt)
+{
+	for (int i = iMin; i <= iMax; i++) for (int j = jMin; j <= jMax; j++) for (int k = kMin; k <= kMax; k++) {
+		float f = F[i+iAlt][j+jAlt][k+kAlt];
+		if (G) {
+			float g = G[i+iAlt][j+jAlt][k+kAlt];
+			F[i][j][k] = cosk * f + sink * g;
+			G[i][j][k] = cosk * g - sink * f;
+		}
+		else F[i][j][k] = cosk * f;
+	}
+}
+
+
+/*
+ * Second-order (three-wave) coupling of the pump worlds' E into WS inside WS's
+ * dispersive objects that carry a d tensor (src/update.c:344-411).  Runs on the host
+ * arrays: the worlds are synchronised, updated and re-committed at the next step.
+ */
+#define XX(G) (G.x[i][j][k])
+#define YY(G) (G.y[i][j][k])
+#define ZZ(G) (G.z[i][j][k])
+#define YX(G) (0.25 * (G.y[i][j][k] + G.y[i-1][j][k] + G.y[i][j+1][k] + G.y[i-1][j+1][k]))
+#define ZX(G) (0.25 * (G.z[i][j][k] + G.z[i-1][j][k] + G.z[i][j][k+1] + G.z[i-1][j][k+1]))
+#define ZY(G) (0.25 * (G.z[i][j][k] + G.z[i][j-1][k] + G.z[i][j][k+1] + G.z[i][j-1][k+1]))
+#define XY(G) (0.25 * (G.x[i][j][k] + G.x[i][j-1][k] + G.x[i+1][j][k] + G.x[i+1][j-1][k]))
+#define XZ(G) (0.25 * (G.x[i][j][k] + G.x[i][j][k-1] + G.x[i+1][j][k] + G.x[i+1][j][k-1]))
+#define YZ(G) (0.25 * (G.y[i][j][k] + G.y[i][j][k-1] + G.y[i][j+1][k] + G.y[i][j+1][k-1]))
+
+#define TERM(D,v,Jv,G) if (M.D) F.v[i][j][k] += 2 * Alt * M.D * C->v[i][j][k] * C->Jv[n][i][j][k] * (G)
+
+/* The d tensor is read from the n-th matter entry, as the reference does. */
+static void coupleThreeWave(coeffs *C, vfield F, vfield A, vfield B, int n, float Alt)
+{
+	matter M = C->M[n];
+	for (int i = C->iMin[n]; i <= C->iMax[n]-1; i++) for (int j = C->jMin[n]; j <= C->jMax[n]-1; j++) for (int k = C->kMin[n]; k <= C->kMax[n]-1; k++) {
+		TERM(d[0][0], x, Jx, XX(A) * XX(B)); TERM(d[0][1], x, Jx, YX(A) * YX(B)); TERM(d[0][2], x, Jx, ZX(A) * ZX(B));
+		TERM(d[0][3], x, Jx, YX(A) * ZX(B)); TERM(d[0][3], x, Jx, ZX(A) * YX(B));
+		TERM(d[0][4], x, Jx, ZX(A) * XX(B)); TERM(d[0][4], x, Jx, XX(A) * ZX(B));
+		TERM(d[0][5], x, Jx, XX(A) * YX(B)); TERM(d[0][5], x, Jx, YX(A) * XX(B));
+		TERM(d[1][0], y, Jy, XY(A) * XY(B)); TERM(d[1][1], y, Jy, YY(A) * YY(B)); TERM(d[1][2], y, Jy, ZY(A) * ZY(B));
+		TERM(d[1][3], y, Jy, YY(A) * ZY(B)); TERM(d[1][3], y, Jy, ZY(A) * YY(B));
+		TERM(d[1][4], y, Jy, ZY(A) * XY(B)); TERM(d[1][4], y, Jy, XY(A) * ZY(B));
+		TERM(d[1][5], y, Jy, XY(A) * YY(B)); TERM(d[1][5], y, Jy, YY(A) * XY(B));
+		TERM(d[2][0], z, Jz, XZ(A) * XZ(B)); TERM(d[2][1], z, Jz, YZ(A) * YZ(B)); TERM(d[2][2], z, Jz, ZZ(A) * ZZ(B));
+		TERM(d[2][3], z, Jz, YZ(A) * ZZ(B)); TERM(d[2][3], z, Jz, ZZ(A) * YZ(B));
+		TERM(d[2][4], z, Jz, ZZ(A) * XZ(B)); TERM(d[2][4], z, Jz, XZ(A) * ZZ(B));
+		TERM(d[2][5], z, Jz, XZ(A) * YZ(B)); TERM(d[2][5], z, Jz, YZ(A) * XZ(B));
+	}
+}
+
+static void threeWave(world WS, world WP1, world WP2, float Alt)
+{
+	syncWorld(WS), syncWorld(WP1), syncWorld(WP2);
+	for (int n = 0; n < WS->CE->N; n++)
+		for (int m = 0; m <= WS->complexField; m++)
+			coupleThreeWave(WS->CE, WS->E[m], WP1->E[m], WP2->E[m], n, Alt);
+	alisDropDevice(WS);
+}
+
+void nonLinearSHG(world WS, world WP, float Alt) { threeWave(WS, WP, WP, Alt); }
+void nonLinearTWM(world WS, world WP1, world WP2, float Alt) { threeWave(WS, WP1, WP2, Alt); }

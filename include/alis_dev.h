@@ -1,0 +1,152 @@
+/*
+ * alis_dev.h -- C ABI between the host library (alis.h) and the MI355X engine.
+ *
+ * The host library describes a world with plain C types (alis_dev_world_desc: extents,
+ * boundary codes, Bloch phases, coefficient arrays and the host arrays that receive device
+ * state) and commits it.  A committed world lives in HBM: fields, CPML slabs, ADE currents,
+ * coefficients, sources, TF/SF strips, ghost-copy programs and phaser accumulators.  The
+ * host library drives it half step by half step in the reference's order; every call is
+ * asynchronous on the world's stream except the sync/read calls.  Functions return 0 on
+ * success or a negative code; alis_dev_error() describes the last failure.
+ *
+ * Array conventions: a "host 3-D array" is a float*** over a box, k fastest, whose elements
+ * form one contiguous block starting at [i0][j0][k0] (the layout of the reference's
+ * makeField, src/world.c:58-70).  1-D coefficient pointers point at the element of index
+ * MIN (they are indexed by grid index, like the reference's shifted arrays).
+ */
+#ifndef ALIS_DEV_H
+#define ALIS_DEV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALIS_DEV_MAXPOLES 6
+
+/* A dispersive object (src/object.c:178-226, 290-336): box, pole tables, fill factors on
+ * the staggered E positions and the host J/K currents. */
+typedef struct {
+	int i0, i1, j0, j1, k0, k1;
+	int nj, nk;                               /* Drude poles [0,nj), Lorentz/CP pairs [nj,nk) */
+	float a[3];                               /* conductivity factors ax, ay, az */
+	float JJ[3][ALIS_DEV_MAXPOLES], JK[3][ALIS_DEV_MAXPOLES], JE[3][ALIS_DEV_MAXPOLES];
+	float KJ[3][ALIS_DEV_MAXPOLES], KK[3][ALIS_DEV_MAXPOLES], KE[3][ALIS_DEV_MAXPOLES];
+	float ***fill[3];
+	float ***J[2][3][ALIS_DEV_MAXPOLES], ***K[2][3][ALIS_DEV_MAXPOLES];   /* [m][axis][pole] */
+} alis_dev_ade;
+
+/* A source of the E (h=0) or H (h=1) chain (src/alis_c.h:305-312): target component, box,
+ * scalar or per-cell amplitude, and whether its waveform has per-cell phases. */
+typedef struct {
+	int h, m, axis;
+	int i0, i1, j0, j1, k0, k1;
+	float amplitude;
+	float ***Amplitude;
+	int cellPhase;
+} alis_dev_src;
+
+/* A total-field/scattered-field box (src/alis_c.h:313-318). */
+typedef struct {
+	int iMin, iMax, jMin, jMax, kMin, kMax, iMIN, iMAX, jMIN, jMAX, kMIN, kMAX;
+} alis_dev_tfsf;
+
+typedef struct {
+	int iMIN, iMAX, jMIN, jMAX, kMIN, kMAX;   /* grid including CPML */
+	int iMin, iMax, jMin, jMax, kMin, kMax;   /* domain */
+	int iNUM, jNUM, kNUM;
+	int xMinSur, xMaxSur, yMinSur, yMaxSur, zMinSur, zMaxSur;
+	int complexField;
+	float coskx, sinkx, cosky, sinky, coskz, sinkz, coskyp, coskym, sinkyp, sinkym;
+	const float *dt[2][3];                    /* [E,H][axis] dt/d */
+	const float *cpml[2][3][3];               /* [E,H][axis][b, c_first, c_second] or NULL */
+	float ***eps[3];                          /* 1/eps of the E grid (extents MIN..MAX(+1)), or NULL */
+	float ***field[2][2][3];                  /* host fields [E,H][re,im][axis], ghost layer included */
+	float ***slab[2][2][6][2];                /* host CPML slabs [E,H][m][face xMin yMin zMin xMax yMax zMax][a,b] */
+	int nobj;
+	const alis_dev_ade *obj;
+	int nsrc[2];
+	const alis_dev_src *src[2];
+	int ntfsf;
+	const alis_dev_tfsf *tfsf;
+	int expandX, expandY;                     /* TF/SF auxiliary world: rows alias beyond iMAX/jMAX */
+} alis_dev_world_desc;
+
+/* A createPhasers array (src/phaser.c:21-66): nf frequencies over one box; host cfield data
+ * starts per [frequency][face xMin xMax yMin yMax zMin zMax][component E1 E2 H1 H2] as
+ * interleaved re/im floats, NULL for absent faces. */
+typedef struct {
+	int nf;
+	int iMin, iMax, jMin, jMax, kMin, kMax;
+	float **cf;                               /* nf*6*4 pointers */
+} alis_dev_phaser_desc;
+
+typedef struct alis_dev_world alis_dev_world;
+
+const char *alis_dev_error(void);
+
+/* Upload a described world (setup arrays and current host state) and return its device
+ * world.  fp64 selects FP64 fields/state; dft64 selects FP64 phaser accumulators.  TF/SF
+ * auxiliary worlds are committed separately and linked with alis_dev_link_tfsf.
+ * Replaces the allocation side of createWorld/putObjectArray/pointDipole/createTFSF
+ * (src/world.c:261-280, src/object.c:339-405, src/source.c:87-124,269-334).  The
+ * descriptor's arrays must stay valid until the world is released. */
+int alis_dev_commit(const alis_dev_world_desc *W, int fp64, int dft64, alis_dev_world **out);
+int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux);
+void alis_dev_release(alis_dev_world *D);
+
+/* Half-step core: for E (h=0) conductivity scaling and -sum J of dispersive objects, the
+ * curl of H and the CPML slabs (src/update.c:87-125 with Alt=+1); for H (h=1) the curl of
+ * E with CPML (Alt=-1) followed by the ADE current update on the new E
+ * (src/update.c:128-158).  Both real and imaginary parts of a Bloch world. */
+int alis_dev_core(alis_dev_world *D, int h);
+
+/* Face-strip corrections of the s-th TF/SF box from its auxiliary world
+ * (src/update.c:194-272); the auxiliary world must have been stepped first. */
+int alis_dev_strips(alis_dev_world *D, int s, int h);
+
+/* Add amplitude*wave to the box of the index-th source of the E (h=0) or H (h=1) chain
+ * (src/update.c:161-179).  cellWave, when not NULL, gives one wave value per box cell
+ * (k fastest) for sources with per-cell phase; otherwise `wave` applies to all cells. */
+int alis_dev_inject(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell);
+
+/* Ghost-layer copies of E (h=0) or H (h=1): periodic, Bloch, PEC, symmetry and hexagonal
+ * boundaries (src/update.c:182-191 with the schedules at :284-297 and :311-339). */
+int alis_dev_ghosts(alis_dev_world *D, int h);
+
+/* Copy device state to the host arrays of the descriptor: everything (fields, CPML slabs,
+ * ADE currents) when i0 > i1, else E/H inside the box and the currents of objects meeting it. */
+int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1);
+
+/* Read / overwrite one whole field array (h: 0 E, 1 H; m: real/imag; axis 0..2) as doubles
+ * in the layout [iMIN-1..iMAX+1][jMIN-1..jMAX+1][kMIN-1..kMAX+1], k fastest. */
+int alis_dev_read_field(alis_dev_world *D, int h, int m, int axis, double *out);
+int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double *in);
+
+/* Phasers (src/phaser.c:21-110): register an array (accumulators start from the host
+ * cfields), accumulate one sample into frequencies q0..q0+nq-1 (expE/expH: nq complex phase
+ * factors e^{i 2 pi f t} at the E and H times as re/im pairs), copy accumulators to the
+ * host cfields, read them as doubles, remove.  Read layout: [frequency][component E1 E2 H1
+ * H2][cell][re,im] with the cells of the present faces concatenated in the order xMin, xMax,
+ * yMin, yMax, zMin, zMax (x faces (j,k), y faces (i,k), z faces (i,j), last index fastest). */
+int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *id);
+int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH);
+int alis_dev_phaser_sync(alis_dev_world *D, int id);
+long alis_dev_phaser_size(alis_dev_world *D, int id);
+int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
+int alis_dev_phaser_remove(alis_dev_world *D, int id);
+
+/* Wait for all work of D. */
+int alis_dev_synchronize(alis_dev_world *D);
+
+/* Timing of the curl kernels with device events on the world's stream: enable (resets the
+ * totals) or disable, then read total milliseconds and launch count per half (h = 0 E, 1 H). */
+int alis_dev_timing(alis_dev_world *D, int enable);
+int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches);
+
+/* Device-resident bytes of D. */
+long alis_dev_bytes(alis_dev_world *D);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,409 @@
+/*
+ * Scenario driver for parity tests.  Written against the public alis.h API only, so the same
+ * file compiles against the reference library (-DALIS_REF, built from the reference sources
+ * by oracle/Makefile) and against this engine.  Each scenario sets up a small world, steps
+ * it, and prints "key value" lines: FNV-1a hashes and sums of squares of setup arrays and of
+ * the final state, probe traces, phaser accumulators, fluxes and far-field samples.
+ *
+ * usage: drive SCENARIO [STEPS]
+ */
+#include "alis.h"
+#include <stdint.h>
+#include <string.h>
+
+static uint64_t fnv(uint64_t h, const void *p, size_t n)
+{
+	const unsigned char *b = p;
+	for (size_t a = 0; a < n; a++) h = (h ^ b[a]) * 1099511628211ULL;
+	return h;
+}
+
+/* Hash and sum of squares of a host array block (float or complex float). */
+static void putBlock(const char *key, const void *p, long n, int cplx)
+{
+	if (!p) return;
+	const float *f = p;
+	long m = cplx ? 2*n : n;
+	double s = 0;
+	for (long a = 0; a < m; a++) s += (double)f[a] * f[a];
+	printf("%s %016llx %.17g\n", key, (unsigned long long)fnv(1469598103934665603ULL, p, m*sizeof(float)), s);
+}
+
+static void put3(const char *key, float ***F, int i0, int i1, int j0, int j1, int k0, int k1)
+{
+	if (!F) return;
+	putBlock(key, &F[i0][j0][k0], (long)(i1-i0+1)*(j1-j0+1)*(k1-k0+1), 0);
+}
+
+static void put1(const char *key, float *a, int i0, int i1)
+{
+	if (!a) return;
+	putBlock(key, &a[i0], i1-i0+1, 0);
+}
+
+static void sync(world W)
+{
+#ifndef ALIS_REF
+	syncWorld(W);
+#else
+	(void)W;
+#endif
+}
+
+/* Setup arrays: grid coefficients, CPML profiles, 1/eps, ADE tables, source amplitudes. */
+static void dumpSetup(const char *tag, world W)
+{
+	char key[256];
+	printf("%s.grid %d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMIN, W->kMAX,
+		W->iMin, W->iMax, W->jMin, W->jMax, W->kMin, W->kMax, W->complexField);
+	printf("%s.phase %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, W->coskx, W->sinkx, W->cosky, W->sinky,
+		W->coskz, W->sinkz, W->coskyp, W->coskym, W->sinkyp, W->sinkym);
+	printf("%s.clock %lu %lu %.9g %.9g\n", tag, W->N, W->T, W->f, W->dt);
+	for (int h = 0; h < 2; h++) {
+		coeffs *C = h ? W->CH : W->CE;
+		const char *c = h ? "H" : "E";
+		snprintf(key, sizeof key, "%s.dt%s.x", tag, c); put1(key, C->dtdx, W->iMIN, W->iMAX);
+		snprintf(key, sizeof key, "%s.dt%s.y", tag, c); put1(key, C->dtdy, W->jMIN, W->jMAX);
+		snprintf(key, sizeof key, "%s.dt%s.z", tag, c); put1(key, C->dtdz, W->kMIN, W->kMAX);
+		snprintf(key, sizeof key, "%s.cpml%s.x", tag, c); if (C->xP) put1(key, C->xP, W->iMIN, W->iMAX), put1(key, C->xPy, W->iMIN, W->iMAX), put1(key, C->xPz, W->iMIN, W->iMAX);
+		snprintf(key, sizeof key, "%s.cpml%s.y", tag, c); if (C->yP) put1(key, C->yP, W->jMIN, W->jMAX), put1(key, C->yPz, W->jMIN, W->jMAX), put1(key, C->yPx, W->jMIN, W->jMAX);
+		snprintf(key, sizeof key, "%s.cpml%s.z", tag, c); if (C->zP) put1(key, C->zP, W->kMIN, W->kMAX), put1(key, C->zPx, W->kMIN, W->kMAX), put1(key, C->zPy, W->kMIN, W->kMAX);
+	}
+	coeffs *C = W->CE;
+	snprintf(key, sizeof key, "%s.eps.x", tag); put3(key, C->x, W->iMIN, W->iMAX+1, W->jMIN, W->jMAX, W->kMIN, W->kMAX);
+	snprintf(key, sizeof key, "%s.eps.y", tag); put3(key, C->y, W->iMIN, W->iMAX, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX);
+	snprintf(key, sizeof key, "%s.eps.z", tag); put3(key, C->z, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMIN, W->kMAX+1);
+	for (int n = 0; n < C->N; n++) {
+		printf("%s.obj%d %d %d %d %d %d %d %d %d %.9g %.9g %.9g\n", tag, n, C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n],
+			C->kMin[n], C->kMax[n], C->NJ[n], C->NK[n], C->ax[n], C->ay[n], C->az[n]);
+		for (int p = 0; p < C->NK[n]; p++) {
+			printf("%s.obj%d.pole%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JJx[n][p], C->JJy[n][p], C->JJz[n][p], C->JEx[n][p], C->JEy[n][p], C->JEz[n][p]);
+			if (p >= C->NJ[n]) printf("%s.obj%d.pair%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JKx[n][p], C->KJx[n][p], C->KKx[n][p], C->KEx[n][p], C->KEy[n][p], C->KEz[n][p]);
+		}
+		snprintf(key, sizeof key, "%s.obj%d.fill.x", tag, n); put3(key, C->Jx[n], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
+		snprintf(key, sizeof key, "%s.obj%d.fill.y", tag, n); put3(key, C->Jy[n], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
+		snprintf(key, sizeof key, "%s.obj%d.fill.z", tag, n); put3(key, C->Jz[n], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
+	}
+	int s = 0;
+	for (int h = 0; h < 2; h++)
+		for (source S = h ? W->SH : W->SE; S; S = S->S, s++) {
+			printf("%s.src%d %d %d %d %d %d %d %lu %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, s, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax,
+				S->N, S->amplitude, S->phase, S->frequency, S->deviation, S->peaktime, S->resolution);
+			snprintf(key, sizeof key, "%s.src%d.amp", tag, s); put3(key, S->Amplitude, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+			snprintf(key, sizeof key, "%s.src%d.phase", tag, s); put3(key, S->Phase, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+		}
+	int t = 0;
+	for (tfsf T = W->S; T; T = T->S, t++) {
+		printf("%s.tfsf%d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, t, T->iMin, T->iMax, T->jMin, T->jMax, T->kMin, T->kMax,
+			T->iMIN, T->iMAX, T->jMIN, T->jMAX, T->kMIN, T->kMAX);
+		char sub[64];
+		snprintf(sub, sizeof sub, "%s.aux%d", tag, t);
+		dumpSetup(sub, T->W);
+	}
+}
+
+/* Final state: fields with ghosts, CPML slabs, ADE currents; TF/SF auxiliary worlds too. */
+static void dumpState(const char *tag, world W)
+{
+	char key[256];
+	sync(W);
+	const char *an = "xyz";
+	for (int h = 0; h < 2; h++) {
+		vfield *V = h ? W->H : W->E;
+		for (int m = 0; m <= W->complexField; m++) {
+			float ***c[3] = { V[m].x, V[m].y, V[m].z };
+			for (int a = 0; a < 3; a++) {
+				snprintf(key, sizeof key, "%s.%s%d.%c", tag, h ? "H" : "E", m, an[a]);
+				putBlock(key, &c[a][W->iMIN-1][W->jMIN-1][W->kMIN-1], (long)(W->iNUM+2)*(W->jNUM+2)*(W->kNUM+2), 0);
+			}
+			int Off = h;
+			float ***sl[12] = { V[m].xMinPy, V[m].xMinPz, V[m].yMinPz, V[m].yMinPx, V[m].zMinPx, V[m].zMinPy,
+			                    V[m].xMaxPy, V[m].xMaxPz, V[m].yMaxPz, V[m].yMaxPx, V[m].zMaxPx, V[m].zMaxPy };
+			for (int f = 0; f < 12; f++) {
+				if (!sl[f]) continue;
+				int face = f / 2, lo, hi;
+				int i0 = W->iMIN, i1 = W->iMAX, j0 = W->jMIN, j1 = W->jMAX, k0 = W->kMIN, k1 = W->kMAX;
+				if (face < 3) lo = (face==0 ? W->iMIN : face==1 ? W->jMIN : W->kMIN) + Off, hi = (face==0 ? W->iMin : face==1 ? W->jMin : W->kMin) + Off - 2;
+				else lo = (face==3 ? W->iMax : face==4 ? W->jMax : W->kMax) + 2, hi = face==3 ? W->iMAX : face==4 ? W->jMAX : W->kMAX;
+				if (face % 3 == 0) i0 = lo, i1 = hi;
+				if (face % 3 == 1) j0 = lo, j1 = hi;
+				if (face % 3 == 2) k0 = lo, k1 = hi;
+				snprintf(key, sizeof key, "%s.%s%d.psi%d", tag, h ? "H" : "E", m, f);
+				put3(key, sl[f], i0, i1, j0, j1, k0, k1);
+			}
+		}
+	}
+	coeffs *C = W->CE;
+	for (int n = 0; n < C->N; n++)
+		for (int m = 0; m <= W->complexField; m++)
+			for (int p = 0; p < C->NK[n]; p++) {
+				float ****J[3] = { W->E[m].Jx[n], W->E[m].Jy[n], W->E[m].Jz[n] };
+				for (int a = 0; a < 3; a++) {
+					snprintf(key, sizeof key, "%s.J%d.%d.%d.%c", tag, n, m, p, an[a]);
+					put3(key, J[a][p], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
+				}
+				if (p >= C->NJ[n]) {
+					float ****K[3] = { W->E[m].Kx[n], W->E[m].Ky[n], W->E[m].Kz[n] };
+					for (int a = 0; a < 3; a++) {
+						snprintf(key, sizeof key, "%s.K%d.%d.%d.%c", tag, n, m, p, an[a]);
+						put3(key, K[a][p], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
+					}
+				}
+			}
+	int t = 0;
+	for (tfsf T = W->S; T; T = T->S, t++) {
+		char sub[64];
+		snprintf(sub, sizeof sub, "%s.aux%d", tag, t);
+		dumpState(sub, T->W);
+	}
+	printf("%s.time %lu %.9g %.9g %.9g\n", tag, W->n, W->t, W->E->t, W->H->t);
+}
+
+static void dumpPhasers(const char *tag, phaser P, int farfield)
+{
+	char key[256];
+	for (int q = 0; P[q].f; q++) {
+		phaser Q = &P[q];
+		printf("%s.ph%d %.9g %.9g %lu %d %d %d %d %d %d\n", tag, q, Q->f, Q->A, Q->N, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
+		cfield *F[6] = { &Q->xMin, &Q->xMax, &Q->yMin, &Q->yMax, &Q->zMin, &Q->zMax };
+		const char *fn[6] = { "xMin", "xMax", "yMin", "yMax", "zMin", "zMax" };
+		for (int f = 0; f < 6; f++) {
+			float complex ***c[6] = { F[f]->Ex, F[f]->Ey, F[f]->Ez, F[f]->Hx, F[f]->Hy, F[f]->Hz };
+			const char *cn[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+			int i = f == 1 ? Q->iMax : Q->iMin, j = f == 3 ? Q->jMax : Q->jMin, k = f == 5 ? Q->kMax : Q->kMin;
+			int ni = f < 2 ? 1 : Q->iMax-Q->iMin+1, nj = (f == 2 || f == 3) ? 1 : Q->jMax-Q->jMin+1, nk = f >= 4 ? 1 : Q->kMax-Q->kMin+1;
+			for (int a = 0; a < 6; a++) {
+				if (!c[a]) continue;
+				snprintf(key, sizeof key, "%s.ph%d.%s.%s", tag, q, fn[f], cn[a]);
+				putBlock(key, &c[a][i][j][k], (long)ni*nj*nk, 1);
+			}
+		}
+		printf("%s.ph%d.flux %.9g\n", tag, q, phaserPoyntingOut(Q));
+		if (farfield) {
+			float th[5] = { 0, 20, 45, 130, 180 }, ph[5] = { 0, 30, 0, 75, 0 };
+			for (int a = 0; a < 5; a++) {
+				float complex p = farField(Q, Ppol, th[a], ph[a]), s = farField(Q, Spol, th[a], ph[a]);
+				printf("%s.ph%d.ff%d %.9g %.9g %.9g %.9g\n", tag, q, a, crealf(p), cimagf(p), crealf(s), cimagf(s));
+			}
+		}
+	}
+}
+
+/* Probe trace: hash of every sampled value plus the last sample of each probe. */
+typedef struct { uint64_t h; double s; float last[16]; int n; } trace;
+
+static void sample(trace *T, world W, field *F[], float pts[][3], int n)
+{
+	for (int a = 0; a < n; a++) {
+		float v = get(W, F[a], pts[a][0], pts[a][1], pts[a][2]);
+		T->h = fnv(T->h ? T->h : 1469598103934665603ULL, &v, sizeof v);
+		T->s += (double)v * v;
+		T->last[a] = v;
+	}
+	T->n = n;
+}
+
+static void putTrace(const char *tag, trace *T)
+{
+	printf("%s.trace %016llx %.17g", tag, (unsigned long long)T->h, T->s);
+	for (int a = 0; a < T->n; a++) printf(" %.9g", T->last[a]);
+	printf("\n");
+}
+
+/* ---- scenarios ---- */
+
+static void dipole(int steps)
+{
+	dom D = {{60}, {60}, {60}};
+	res R = {5, {10}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	pointDipole(W, Ez, 0, 0, 0, Pulse, 600, 300);
+	pointDipole(W, Hx, 20, -10, 30, Sine, 500, 2, 0.5);
+	dumpSetup("dipole", W);
+	field *F[6] = { Ez, Ez, Hy, Ex, Hz, Ey };
+	float pts[6][3] = {{0,0,0}, {30,-20,10}, {10,10,-40}, {-50,40,0}, {20,20,20}, {-30,0,50}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		sample(&T, W, F, pts, 6);
+	}
+	putTrace("dipole", &T);
+	dumpState("dipole", W);
+}
+
+static void symband(int steps)
+{
+	dom D = {{150}, {150}, {150}};
+	res R = {10, {20}, {1240}, {{2, 60}, {2, 60}, {2, 60}}};
+	sur S = {{SYM, PML}, {SYM, PML}, {SYM, PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	pointDipole(W, Ex, 0, 0, 0, Band, 400, 800);
+	phaser P = createPhasers(W, 400, 800, 100);
+	dumpSetup("symband", W);
+	field *F[4] = { Ex, Ex, Hy, Ez };
+	float pts[4][3] = {{0,0,100}, {-40,60,0}, {20,-20,80}, {100,0,0}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		updatePhaser(W, P);
+		sample(&T, W, F, pts, 4);
+	}
+	sync(W);
+	putTrace("symband", &T);
+	dumpState("symband", W);
+	dumpPhasers("symband", P, 1);
+}
+
+static void nanoblock(int steps)
+{
+	dom DTF = {{36}, {36}, {36}};
+	dom DSF = {{48}, {48}, {48}};
+	res R = {2, {4}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	world W = createWorld(DSF, R, S, "");
+	matter Gold = {{1.5}, {{9, 0.07}, {2.6, 0.3, 0.5}, {3.5, 0.6, 0.8}}};
+	object Block = {Box, {{-16, 16}, {-16, 16}, {-8, 8}}};
+	object Substrate = {Box, {{-INF, INF}, {-INF, INF}, {-INF, -8}}};
+	putObjects(W, Gold, Block, n(1.5), Substrate, Air);
+	planewave(W, DTF, Ppol, 0, 0, Pulse, 600, 400);
+	phaser P = createPhasers(W, 500, 700, 100, -42, 42, -42, 42, -42, 42);
+	dumpSetup("nanoblock", W);
+	field *F[5] = { Ex, Ex, Hy, Ez, Ey };
+	float pts[5][3] = {{0,0,12}, {0,0,0}, {10,-6,20}, {-20,4,-20}, {40,40,40}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		updatePhaser(W, P);
+		sample(&T, W, F, pts, 5);
+	}
+	sync(W);
+	putTrace("nanoblock", &T);
+	dumpState("nanoblock", W);
+	dumpPhasers("nanoblock", P, 1);
+}
+
+static void oblique(int steps)
+{
+	dom DTF = {{50}, {50}, {30}};
+	dom DSF = {{50}, {50}, {50}};
+	res R = {5, {10}, {1240}};
+	sur S = {{BBC}, {SYM}, {PML}, {8}};
+	world W = createWorld(DSF, R, S, "");
+	object Block = {Lattice, {{100}, {100}}, objects {{Box, {{0, 30}, {-20, 20}, {-20, 20}}}}};
+	object Substrate = {Box, {{-INF, INF}, {-INF, INF}, {-INF, -20}}};
+	putObjects(W, n(2), Block, n(1.5), Substrate, Air);
+	planewave(W, DTF, Ppol, 35, 0, Sine, 530, 3);
+	phaser P = createPhaser(W, 530, -300, 300, -300, 300);
+	dumpSetup("oblique", W);
+	field *F[4] = { Ex, iEx, Hy, iHz };
+	float pts[4][3] = {{0,0,40}, {20,10,0}, {-30,0,-40}, {10,-20,20}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		updatePhaser(W, P);
+		sample(&T, W, F, pts, 4);
+	}
+	sync(W);
+	putTrace("oblique", &T);
+	dumpState("oblique", W);
+	dumpPhasers("oblique", P, 1);
+}
+
+static void periodic(int steps)
+{
+	dom D = {{40}, {40}, {-40, 80}};
+	res R = {5, {10}, {1240}};
+	sur S = {{SYM, PBC}, {SYM, PBC}, {PEC, PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	matter InGaN = nk(2.46, 0.045, 1240/450);
+	object Slab = {Box, {{-INF, INF}, {-INF, INF}, {0, 40}}};
+	object Mirror = {Box, {{-INF, INF}, {-INF, INF}, {-INF, 0}}};
+	object Layer = {Box, {{-INF, INF}, {-INF, INF}, {10, 20}}};
+	putObjects(W, InGaN, Layer, n(2.46), Slab, Ag, Mirror, n(1.4));
+	pointDipole(W, Ex, 0, 0, 20, Pulse, 450, 100);
+	dumpSetup("periodic", W);
+	field *F[4] = { Ex, Ey, Hy, Hz };
+	float pts[4][3] = {{0,0,20}, {20,10,40}, {-10,30,60}, {30,-30,5}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		sample(&T, W, F, pts, 4);
+	}
+	putTrace("periodic", &T);
+	dumpState("periodic", W);
+}
+
+static void hexagonal(int steps)
+{
+	float kx = 0.2, ky = 1;
+	dom D = {{50}, {0, 50*sqrtf(3)}, {60}};
+	res R = {5, {10, 10*sqrtf(3)/2, 10}, {1240}};
+	sur S = {{BBC, 100*3/kx}, {HBC, 100*sqrtf(3)/ky}, {SYM, PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	object Slab = {Difference, {2}, objects {
+		{Box, {{-INF, INF}, {-INF, INF}, {-20, 20}}},
+		{DLattice, {{100}, {100*sqrtf(3)}}, objects {{RodZ, {{-30, 30}, {-30, 30}, {-INF, INF}}}}},
+	}};
+	putObjects(W, n(3.5), Slab, Air);
+	pointDipole(W, iHz, 20, 10*sqrtf(3), 0, Band, 750, 2000);
+	dumpSetup("hexagonal", W);
+	field *F[3] = { Hz, iHz, Ex };
+	float pts[3][3] = {{20, 0, 0}, {-20, 30, 10}, {0, 60, 30}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		sample(&T, W, F, pts, 3);
+	}
+	putTrace("hexagonal", &T);
+	dumpState("hexagonal", W);
+}
+
+static void multipole(int steps)
+{
+	dom D = {{50}, {50}, {50}};
+	res R = {2.5, {5}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	object A = {Ball, {{-30, 0}, {-20, 10}, {-15, 15}}};
+	object B = {Box, {{-5, 25}, {-10, 20}, {-10, 10}}};
+	object C = {RodY, {{10, 30}, {-40, 40}, {-25, -5}}};
+	object Dd = {Diamond, {{-40, -10}, {10, 40}, {0, 30}}};
+	matter Aniso = {{2, 3, 4}, {{5, 0.1, 1, 0}, {6, 0.2, 1.2, 0}, {4, 0.3, 0.8, 0}}};
+	putObjects(W, Au, A, Ag, B, Si, C, Aniso, Dd, Air);
+	pointDipole(W, Ey, 0, 0, 0, Pulse, 500, 300);
+	pointDipole(W, Ez, 10, -10, 5, Pulse, 550, 200, 2, 45);
+	dumpSetup("multipole", W);
+	field *F[4] = { Ey, Ez, Hx, Jy };
+	float pts[4][3] = {{0,0,0}, {10,-10,5}, {-20,20,10}, {15,5,0}};
+	trace T = {0};
+	for (int n = 1; n <= steps; n++) {
+		updateE(W);
+		updateH(W);
+		sample(&T, W, F, pts, 4);
+	}
+	putTrace("multipole", &T);
+	dumpState("multipole", W);
+}
+
+int main(int argc, char **argv)
+{
+	if (argc < 2) return fprintf(stderr, "usage: %s SCENARIO [STEPS]\n", argv[0]), 2;
+	int steps = argc > 2 ? atoi(argv[2]) : 60;
+	setvbuf(stdout, 0, _IOLBF, 0);
+	if (!strcmp(argv[1], "dipole")) dipole(steps);
+	else if (!strcmp(argv[1], "symband")) symband(steps);
+	else if (!strcmp(argv[1], "nanoblock")) nanoblock(steps);
+	else if (!strcmp(argv[1], "oblique")) oblique(steps);
+	else if (!strcmp(argv[1], "periodic")) periodic(steps);
+	else if (!strcmp(argv[1], "hexagonal")) hexagonal(steps);
+	else if (!strcmp(argv[1], "multipole")) multipole(steps);
+	else return fprintf(stderr, "unknown scenario %s\n", argv[1]), 2;
+	return 0;
+}
