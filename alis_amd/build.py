@@ -2,9 +2,12 @@
 reference's own C code for the oracle checks.
 
 Products (all git-ignored, shipped to GPU boxes with the tree):
-  alis_amd/libalis.so   host library (alis.h API, C99) + HIP engine for gfx950
-  oracle/liboracle.so   CPU restatement used as the test oracle
-  oracle/_ref/          the reference library built from its sources (oracle/Makefile)
+  alis_amd/libalis.so              host library (alis.h API, C99) + HIP engine for gfx950
+  alis_amd/libalis_workloads.so    benchmark workloads (alis_amd/workloads.c) over libalis.so
+  tests/golden/libdrive.so         parity scenarios (tests/golden/drive.c) over libalis.so
+  oracle/liboracle.so              CPU restatement of the engine ABI under the host library
+  tests/golden/libdrive_oracle.so  the same scenarios over liboracle.so
+  oracle/_ref/                     the reference built from its sources (oracle/Makefile)
 """
 import os
 import subprocess
@@ -55,18 +58,37 @@ def build_library():
     tmp = PKG / "libalis.so.tmp"
     # -static-libgcc: complex division (__divsc3) from the compiler's own libgcc.a, the one
     # a program built from the reference's sources with gcc links; the system libgcc_s may
-    # be newer and round float complex quotients differently.
-    run(["g++", "-shared", "-static-libgcc", "-o", str(tmp), *objs, "-fopenmp", "-lz", f"-L{ROCM/'lib'}", "-lamdhip64",
+    # be newer and round float complex quotients differently.  -Bsymbolic: the library's
+    # own calls bind inside it, also when another alis.h implementation is loaded.
+    run(["g++", "-shared", "-static-libgcc", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-fopenmp", "-lz", f"-L{ROCM/'lib'}", "-lamdhip64",
          f"-Wl,-rpath,{ROCM/'lib'}", "-Wl,--no-undefined"])
     os.replace(tmp, out)
+    clients(PKG, "libalis.so", "alis")
     return out
+
+
+def client(src, out, libdir, libname, defines=()):
+    """A C file over one alis.h implementation, as a shared library."""
+    tmp = out.with_suffix(".so.tmp")
+    run(["gcc", *CFLAGS, *defines, "-shared", "-o", str(tmp), str(src), f"-L{libdir}", f"-l{libname}",
+         f"-Wl,-rpath,$ORIGIN/{os.path.relpath(libdir, out.parent)}", "-lm"])
+    os.replace(tmp, out)
+
+
+def clients(libdir, soname, libname):
+    if libname == "alis":
+        client(PKG / "workloads.c", PKG / "libalis_workloads.so", libdir, libname)
+        client(ROOT / "tests" / "golden" / "drive.c", ROOT / "tests" / "golden" / "libdrive.so", libdir, libname)
+    else:
+        client(ROOT / "tests" / "golden" / "drive.c", ROOT / "tests" / "golden" / "libdrive_oracle.so", libdir, libname)
 
 
 def build_oracle():
     """CPU oracle (oracle/Makefile) and, when /root/reference exists, the reference build."""
     run(["make", "-s", "-C", str(ROOT / "oracle"), "liboracle.so"])
+    clients(ROOT / "oracle", "liboracle.so", "oracle")
     if Path("/root/reference/src/update.c").exists():
-        run(["make", "-s", "-C", str(ROOT / "oracle"), "ref"])
+        run(["make", "-s", "-C", str(ROOT / "oracle"), "ref", "ref-programs"])
 
 
 def build(verbose=False):

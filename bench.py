@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of the FDTD hot path on one MI355X.
+
+Workload: the 256^3 nanoblock scattering run (grid edge 256 including a 16-cell CPML on
+every face, dx = 2 nm, Courant 0.5; a 60x60x40 nm gold block with a Drude pole and two
+Lorentz pairs on an n = 1.5 substrate; TF/SF x-polarised plane wave with a 400-900 nm band
+waveform; 32-frequency phaser box), built by alis_amd/workloads.c through the public
+alis.h API.  One step = updateE + updateH + updatePhaser.  Fields in FP32 with FP64 phaser
+accumulators (the headline), then the same run with FP64 fields for comparison.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 256] [--cpu-steps S]
+
+Prints one JSON line.  `value` is Mcell-updates/s of the FP32 run (cells including CPML x
+steps / wall time, inputs resident in HBM, device synchronised on both sides of the timed
+region).  `roofline` is the dominant kernel (the E-field curl + CPML update): algorithmic
+bytes per launch / mean launch time from HIP events recorded on the world's stream around
+every curl launch of the timed region.  `cpu_baseline` times the reference's own C code
+(compiled from its sources into oracle/_ref) on the same workload on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+PEAK_HBM = 8.0e12        # MI355X HBM3E, bytes/s
+
+
+class quiet:
+    """Send the library's progress lines (C stdout) to /dev/null; the JSON line is the only
+    thing this program prints."""
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(null, 1)
+        os.close(null)
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def load():
+    lib = ROOT / "alis_amd" / "libalis.so"
+    wl = ROOT / "alis_amd" / "libalis_workloads.so"
+    if not lib.exists() or not wl.exists():
+        sys.exit("bench: alis_amd/libalis.so is not built (run __graft_entry__.build())")
+    alis = ctypes.CDLL(str(lib))
+    w = ctypes.CDLL(str(wl))
+    w.wl_create.restype = ctypes.c_void_p
+    w.wl_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    w.wl_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    w.wl_cells.restype = ctypes.c_long
+    w.wl_cells.argtypes = [ctypes.c_void_p]
+    w.wl_phaser_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+    w.wl_geometry.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    w.wl_device.restype = ctypes.c_void_p
+    w.wl_device.argtypes = [ctypes.c_void_p]
+    w.wl_free.argtypes = [ctypes.c_void_p]
+    w.wl_checksum.restype = ctypes.c_double
+    w.wl_checksum.argtypes = [ctypes.c_void_p]
+    alis.alis_dev_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    alis.alis_dev_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+    alis.alis_dev_synchronize.argtypes = [ctypes.c_void_p]
+    alis.alis_dev_bytes.restype = ctypes.c_long
+    alis.alis_dev_bytes.argtypes = [ctypes.c_void_p]
+    hip = ctypes.CDLL("libamdhip64.so")
+    return alis, w, hip
+
+
+def curl_bytes(num, pml, s, has_eps):
+    """Algorithmic HBM bytes of one curl launch: E from H reads E, H (and 1/eps) and writes
+    E, 3 components each; H from E reads H, E and writes H.  CPML: each slab cell reads and
+    writes its two psi values.  Slabs are Num = pml-1 cells thick over the full cross
+    section (src/update.c:119-124)."""
+    n = num[0] * num[1] * num[2]
+    e = n * (12 if has_eps else 9) * s
+    h = n * 9 * s
+    slab = 0
+    for f in range(6):
+        if pml[f]:
+            ax = f // 2
+            area = num[0] * num[1] * num[2] // num[ax]
+            slab += (pml[f] - 1) * area
+    return e + slab * 4 * s, h + slab * 4 * s
+
+
+def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps):
+    ctx = w.wl_create(name.encode(), cells, fbits, dbits)
+    if not ctx:
+        sys.exit(f"bench: unknown workload {name}")
+    w.wl_step(ctx, max(warmup, 1))                  # the first step commits the world to HBM
+    dev = w.wl_device(ctx)
+    hip.hipDeviceSynchronize()
+    alis.alis_dev_timing(dev, 1)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    w.wl_step(ctx, steps)
+    hip.hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    ms, n = ctypes.c_double(), ctypes.c_long()
+    kern = []
+    for h in (0, 1):
+        alis.alis_dev_timing_read(dev, h, ctypes.byref(ms), ctypes.byref(n))
+        kern.append((ms.value, n.value))
+    alis.alis_dev_timing(dev, 0)
+    num, pml = (ctypes.c_int * 3)(), (ctypes.c_int * 6)()
+    nobj = w.wl_geometry(ctx, num, pml)
+    surf = ctypes.c_long()
+    nf = w.wl_phaser_info(ctx, ctypes.byref(surf))
+    out = {"cells": w.wl_cells(ctx), "sec": t1 - t0, "kern": kern, "num": list(num), "pml": list(pml),
+           "nobj": nobj, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev)}
+    w.wl_free(ctx)
+    return out
+
+
+def roofline(r, s, dft_s, steps):
+    be, bh = curl_bytes(r["num"], r["pml"], s, r["nobj"] > 0)
+    (mse, ne), (msh, nh) = r["kern"]
+    te, th = mse / max(ne, 1) * 1e-3, msh / max(nh, 1) * 1e-3
+    dom = ("E", be, te, ne) if mse >= msh else ("H", bh, th, nh)
+    ach = dom[1] / dom[2] if dom[2] else 0.0
+    # whole step: both curls + phaser accumulators (read+write, re/im, 4 components)
+    phaser = r["surf"] * 4 * r["nf"] * 2 * 2 * dft_s
+    step_bytes = be + bh + phaser
+    return {
+        "bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+        "frac": round(ach / PEAK_HBM, 4), "traffic": None,
+        "kernel": f"k_curl ({dom[0]} half step, {'with' if dom[0] == 'E' and r['nobj'] else 'no'} 1/eps)",
+        "bytes_per_launch": dom[1], "launch_ms": round(dom[2] * 1e3, 5), "launches": dom[3],
+        "other_curl": {"bytes_per_launch": bh if dom[0] == "E" else be,
+                       "launch_ms": round((th if dom[0] == "E" else te) * 1e3, 5)},
+        "step_bytes": step_bytes,
+        "step_frac": round(step_bytes / (r["sec"] / steps) / PEAK_HBM, 4),
+    }
+
+
+def cpu_baseline(cells, steps):
+    exe = ROOT / "oracle" / "_ref" / "wl_ref"
+    if not exe.exists():
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    r = subprocess.run([str(exe), "nanoblock", str(cells), "2", str(steps)], capture_output=True, text=True,
+                       env=env, timeout=600)
+    if r.returncode:
+        return None
+    sec, ncell, threads = r.stdout.split()[:3]
+    sec, ncell = float(sec), int(ncell)
+    return {"value": round(ncell / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": int(threads),
+            "kind": "reference",
+            "sample": f"reference C (gcc -O3 -fopenmp, its own sources) on the same {cells}^3 nanoblock workload, "
+                      f"{steps} steps after 2 warm-up steps, OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent worlds would shard with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    alis, w, hip = load()
+    with quiet():
+        r32 = run(alis, w, hip, "nanoblock", args.cells, 32, 64, args.warmup, args.steps)
+    value = r32["cells"] * args.steps / r32["sec"] / 1e6
+    rl = roofline(r32, 4, 8, args.steps)
+    line = {
+        "metric": "Mcell-updates/s & %HBM peak (FP32 vs FP64 on GPU); p50/p99 solve latency",
+        "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r32["sec"] / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (nanoblock scattering set up through alis.h; no external data)",
+        "config": {"workload": f"nanoblock {args.cells}^3 (incl. 16-cell CPML), dx 2 nm, Au Drude+2 Lorentz "
+                               f"60x60x40 nm block on n=1.5 substrate, TF/SF 400-900 nm band, 32-frequency phaser",
+                   "cells": r32["cells"], "fields": "fp32", "dft": "fp64", "hbm_resident_bytes": r32["hbm_bytes"]},
+        "roofline": rl,
+    }
+    if not args.no_fp64:
+        with quiet():
+            r64 = run(alis, w, hip, "nanoblock", args.cells, 64, 64, args.warmup, max(args.steps // 2, 1))
+        v64 = r64["cells"] * max(args.steps // 2, 1) / r64["sec"] / 1e6
+        line["fp64"] = {"value": round(v64, 1), "ms_per_step": round(r64["sec"] / max(args.steps // 2, 1) * 1e3, 4),
+                        "roofline_frac": roofline(r64, 8, 8, max(args.steps // 2, 1))["frac"], "fp32_over_fp64": round(value / v64, 2)}
+    line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

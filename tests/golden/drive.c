@@ -5,11 +5,16 @@
  * it, and prints "key value" lines: FNV-1a hashes and sums of squares of setup arrays and of
  * the final state, probe traces, phaser accumulators, fluxes and far-field samples.
  *
- * usage: drive SCENARIO [STEPS]
+ * usage (program): drive SCENARIO [STEPS [OUTPATH]]
+ * usage (library): alis_drive(SCENARIO, STEPS, OUTPATH) writes the same lines to OUTPATH.
  */
 #include "alis.h"
 #include <stdint.h>
 #include <string.h>
+
+static FILE *OUT;
+static world LAST;      /* the scenario's world and phaser array, for the raw dumps */
+static phaser LASTP;
 
 static uint64_t fnv(uint64_t h, const void *p, size_t n)
 {
@@ -26,7 +31,7 @@ static void putBlock(const char *key, const void *p, long n, int cplx)
 	long m = cplx ? 2*n : n;
 	double s = 0;
 	for (long a = 0; a < m; a++) s += (double)f[a] * f[a];
-	printf("%s %016llx %.17g\n", key, (unsigned long long)fnv(1469598103934665603ULL, p, m*sizeof(float)), s);
+	fprintf(OUT, "%s %016llx %.17g\n", key, (unsigned long long)fnv(1469598103934665603ULL, p, m*sizeof(float)), s);
 }
 
 static void put3(const char *key, float ***F, int i0, int i1, int j0, int j1, int k0, int k1)
@@ -54,11 +59,11 @@ static void sync(world W)
 static void dumpSetup(const char *tag, world W)
 {
 	char key[256];
-	printf("%s.grid %d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMIN, W->kMAX,
+	fprintf(OUT, "%s.grid %d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMIN, W->kMAX,
 		W->iMin, W->iMax, W->jMin, W->jMax, W->kMin, W->kMax, W->complexField);
-	printf("%s.phase %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, W->coskx, W->sinkx, W->cosky, W->sinky,
+	fprintf(OUT, "%s.phase %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, W->coskx, W->sinkx, W->cosky, W->sinky,
 		W->coskz, W->sinkz, W->coskyp, W->coskym, W->sinkyp, W->sinkym);
-	printf("%s.clock %lu %lu %.9g %.9g\n", tag, W->N, W->T, W->f, W->dt);
+	fprintf(OUT, "%s.clock %lu %lu %.9g %.9g\n", tag, W->N, W->T, W->f, W->dt);
 	for (int h = 0; h < 2; h++) {
 		coeffs *C = h ? W->CH : W->CE;
 		const char *c = h ? "H" : "E";
@@ -74,11 +79,11 @@ static void dumpSetup(const char *tag, world W)
 	snprintf(key, sizeof key, "%s.eps.y", tag); put3(key, C->y, W->iMIN, W->iMAX, W->jMIN, W->jMAX+1, W->kMIN, W->kMAX);
 	snprintf(key, sizeof key, "%s.eps.z", tag); put3(key, C->z, W->iMIN, W->iMAX, W->jMIN, W->jMAX, W->kMIN, W->kMAX+1);
 	for (int n = 0; n < C->N; n++) {
-		printf("%s.obj%d %d %d %d %d %d %d %d %d %.9g %.9g %.9g\n", tag, n, C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n],
+		fprintf(OUT, "%s.obj%d %d %d %d %d %d %d %d %d %.9g %.9g %.9g\n", tag, n, C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n],
 			C->kMin[n], C->kMax[n], C->NJ[n], C->NK[n], C->ax[n], C->ay[n], C->az[n]);
 		for (int p = 0; p < C->NK[n]; p++) {
-			printf("%s.obj%d.pole%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JJx[n][p], C->JJy[n][p], C->JJz[n][p], C->JEx[n][p], C->JEy[n][p], C->JEz[n][p]);
-			if (p >= C->NJ[n]) printf("%s.obj%d.pair%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JKx[n][p], C->KJx[n][p], C->KKx[n][p], C->KEx[n][p], C->KEy[n][p], C->KEz[n][p]);
+			fprintf(OUT, "%s.obj%d.pole%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JJx[n][p], C->JJy[n][p], C->JJz[n][p], C->JEx[n][p], C->JEy[n][p], C->JEz[n][p]);
+			if (p >= C->NJ[n]) fprintf(OUT, "%s.obj%d.pair%d %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, n, p, C->JKx[n][p], C->KJx[n][p], C->KKx[n][p], C->KEx[n][p], C->KEy[n][p], C->KEz[n][p]);
 		}
 		snprintf(key, sizeof key, "%s.obj%d.fill.x", tag, n); put3(key, C->Jx[n], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
 		snprintf(key, sizeof key, "%s.obj%d.fill.y", tag, n); put3(key, C->Jy[n], C->iMin[n], C->iMax[n], C->jMin[n], C->jMax[n], C->kMin[n], C->kMax[n]);
@@ -87,14 +92,14 @@ static void dumpSetup(const char *tag, world W)
 	int s = 0;
 	for (int h = 0; h < 2; h++)
 		for (source S = h ? W->SH : W->SE; S; S = S->S, s++) {
-			printf("%s.src%d %d %d %d %d %d %d %lu %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, s, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax,
+			fprintf(OUT, "%s.src%d %d %d %d %d %d %d %lu %.9g %.9g %.9g %.9g %.9g %.9g\n", tag, s, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax,
 				S->N, S->amplitude, S->phase, S->frequency, S->deviation, S->peaktime, S->resolution);
 			snprintf(key, sizeof key, "%s.src%d.amp", tag, s); put3(key, S->Amplitude, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
 			snprintf(key, sizeof key, "%s.src%d.phase", tag, s); put3(key, S->Phase, S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
 		}
 	int t = 0;
 	for (tfsf T = W->S; T; T = T->S, t++) {
-		printf("%s.tfsf%d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, t, T->iMin, T->iMax, T->jMin, T->jMax, T->kMin, T->kMax,
+		fprintf(OUT, "%s.tfsf%d %d %d %d %d %d %d %d %d %d %d %d %d\n", tag, t, T->iMin, T->iMax, T->jMin, T->jMax, T->kMin, T->kMax,
 			T->iMIN, T->iMAX, T->jMIN, T->jMAX, T->kMIN, T->kMAX);
 		char sub[64];
 		snprintf(sub, sizeof sub, "%s.aux%d", tag, t);
@@ -156,7 +161,7 @@ static void dumpState(const char *tag, world W)
 		snprintf(sub, sizeof sub, "%s.aux%d", tag, t);
 		dumpState(sub, T->W);
 	}
-	printf("%s.time %lu %.9g %.9g %.9g\n", tag, W->n, W->t, W->E->t, W->H->t);
+	fprintf(OUT, "%s.time %lu %.9g %.9g %.9g\n", tag, W->n, W->t, W->E->t, W->H->t);
 }
 
 static void dumpPhasers(const char *tag, phaser P, int farfield)
@@ -164,7 +169,7 @@ static void dumpPhasers(const char *tag, phaser P, int farfield)
 	char key[256];
 	for (int q = 0; P[q].f; q++) {
 		phaser Q = &P[q];
-		printf("%s.ph%d %.9g %.9g %lu %d %d %d %d %d %d\n", tag, q, Q->f, Q->A, Q->N, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
+		fprintf(OUT, "%s.ph%d %.9g %.9g %lu %d %d %d %d %d %d\n", tag, q, Q->f, Q->A, Q->N, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
 		cfield *F[6] = { &Q->xMin, &Q->xMax, &Q->yMin, &Q->yMax, &Q->zMin, &Q->zMax };
 		const char *fn[6] = { "xMin", "xMax", "yMin", "yMax", "zMin", "zMax" };
 		for (int f = 0; f < 6; f++) {
@@ -178,12 +183,12 @@ static void dumpPhasers(const char *tag, phaser P, int farfield)
 				putBlock(key, &c[a][i][j][k], (long)ni*nj*nk, 1);
 			}
 		}
-		printf("%s.ph%d.flux %.9g\n", tag, q, phaserPoyntingOut(Q));
+		fprintf(OUT, "%s.ph%d.flux %.9g\n", tag, q, phaserPoyntingOut(Q));
 		if (farfield) {
 			float th[5] = { 0, 20, 45, 130, 180 }, ph[5] = { 0, 30, 0, 75, 0 };
 			for (int a = 0; a < 5; a++) {
 				float complex p = farField(Q, Ppol, th[a], ph[a]), s = farField(Q, Spol, th[a], ph[a]);
-				printf("%s.ph%d.ff%d %.9g %.9g %.9g %.9g\n", tag, q, a, crealf(p), cimagf(p), crealf(s), cimagf(s));
+				fprintf(OUT, "%s.ph%d.ff%d %.9g %.9g %.9g %.9g\n", tag, q, a, crealf(p), cimagf(p), crealf(s), cimagf(s));
 			}
 		}
 	}
@@ -205,9 +210,9 @@ static void sample(trace *T, world W, field *F[], float pts[][3], int n)
 
 static void putTrace(const char *tag, trace *T)
 {
-	printf("%s.trace %016llx %.17g", tag, (unsigned long long)T->h, T->s);
-	for (int a = 0; a < T->n; a++) printf(" %.9g", T->last[a]);
-	printf("\n");
+	fprintf(OUT, "%s.trace %016llx %.17g", tag, (unsigned long long)T->h, T->s);
+	for (int a = 0; a < T->n; a++) fprintf(OUT, " %.9g", T->last[a]);
+	fprintf(OUT, "\n");
 }
 
 /* ---- scenarios ---- */
@@ -231,6 +236,7 @@ static void dipole(int steps)
 	}
 	putTrace("dipole", &T);
 	dumpState("dipole", W);
+	LAST = W;
 }
 
 static void symband(int steps)
@@ -255,6 +261,7 @@ static void symband(int steps)
 	putTrace("symband", &T);
 	dumpState("symband", W);
 	dumpPhasers("symband", P, 1);
+	LAST = W, LASTP = P;
 }
 
 static void nanoblock(int steps)
@@ -284,6 +291,7 @@ static void nanoblock(int steps)
 	putTrace("nanoblock", &T);
 	dumpState("nanoblock", W);
 	dumpPhasers("nanoblock", P, 1);
+	LAST = W, LASTP = P;
 }
 
 static void oblique(int steps)
@@ -312,6 +320,7 @@ static void oblique(int steps)
 	putTrace("oblique", &T);
 	dumpState("oblique", W);
 	dumpPhasers("oblique", P, 1);
+	LAST = W, LASTP = P;
 }
 
 static void periodic(int steps)
@@ -337,6 +346,7 @@ static void periodic(int steps)
 	}
 	putTrace("periodic", &T);
 	dumpState("periodic", W);
+	LAST = W;
 }
 
 static void hexagonal(int steps)
@@ -363,6 +373,7 @@ static void hexagonal(int steps)
 	}
 	putTrace("hexagonal", &T);
 	dumpState("hexagonal", W);
+	LAST = W;
 }
 
 static void multipole(int steps)
@@ -390,20 +401,146 @@ static void multipole(int steps)
 	}
 	putTrace("multipole", &T);
 	dumpState("multipole", W);
+	LAST = W;
 }
 
-int main(int argc, char **argv)
+static int run(const char *name, int steps)
 {
-	if (argc < 2) return fprintf(stderr, "usage: %s SCENARIO [STEPS]\n", argv[0]), 2;
-	int steps = argc > 2 ? atoi(argv[2]) : 60;
-	setvbuf(stdout, 0, _IOLBF, 0);
-	if (!strcmp(argv[1], "dipole")) dipole(steps);
-	else if (!strcmp(argv[1], "symband")) symband(steps);
-	else if (!strcmp(argv[1], "nanoblock")) nanoblock(steps);
-	else if (!strcmp(argv[1], "oblique")) oblique(steps);
-	else if (!strcmp(argv[1], "periodic")) periodic(steps);
-	else if (!strcmp(argv[1], "hexagonal")) hexagonal(steps);
-	else if (!strcmp(argv[1], "multipole")) multipole(steps);
-	else return fprintf(stderr, "unknown scenario %s\n", argv[1]), 2;
+	if (!strcmp(name, "dipole")) dipole(steps);
+	else if (!strcmp(name, "symband")) symband(steps);
+	else if (!strcmp(name, "nanoblock")) nanoblock(steps);
+	else if (!strcmp(name, "oblique")) oblique(steps);
+	else if (!strcmp(name, "periodic")) periodic(steps);
+	else if (!strcmp(name, "hexagonal")) hexagonal(steps);
+	else if (!strcmp(name, "multipole")) multipole(steps);
+	else return fprintf(stderr, "unknown scenario %s\n", name), 2;
 	return 0;
 }
+
+/* Free the scenario's world.  Against the reference library the objects are left alive:
+ * its deletePhaser passes the array's end sentinel, not its start, to free() and aborts
+ * (src/phaser.c:135-138). */
+static void release(void)
+{
+#ifdef ALIS_REF
+	LAST = 0, LASTP = 0;
+	return;
+#endif
+	if (LASTP) deletePhaser(LASTP);
+	if (LAST) deleteWorld(LAST);
+	LAST = 0, LASTP = 0;
+}
+
+int alis_drive(const char *name, int steps, const char *path)
+{
+	OUT = fopen(path, "w");
+	if (!OUT) return 1;
+	int rc = run(name, steps);
+	fclose(OUT);
+	OUT = 0;
+	release();
+	return rc;
+}
+
+#ifndef ALIS_REF
+#include "alis_dev.h"
+/* Run a scenario and write the final device state of its main world as doubles: for E then
+ * H, every component of the real (and imaginary) part over the ghost-inclusive grid, then
+ * the phaser accumulators if the scenario has phasers.  Header: 8 longs (iNUM, jNUM, kNUM,
+ * complexField, field doubles, phaser doubles, 0, 0). */
+int alis_drive_raw(const char *name, int steps, const char *path)
+{
+	OUT = fopen("/dev/null", "w");
+	int rc = run(name, steps);
+	fclose(OUT);
+	OUT = 0;
+	if (rc || !LAST || !LAST->dev) return rc ? rc : 3;
+	FILE *f = fopen(path, "wb");
+	if (!f) return 1;
+	world W = LAST;
+	long per = (long)(W->iNUM+2) * (W->jNUM+2) * (W->kNUM+2);
+	long nf = 6 * per * (1 + W->complexField);
+	long np = 0;
+	int pid = -1;
+	if (LASTP) {
+		phaser B = LASTP;
+		pid = B->devId;
+		if (pid >= 0) np = alis_dev_phaser_size(W->dev, pid);
+	}
+	long head[8] = { W->iNUM, W->jNUM, W->kNUM, W->complexField, nf, np > 0 ? np : 0, 0, 0 };
+	fwrite(head, sizeof head, 1, f);
+	double *buf = malloc(per * sizeof(double));
+	for (int h = 0; h < 2; h++)
+		for (int m = 0; m <= W->complexField; m++)
+			for (int a = 0; a < 3; a++) {
+				if (alis_dev_read_field(W->dev, h, m, a, buf)) rc = 4;
+				fwrite(buf, sizeof(double), per, f);
+			}
+	free(buf);
+	if (np > 0) {
+		double *pb = malloc(np * sizeof(double));
+		if (alis_dev_phaser_read(W->dev, pid, pb, np)) rc = 4;
+		fwrite(pb, sizeof(double), np, f);
+		free(pb);
+	}
+	fclose(f);
+	release();
+	return rc;
+}
+
+/* Closed-form check of the Yee update: a vacuum box periodic in x, y and z (node iMAX is
+ * node iMIN, period iNUM-1) holding Ez = cos(2 pi (m i/(iNUM-1) + p j/(jNUM-1))) and H = 0.
+ * This is an eigenvector of the discrete curl-curl, so every sample obeys the leapfrog
+ * recurrence E^{n+1} + E^{n-1} = r E^n with r = 2 - 4 sum_a (dtE/da)(dtH/da) sin^2(k_a d_a/2),
+ * for n >= 2 counted from the sample the mode is written at,
+ * i.e. sin^2(w dt/2)/dt^2 = sum_a sin^2(k_a d_a/2)/d_a^2.  Returns r in *expect and the
+ * recurrence ratio at every step n = 2..steps-1 (where |E^n| > 0.05) in ratio[]; the
+ * return value is the number of ratios written. */
+int alis_physics_dispersion(int cells, int m, int p, int steps, int fp64, double *ratio, double *expect)
+{
+	dom D = {{0, 10.0f * (cells-1)}, {0, 10.0f * (cells-1)}, {0, 40}};
+	res R = {5, {10}, {1240}};
+	sur S = {{PBC}, {PBC}, {PBC}};
+	world W = createWorld(D, R, S, "");
+	worldPrecision(W, fp64 ? 64 : 32, 64);
+	const int ni = W->iNUM - 1, nj = W->jNUM - 1;
+	const long per = (long)(W->iNUM+2) * (W->jNUM+2) * (W->kNUM+2);
+	double *e = malloc((steps + 1) * sizeof(double)), *buf = malloc(per * sizeof(double));
+	/* the first half step commits the (zero) world; the mode is then written in double */
+	updateE(W);
+	long c0 = 0;
+	for (int i = W->iMIN-1; i <= W->iMAX+1; i++)
+		for (int j = W->jMIN-1; j <= W->jMAX+1; j++)
+			for (int k = W->kMIN-1; k <= W->kMAX+1; k++)
+				buf[c0++] = cos(2 * M_PI * ((double)m * (i - W->iMIN) / ni + (double)p * (j - W->jMIN) / nj));
+	alis_dev_write_field(W->dev, 0, 0, 2, buf);
+	const int i0 = W->iMIN + 1, j0 = W->jMIN, k0 = W->kMIN;
+	const double sx = sin(M_PI * m / ni), sy = sin(M_PI * p / nj);
+	*expect = 2 - 4 * (double)W->CE->dtdx[i0] * W->CH->dtdx[i0] * sx * sx - 4 * (double)W->CE->dtdy[j0] * W->CH->dtdy[j0] * sy * sy;
+	const long at = ((long)(i0 - W->iMIN + 1) * (W->jNUM+2) + (j0 - W->jMIN + 1)) * (W->kNUM+2) + (k0 - W->kMIN + 1);
+	e[1] = buf[at];
+	for (int n = 2; n <= steps; n++) {
+		updateH(W);
+		updateE(W);
+		alis_dev_read_field(W->dev, 0, 0, 2, buf);
+		e[n] = buf[at];
+	}
+	int c = 0;
+	for (int n = 2; n < steps; n++)
+		if (fabs(e[n]) > 0.05) ratio[c++] = (e[n+1] + e[n-1]) / e[n];
+	free(e), free(buf);
+	deleteWorld(W);
+	return c;
+}
+#endif
+
+#ifdef DRIVE_MAIN
+int main(int argc, char **argv)
+{
+	if (argc < 2) return fprintf(stderr, "usage: %s SCENARIO [STEPS [OUTPATH]]\n", argv[0]), 2;
+	if (argc > 3) return alis_drive(argv[1], atoi(argv[2]), argv[3]);
+	OUT = stdout;
+	setvbuf(stdout, 0, _IOLBF, 0);
+	return run(argv[1], argc > 2 ? atoi(argv[2]) : 60);
+}
+#endif

@@ -1,0 +1,53 @@
+"""Regenerate the golden fixtures of the parity scenarios from the reference itself.
+
+Runs only where the reference sources are present: oracle/Makefile compiles them (with the
+reference's own flags) into oracle/_ref/libref.a and links tests/golden/drive.c against it
+(oracle/_ref/drive_ref).  Each scenario's output lines are written to
+tests/golden/<scenario>.txt, and manifest.json records the step counts and a SHA-256 of the
+reference sources the fixtures came from.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+REF = Path("/root/reference/src")
+
+# scenario -> steps (each finishes in well under a second on the reference)
+SCENARIOS = {
+    "dipole": 200,
+    "symband": 200,
+    "nanoblock": 200,
+    "oblique": 200,
+    "periodic": 200,
+    "hexagonal": 200,
+    "multipole": 200,
+}
+
+
+def main():
+    if not (REF / "update.c").exists():
+        sys.exit("reference sources not found; fixtures can only be regenerated where they are")
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "ref", "ref-programs"], check=True)
+    drive = ROOT / "oracle" / "_ref" / "drive_ref"
+    for name, steps in SCENARIOS.items():
+        out = HERE / f"{name}.txt"
+        subprocess.run([str(drive), name, str(steps), str(out)], check=True, stdout=subprocess.DEVNULL,
+                       env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"})
+        print(f"{name}: {sum(1 for _ in out.open())} lines")
+    h = hashlib.sha256()
+    for f in sorted(REF.glob("*.[ch]")):
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    manifest = {"scenarios": SCENARIOS, "reference_sources_sha256": h.hexdigest(),
+                "generator": "oracle/_ref/drive_ref (tests/golden/drive.c over the reference library)"}
+    (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
+
+
+if __name__ == "__main__":
+    main()
