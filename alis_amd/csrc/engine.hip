@@ -447,7 +447,8 @@ struct alis_dev_world {
 	std::vector<std::vector<Op<float>>> ghostF[2];  // float programs (FP32 worlds)
 	std::vector<std::vector<Op<double>>> ghostD[2]; // double programs (FP64 worlds)
 	std::vector<DevPhaser> phs;
-	hipStream_t st = nullptr;
+	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
+	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
 	void *scratch = nullptr;    // pack/unpack buffer (double-sized)
 	long scratchElems = 0;
 	std::vector<void *> allocs;
@@ -1190,6 +1191,7 @@ int alis_dev_commit(const alis_dev_world_desc *W, int fp64, int dft64, alis_dev_
 		delete D;
 		return fail("hipStreamCreate failed");
 	}
+	D->streamRefs = new int(1);
 	int rc = D->fp64 ? commitWorld<double>(D) : commitWorld<float>(D);
 	if (rc) {
 		std::string keep = g_error;
@@ -1211,6 +1213,18 @@ int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
 	if (s < 0 || s >= (int)D->tf.size()) return fail("TF/SF index out of range");
 	if (aux->fp64 != D->fp64) return fail("TF/SF auxiliary world precision differs");
 	D->tf[s].aux = aux;
+	// The strips read the auxiliary fields between its half steps, so both worlds run on
+	// one stream: the auxiliary world gives up its own and joins the main world's.
+	if (aux->st != D->st) {
+		HIPCHECK(hipStreamSynchronize(aux->st));
+		if (--*aux->streamRefs == 0) {
+			HIPCHECK(hipStreamDestroy(aux->st));
+			delete aux->streamRefs;
+		}
+		aux->st = D->st;
+		aux->streamRefs = D->streamRefs;
+		++*D->streamRefs;
+	}
 	return 0;
 }
 
@@ -1227,7 +1241,10 @@ void alis_dev_release(alis_dev_world *D)
 	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
-	if (D->st) hipStreamDestroy(D->st);
+	if (D->st && --*D->streamRefs == 0) {
+		hipStreamDestroy(D->st);
+		delete D->streamRefs;
+	}
 	delete D;
 }
 
