@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cells", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
