@@ -82,28 +82,54 @@ template<class T> struct CurlArgs {
 	CurlFace<T> face[6];        // xMin yMin zMin xMax yMax zMax
 };
 
+// Streaming access: arrays touched once per launch (the updated field, 1/eps, CPML psi)
+// are read and written non-temporally so they do not displace the other field's rows and
+// planes, which neighbouring workgroups read again from L2.
+template<bool NT, class T> __device__ inline T ldS(const T *p)
+{
+	if (NT) return __builtin_nontemporal_load(p);
+	return *p;
+}
+template<bool NT, class T> __device__ inline void stS(T *p, T v)
+{
+	if (NT) __builtin_nontemporal_store(v, p);
+	else *p = v;
+}
+
 // Curl update of one half step plus the CPML slabs (src/update.c:48-125).  One thread per
 // cell; 64 lanes along z, 4 rows along y per workgroup, one x plane per grid z index.
-template<class T, bool HASC>
-__global__ __launch_bounds__(256) void k_curl(CurlArgs<T> a)
+// NT: non-temporal access to the single-use arrays; NTG also to the centre values of the
+// other field.
+// BY: rows along y per workgroup (64 x BY threads).  XCD: one-dimensional grid of tiles
+// dealt out per XCD (see xcdTile), otherwise grid (k tiles, y tiles, x planes).
+__device__ inline int xcdTile(int wid, int nwg);
+
+template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
+__global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 {
-	const int k = a.kMIN + (int)blockIdx.x * 64 + (int)threadIdx.x;
-	const int j = a.jMIN + (int)blockIdx.y * 4 + (int)threadIdx.y;
-	const int i = a.iMIN + (int)blockIdx.z;
+	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+	if (XCD) {
+		const int nkb = (a.nk + 63) / 64, njb = (a.nj + BY - 1) / BY;
+		const int tile = xcdTile((int)blockIdx.x, (int)gridDim.x);
+		bx = tile % nkb, by = (tile / nkb) % njb, bz = tile / (nkb * njb);
+	}
+	const int k = a.kMIN + bx * 64 + (int)threadIdx.x;
+	const int j = a.jMIN + by * BY + (int)threadIdx.y;
+	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
 
 	const long id = a.L.at(i, j, k);
 	const long oI = a.alt * a.L.si, oJ = a.alt * a.L.sj, oK = a.alt;
-	const T gx = a.G0[id], gy = a.G1[id], gz = a.G2[id];
+	const T gx = ldS<NTG>(a.G0 + id), gy = ldS<NTG>(a.G1 + id), gz = ldS<NTG>(a.G2 + id);
 	const T dx = a.dtx[i - a.iMIN], dy = a.dty[j - a.jMIN], dz = a.dtz[k - a.kMIN];
-	T fx = a.F0[id], fy = a.F1[id], fz = a.F2[id];
+	T fx = ldS<NT>(a.F0 + id), fy = ldS<NT>(a.F1 + id), fz = ldS<NT>(a.F2 + id);
 	T cx = 1, cy = 1, cz = 1;
 
 	T ux = dz * (gy - a.G1[id + oK]) + dy * (a.G2[id + oJ] - gz);
 	T uy = dx * (gz - a.G2[id + oI]) + dz * (a.G0[id + oK] - gx);
 	T uz = dy * (gx - a.G0[id + oJ]) + dx * (a.G1[id + oI] - gy);
 	if (HASC) {
-		cx = a.C0[id], cy = a.C1[id], cz = a.C2[id];
+		cx = ldS<NT>(a.C0 + id), cy = ldS<NT>(a.C1 + id), cz = ldS<NT>(a.C2 + id);
 		fx = fx + ux * cx, fy = fy + uy * cy, fz = fz + uz * cz;
 	}
 	else {
@@ -121,28 +147,37 @@ __global__ __launch_bounds__(256) void k_curl(CurlArgs<T> a)
 		const int q = c - (ax == 0 ? a.iMIN : ax == 1 ? a.jMIN : a.kMIN);
 		const T b = P.b[q], ca = P.ca[q], cb = P.cb[q];
 		if (ax == 0) {
-			T pa = b * P.pa[p] + ca * (gz - a.G2[id + oI]);
-			T pb = b * P.pb[p] + cb * (gy - a.G1[id + oI]);
-			P.pa[p] = pa, P.pb[p] = pb;
+			T pa = b * ldS<NT>(P.pa + p) + ca * (gz - a.G2[id + oI]);
+			T pb = b * ldS<NT>(P.pb + p) + cb * (gy - a.G1[id + oI]);
+			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
 			fy = fy + (HASC ? pa * cy : pa);
 			fz = fz + (HASC ? pb * cz : pb);
 		}
 		else if (ax == 1) {
-			T pa = b * P.pa[p] + ca * (gx - a.G0[id + oJ]);
-			T pb = b * P.pb[p] + cb * (gz - a.G2[id + oJ]);
-			P.pa[p] = pa, P.pb[p] = pb;
+			T pa = b * ldS<NT>(P.pa + p) + ca * (gx - a.G0[id + oJ]);
+			T pb = b * ldS<NT>(P.pb + p) + cb * (gz - a.G2[id + oJ]);
+			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
 			fz = fz + (HASC ? pa * cz : pa);
 			fx = fx + (HASC ? pb * cx : pb);
 		}
 		else {
-			T pa = b * P.pa[p] + ca * (gy - a.G1[id + oK]);
-			T pb = b * P.pb[p] + cb * (gx - a.G0[id + oK]);
-			P.pa[p] = pa, P.pb[p] = pb;
+			T pa = b * ldS<NT>(P.pa + p) + ca * (gy - a.G1[id + oK]);
+			T pb = b * ldS<NT>(P.pb + p) + cb * (gx - a.G0[id + oK]);
+			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
 			fx = fx + (HASC ? pa * cx : pa);
 			fy = fy + (HASC ? pb * cy : pb);
 		}
 	}
-	a.F0[id] = fx, a.F1[id] = fy, a.F2[id] = fz;
+	stS<NT>(a.F0 + id, fx), stS<NT>(a.F1 + id, fy), stS<NT>(a.F2 + id, fz);
+}
+
+// Workgroup order for the 8 XCDs of MI355X: the dispatcher deals consecutive workgroup ids
+// to XCDs in turn, so tile numbers are handed out as 8 contiguous ranges, one per XCD, and
+// tiles that share halo rows and planes share an L2.
+__device__ inline int xcdTile(int wid, int nwg)
+{
+	const int per = nwg / 8, rem = nwg % 8, x = wid % 8;
+	return x * per + min(x, rem) + wid / 8;
 }
 
 // ADE: per dispersive object, over its box clipped to the grid.
@@ -266,14 +301,18 @@ __global__ __launch_bounds__(256) void k_ops(OpGroup<T> g)
 	}
 }
 
-// Phaser running DFT (src/phaser.c:83-110): one thread per face cell, the four face
-// components are read once and accumulated into every frequency of the launch.
+// Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
+// frequencies (grid y runs over frequency pairs, so even a small phaser box fills the chip).
+// The face components are read once per thread; accumulators stream coalesced per
+// [frequency][component] row.
 struct PhsFaceGeom {
 	int kind;                   // 0 xMin 1 xMax 2 yMin 3 yMax 4 zMin 5 zMax
 	int fixed;                  // the face's fixed index
 	int n2;                     // cells along the fast face axis
 	long off;                   // first cell of this face in the concatenated cell list
 };
+
+constexpr int PHS_QPT = 2;      // frequencies per thread
 
 template<class T> struct PhsArgs {
 	const T *E[3], *H[3];
@@ -318,7 +357,8 @@ __global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
 	}
 	TA *acc = (TA *)a.acc;
 	const TA half = TA(0.5);
-	for (int q = 0; q < a.nq; q++) {
+	const int qa = (int)blockIdx.y * PHS_QPT, qb = min(a.nq, qa + PHS_QPT);
+	for (int q = qa; q < qb; q++) {
 		const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
 		const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
 		const long base = (long)(a.q0 + q) * 4 * a.ncell;
@@ -1150,7 +1190,8 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 			a.eE[q][0] = expE[2 * (q0 + q)], a.eE[q][1] = expE[2 * (q0 + q) + 1];
 			a.eH[q][0] = expH[2 * (q0 + q)], a.eH[q][1] = expH[2 * (q0 + q) + 1];
 		}
-		k_phaser<T, TA><<<blocksFor(P.ncell), 256, 0, D->st>>>(a);
+		dim3 grid(blocksFor(P.ncell), (a.nq + PHS_QPT - 1) / PHS_QPT);
+		k_phaser<T, TA><<<grid, 256, 0, D->st>>>(a);
 		HIPCHECK(hipGetLastError());
 	}
 	return 0;
