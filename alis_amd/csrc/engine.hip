@@ -71,6 +71,19 @@ template<class T> struct CurlFace {
 	int lo, hi, on;
 };
 
+// A dispersive object as the kernels see it (device memory, one table per world):
+// its box clipped to the grid, the compact layout of its arrays, pole coefficients,
+// fill factors on the staggered E positions and the J/K currents per real/imaginary part.
+template<class T> struct ObjDev {
+	int i0, i1, j0, j1, k0, k1;
+	Lay LO;
+	int NJ, NK, scale;          // scale: any conductivity factor non-zero
+	T a[3];
+	T JJ[3][MAXPOLE], JK[3][MAXPOLE], JE[3][MAXPOLE], KJ[3][MAXPOLE], KK[3][MAXPOLE], KE[3][MAXPOLE];
+	const T *fill[3];
+	T *J[2][3][MAXPOLE], *K[2][3][MAXPOLE];
+};
+
 template<class T> struct CurlArgs {
 	T *F0, *F1, *F2;
 	const T *G0, *G1, *G2;
@@ -80,6 +93,8 @@ template<class T> struct CurlArgs {
 	int iMIN, jMIN, kMIN, ni, nj, nk;
 	int alt;                    // +1: E from H, -1: H from E
 	CurlFace<T> face[6];        // xMin yMin zMin xMax yMax zMax
+	const ObjDev<T> *obj;       // dispersive objects (E: scaling and -sum J before the curl;
+	int nobj, m;                //  H: current update on the new E), part m
 };
 
 // Streaming access: arrays touched once per launch (the updated field, 1/eps, CPML psi)
@@ -125,6 +140,30 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	T fx = ldS<NT>(a.F0 + id), fy = ldS<NT>(a.F1 + id), fz = ldS<NT>(a.F2 + id);
 	T cx = 1, cy = 1, cz = 1;
 
+	// E: conductivity scaling of every object, then -sum_p J_p of every object
+	// (src/update.c:89-108), before the curl
+	if (a.alt > 0 && a.nobj) {
+		for (int n = 0; n < a.nobj; n++) {
+			const ObjDev<T> &O = a.obj[n];
+			if (!O.scale || i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
+			const long o = O.LO.at(i, j, k);
+			T f0 = O.fill[0][o], f1 = O.fill[1][o], f2 = O.fill[2][o];
+			fx *= (T(1) - O.a[0] * f0) / (T(1) + O.a[0] * f0);
+			fy *= (T(1) - O.a[1] * f1) / (T(1) + O.a[1] * f1);
+			fz *= (T(1) - O.a[2] * f2) / (T(1) + O.a[2] * f2);
+		}
+		for (int n = 0; n < a.nobj; n++) {
+			const ObjDev<T> &O = a.obj[n];
+			if (i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
+			const long o = O.LO.at(i, j, k);
+			for (int p = 0; p < O.NK; p++) {
+				fx -= O.J[a.m][0][p][o];
+				fy -= O.J[a.m][1][p][o];
+				fz -= O.J[a.m][2][p][o];
+			}
+		}
+	}
+
 	T ux = dz * (gy - a.G1[id + oK]) + dy * (a.G2[id + oJ] - gz);
 	T uy = dx * (gz - a.G2[id + oI]) + dz * (a.G0[id + oK] - gx);
 	T uz = dy * (gx - a.G0[id + oJ]) + dx * (a.G1[id + oI] - gy);
@@ -169,6 +208,30 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 		}
 	}
 	stS<NT>(a.F0 + id, fx), stS<NT>(a.F1 + id, fy), stS<NT>(a.F2 + id, fz);
+
+	// H: polarisation currents on the new E (src/update.c:128-158); E is this kernel's G
+	if (a.alt < 0 && a.nobj) {
+		const T e[3] = { gx, gy, gz };
+		for (int n = 0; n < a.nobj; n++) {
+			const ObjDev<T> &O = a.obj[n];
+			if (i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
+			const long o = O.LO.at(i, j, k);
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				const T fl = O.fill[c][o];
+				for (int p = 0; p < O.NJ; p++) {
+					T *J = O.J[a.m][c][p] + o;
+					*J = O.JJ[c][p] * *J + O.JE[c][p] * fl * e[c];
+				}
+				for (int p = O.NJ; p < O.NK; p++) {
+					T *J = O.J[a.m][c][p] + o, *K = O.K[a.m][c][p] + o;
+					T Jn = O.JJ[c][p] * *J + O.JK[c][p] * *K + O.JE[c][p] * fl * e[c];
+					T Kn = O.KJ[c][p] * Jn + O.KK[c][p] * *K + O.KE[c][p] * fl * e[c];
+					*J = Jn, *K = Kn;
+				}
+			}
+		}
+	}
 }
 
 // Workgroup order for the 8 XCDs of MI355X: the dispatcher deals consecutive workgroup ids
@@ -180,19 +243,6 @@ __device__ inline int xcdTile(int wid, int nwg)
 	return x * per + min(x, rem) + wid / 8;
 }
 
-// ADE: per dispersive object, over its box clipped to the grid.
-template<class T> struct AdeArgs {
-	T *F[3];                    // field components
-	Lay L;                      // field layout
-	const T *fill[3];           // fill factors (compact over the object box)
-	T *J[3][MAXPOLE], *K[3][MAXPOLE];
-	Lay LO;                     // compact object-box layout
-	int i0, j0, k0, ni, nj, nk; // clipped box
-	T a[3];                     // conductivity factors
-	T JJ[3][MAXPOLE], JK[3][MAXPOLE], JE[3][MAXPOLE], KJ[3][MAXPOLE], KK[3][MAXPOLE], KE[3][MAXPOLE];
-	int NJ, NK, mode;           // mode 0: scale, 1: subtract J, 2: update J/K
-};
-
 __device__ inline void boxCell(long t, int ni, int nj, int nk, int &di, int &dj, int &dk)
 {
 	dk = (int)(t % nk);
@@ -202,103 +252,145 @@ __device__ inline void boxCell(long t, int ni, int nj, int nk, int &di, int &dj,
 	(void)ni;
 }
 
-// Conductivity scaling, current subtraction and current update of one object
-// (src/update.c:89-108, 128-158).
-template<class T>
-__global__ __launch_bounds__(256) void k_ade(AdeArgs<T> a)
-{
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= (long)a.ni * a.nj * a.nk) return;
-	int di, dj, dk;
-	boxCell(t, a.ni, a.nj, a.nk, di, dj, dk);
-	const int i = a.i0 + di, j = a.j0 + dj, k = a.k0 + dk;
-	const long id = a.L.at(i, j, k), o = a.LO.at(i, j, k);
-	if (a.mode == 0) {
-#pragma unroll
-		for (int c = 0; c < 3; c++) {
-			T fl = a.fill[c][o];
-			a.F[c][id] *= (T(1) - a.a[c] * fl) / (T(1) + a.a[c] * fl);
-		}
-	}
-	else if (a.mode == 1) {
-		T f0 = a.F[0][id], f1 = a.F[1][id], f2 = a.F[2][id];
-		for (int p = 0; p < a.NK; p++) {
-			f0 -= a.J[0][p][o];
-			f1 -= a.J[1][p][o];
-			f2 -= a.J[2][p][o];
-		}
-		a.F[0][id] = f0, a.F[1][id] = f1, a.F[2][id] = f2;
-	}
-	else {
-#pragma unroll
-		for (int c = 0; c < 3; c++) {
-			const T e = a.F[c][id], fl = a.fill[c][o];
-			for (int p = 0; p < a.NJ; p++)
-				a.J[c][p][o] = a.JJ[c][p] * a.J[c][p][o] + a.JE[c][p] * fl * e;
-			for (int p = a.NJ; p < a.NK; p++) {
-				T Jn = a.JJ[c][p] * a.J[c][p][o] + a.JK[c][p] * a.K[c][p][o] + a.JE[c][p] * fl * e;
-				T Kn = a.KJ[c][p] * Jn + a.KK[c][p] * a.K[c][p][o] + a.KE[c][p] * fl * e;
-				a.J[c][p][o] = Jn, a.K[c][p][o] = Kn;
-			}
-		}
-	}
-}
-
-// Small patch operations on boxes: ghost copies, TF/SF strips and source injection.
+// The end of a half step in one launch: TF/SF strips and sources (additive operations,
+// src/update.c:161-179,194-272) followed by the ghost copies (src/update.c:182-191 with the
+// schedules at :284-297 and :311-339).  The operations form a program in the reference's
+// call order; one thread per (operation, cell) reproduces its sequential result:
+//  * a cell touched by additive operations on one array is handled by the first of them,
+//    which applies all of them in program order;
+//  * a cell written by ghost copies of one component is handled by the last of them, which
+//    walks its copy chain back to a cell no earlier copy writes and replays the chain
+//    forward from that cell's value.
+// The host splits a program into several launches where a chain's base cell is written by
+// the same launch (see buildPost), so every launch only reads cells it does not write.
 enum { OP_COPY = 0, OP_STRIP = 1, OP_SOURCE = 2 };
 
-template<class T> struct Op {
-	int kind;
-	T *F, *G;                   // COPY: re / im (G may be null); STRIP: target / aux source
-	const T *C;                 // STRIP: inverse permittivity or null; SOURCE: amplitude array or null
-	const T *W;                 // SOURCE: per-cell wave or null
-	Lay LF, LG, LC;             // layouts of F, G (aux for STRIP), C/W (compact box for SOURCE)
+template<class T> struct PostOp {
+	int kind, slot;             // slot: TF/SF box (STRIP) or source index (SOURCE)
+	T *F, *G;                   // target; COPY: imaginary partner or null
+	const T *A;                 // STRIP: auxiliary field
+	const T *C;                 // STRIP: 1/eps or null; SOURCE: amplitude array or null
+	const T *W;                 // SOURCE: per-cell waves or null
+	Lay LF, LA, LC;
 	Box b;
 	int di, dj, dk;
-	T c, s;                     // COPY: cos, sin; STRIP: dt/ds; SOURCE: amplitude, wave
+	T c, s;                     // COPY: cos, sin; STRIP: dt/ds; SOURCE: scalar amplitude
 	int exX, exY, auxIMAX, auxJMAX, altX, altY;
+	long off;                   // first thread of this operation in the launch
 };
 
-constexpr int MAXOPS = 12;
-template<class T> struct OpGroup {
-	int n;
-	Op<T> op[MAXOPS];
+constexpr int MAXSRC = 128;     // sources per chain (E or H) of one world
+constexpr int MAXCHAIN = 24;    // copies per launch
+constexpr int MAXPOST = 48;     // operations per launch (staged in LDS)
+
+template<class T> struct PostArgs {
+	const PostOp<T> *ops;
+	int nops, nadd;             // additive operations first: [0, nadd), copies [nadd, nops)
+	long total;
+	unsigned strips;            // TF/SF boxes whose strips apply this half step (bit s)
+	unsigned src[MAXSRC / 32];  // sources active this half step
+	T wave[MAXSRC];             // their scalar wave values
 };
 
-template<class T>
-__global__ __launch_bounds__(256) void k_ops(OpGroup<T> g)
+__device__ inline bool inBox(const Box &b, int i, int j, int k)
 {
-	const Op<T> &o = g.op[blockIdx.y];
-	const int ni = o.b.i1 - o.b.i0 + 1, nj = o.b.j1 - o.b.j0 + 1, nk = o.b.k1 - o.b.k0 + 1;
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= (long)ni * nj * nk) return;
-	int di, dj, dk;
-	boxCell(t, ni, nj, nk, di, dj, dk);
-	const int i = o.b.i0 + di, j = o.b.j0 + dj, k = o.b.k0 + dk;
-	const long d = o.LF.at(i, j, k);
-	if (o.kind == OP_COPY) {
-		const long s = o.LF.at(i + o.di, j + o.dj, k + o.dk);
-		if (o.G) {
-			const T fs = o.F[s], gs = o.G[s];
-			o.F[d] = o.c * fs + o.s * gs;
-			o.G[d] = o.c * gs - o.s * fs;
-		}
-		else o.F[d] = o.c * o.F[s];
-	}
-	else if (o.kind == OP_STRIP) {
+	return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
+}
+
+template<class T> __device__ inline bool opOn(const PostArgs<T> &a, const PostOp<T> &o)
+{
+	if (o.kind == OP_STRIP) return (a.strips >> o.slot) & 1u;
+	if (o.kind == OP_SOURCE) return (a.src[o.slot >> 5] >> (o.slot & 31)) & 1u;
+	return true;
+}
+
+// F(i,j,k) + dt/ds G_aux (1/eps), or F + amplitude wave, as the reference evaluates them.
+template<class T> __device__ inline T addOne(const PostArgs<T> &a, const PostOp<T> &o, T f, int i, int j, int k)
+{
+	if (o.kind == OP_STRIP) {
 		int ia = i + o.di, ja = j + o.dj;
 		if (o.exX && ia > o.auxIMAX) ia = o.auxIMAX + o.altX;
 		if (o.exY && ja > o.auxJMAX) ja = o.auxJMAX + o.altY;
-		const T g = o.G[o.LG.at(ia, ja, k + o.dk)];
-		if (o.C) o.F[d] = o.F[d] + o.c * g * o.C[o.LC.at(i, j, k)];
-		else o.F[d] = o.F[d] + o.c * g;
+		const T g = o.A[o.LA.at(ia, ja, k + o.dk)];
+		return o.C ? f + o.c * g * o.C[o.LC.at(i, j, k)] : f + o.c * g;
 	}
-	else {
-		const long c = o.LC.at(i, j, k);
-		const T amp = o.C ? o.C[c] : o.c;
-		const T w = o.W ? o.W[c] : o.s;
-		o.F[d] = o.F[d] + amp * w;
+	const long c = o.LC.at(i, j, k);
+	const T amp = o.C ? o.C[c] : o.c;
+	const T w = o.W ? o.W[c] : a.wave[o.slot];
+	return f + amp * w;
+}
+
+template<class T>
+__global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
+{
+	// the launch's operations in LDS; every operation starts on a workgroup boundary, so a
+	// workgroup belongs to exactly one operation
+	__shared__ PostOp<T> ops[MAXPOST];
+	{
+		const int words = a.nops * (int)sizeof(PostOp<T>) / 4;
+		const int *src = (const int *)a.ops;
+		int *dst = (int *)ops;
+		for (int w = (int)threadIdx.x; w < words; w += (int)blockDim.x) dst[w] = src[w];
 	}
+	__syncthreads();
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	int p = 0;
+	while (p + 1 < a.nops && (long)blockIdx.x * blockDim.x >= ops[p + 1].off) p++;
+	const PostOp<T> &o = ops[p];
+	if (!opOn(a, o)) return;
+	const int ni = o.b.i1 - o.b.i0 + 1, nj = o.b.j1 - o.b.j0 + 1, nk = o.b.k1 - o.b.k0 + 1;
+	if (t - o.off >= (long)ni * nj * nk) return;
+	int di, dj, dk;
+	boxCell(t - o.off, ni, nj, nk, di, dj, dk);
+	const int i = o.b.i0 + di, j = o.b.j0 + dj, k = o.b.k0 + dk;
+	if (o.kind != OP_COPY) {
+		for (int q = 0; q < p; q++) {
+			const PostOp<T> &r = ops[q];
+			if (r.F == o.F && opOn(a, r) && inBox(r.b, i, j, k)) return;
+		}
+		for (int q = a.nadd; q < a.nops; q++) {
+			const PostOp<T> &r = ops[q];
+			if ((r.F == o.F || r.G == o.F) && inBox(r.b, i, j, k)) return;
+		}
+		const long d = o.LF.at(i, j, k);
+		T f = o.F[d];
+		for (int q = p; q < a.nadd; q++) {
+			const PostOp<T> &r = ops[q];
+			if (r.F == o.F && opOn(a, r) && inBox(r.b, i, j, k)) f = addOne(a, r, f, i, j, k);
+		}
+		o.F[d] = f;
+		return;
+	}
+	for (int q = p + 1; q < a.nops; q++) {
+		const PostOp<T> &r = ops[q];
+		if (r.F == o.F && inBox(r.b, i, j, k)) return;
+	}
+	int chain[MAXCHAIN], n = 0, q = p;
+	int ci = i, cj = j, ck = k;
+	for (;;) {
+		chain[n++] = q;
+		const PostOp<T> &r = ops[q];
+		ci += r.di, cj += r.dj, ck += r.dk;
+		int prev = -1;
+		for (int u = q - 1; u >= a.nadd; u--)
+			if (ops[u].F == o.F && inBox(ops[u].b, ci, cj, ck)) { prev = u; break; }
+		if (prev < 0 || n == MAXCHAIN) break;
+		q = prev;
+	}
+	const long sidx = o.LF.at(ci, cj, ck);
+	T f = o.F[sidx], g = o.G ? o.G[sidx] : T(0);
+	for (int u = n - 1; u >= 0; u--) {
+		const PostOp<T> &r = ops[chain[u]];
+		if (o.G) {
+			const T fs = f, gs = g;
+			f = r.c * fs + r.s * gs;
+			g = r.c * gs - r.s * fs;
+		}
+		else f = r.c * f;
+	}
+	const long d = o.LF.at(i, j, k);
+	o.F[d] = f;
+	if (o.G) o.G[d] = g;
 }
 
 // Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
@@ -366,10 +458,11 @@ __global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
 		TA *p1 = acc + 2 * (base + a.ncell + t);
 		TA *p2 = acc + 2 * (base + 2 * a.ncell + t);
 		TA *p3 = acc + 2 * (base + 3 * a.ncell + t);
-		p0[0] = p0[0] + er * TA(e1), p0[1] = p0[1] + ei * TA(e1);
-		p1[0] = p1[0] + er * TA(e2), p1[1] = p1[1] + ei * TA(e2);
-		p2[0] = p2[0] + hr * TA(h1) * half, p2[1] = p2[1] + hi * TA(h1) * half;
-		p3[0] = p3[0] + hr * TA(h2) * half, p3[1] = p3[1] + hi * TA(h2) * half;
+		// accumulators are streamed once per sample: non-temporal
+		stS<true>(p0, ldS<true>(p0) + er * TA(e1)), stS<true>(p0 + 1, ldS<true>(p0 + 1) + ei * TA(e1));
+		stS<true>(p1, ldS<true>(p1) + er * TA(e2)), stS<true>(p1 + 1, ldS<true>(p1 + 1) + ei * TA(e2));
+		stS<true>(p2, ldS<true>(p2) + hr * TA(h1) * half), stS<true>(p2 + 1, ldS<true>(p2 + 1) + hi * TA(h1) * half);
+		stS<true>(p3, ldS<true>(p3) + hr * TA(h2) * half), stS<true>(p3 + 1, ldS<true>(p3 + 1) + hi * TA(h2) * half);
 	}
 }
 
@@ -450,6 +543,12 @@ struct DevSrc {
 	long ncell = 0;
 };
 
+struct PostSeg {
+	void *ops = nullptr;        // PostOp<T>[nops] in device memory
+	int nops = 0, nadd = 0;
+	long total = 0;
+};
+
 struct DevTfsf {
 	alis_dev_tfsf S{};
 	alis_dev_world *aux = nullptr;
@@ -482,10 +581,16 @@ struct alis_dev_world {
 	void *cp[2][3][3] = {};     // [h][axis][b, ca, cb] CPML arrays or null
 	DevSlab slab[2][6];
 	std::vector<DevObj> obj;
+	void *objTab = nullptr;     // ObjDev<T>[obj.size()] in device memory
 	std::vector<DevSrc> src[2];
 	std::vector<DevTfsf> tf;
-	std::vector<std::vector<Op<float>>> ghostF[2];  // float programs (FP32 worlds)
-	std::vector<std::vector<Op<double>>> ghostD[2]; // double programs (FP64 worlds)
+	// end-of-half-step programs (k_post launches), built at the first ghosts call after the
+	// commit or a TF/SF link; strips and sources are recorded during the half step
+	bool postBuilt = false;
+	std::vector<PostSeg> post[2];
+	unsigned pendStrips[2] = {0, 0};
+	std::vector<unsigned> pendSrc[2];
+	std::vector<double> pendWave[2];
 	std::vector<DevPhaser> phs;
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
@@ -664,78 +769,40 @@ bool boxesMeet(const Box &a, const Box &b)
 	return !(a.i1 < b.i0 || b.i1 < a.i0 || a.j1 < b.j0 || b.j1 < a.j0 || a.k1 < b.k0 || b.k1 < a.k0);
 }
 
-Box shifted(const Box &a, int di, int dj, int dk)
+inline bool inBoxH(const Box &b, int i, int j, int k)
 {
-	return Box{a.i0 + di, a.i1 + di, a.j0 + dj, a.j1 + dj, a.k0 + dk, a.k1 + dk};
+	return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
 }
 
-// Two operations may run in the same launch when neither writes what the other reads or
-// writes (the launch has no ordering between its operations).  Strips and sources only
-// write their target F (their other arrays are read-only); copies read and write both the
-// real and imaginary arrays.
+// Would a launch of the additive operations A and the copies C read a cell it writes?  The
+// only reads of target arrays are the base cells of copy chains (see k_post): a base cell
+// must be neither a copy destination nor an additive target of the same launch.
 template<class T>
-bool independent(const Op<T> &a, const Op<T> &b)
+bool postConflict(const std::vector<PostOp<T>> &A, const std::vector<PostOp<T>> &C)
 {
-	if (a.kind != OP_COPY && b.kind != OP_COPY) return a.F != b.F || !boxesMeet(a.b, b.b);
-	bool shared = a.F == b.F || (a.G && (a.G == b.F || a.G == b.G)) || (b.G && b.G == a.F);
-	if (!shared) return true;
-	Box ra = a.kind == OP_COPY ? shifted(a.b, a.di, a.dj, a.dk) : a.b;
-	Box rb = b.kind == OP_COPY ? shifted(b.b, b.di, b.dj, b.dk) : b.b;
-	return !boxesMeet(a.b, b.b) && !boxesMeet(a.b, rb) && !boxesMeet(ra, b.b);
-}
-
-template<class T>
-std::vector<std::vector<Op<T>>> groupOps(const std::vector<Op<T>> &ops)
-{
-	std::vector<std::vector<Op<T>>> groups;
-	for (const Op<T> &o : ops) {
-		if (o.b.count() == 0) continue;
-		bool fits = !groups.empty() && (int)groups.back().size() < MAXOPS;
-		if (fits) for (const Op<T> &p : groups.back()) if (!independent(o, p)) { fits = false; break; }
-		if (fits) groups.back().push_back(o);
-		else groups.push_back({o});
+	const int n = (int)C.size();
+	for (int p = 0; p < n; p++) {
+		const Box &b = C[p].b;
+		for (int i = b.i0; i <= b.i1; i++)
+			for (int j = b.j0; j <= b.j1; j++)
+				for (int k = b.k0; k <= b.k1; k++) {
+					bool last = true;
+					for (int q = p + 1; q < n && last; q++) if (C[q].F == C[p].F && inBoxH(C[q].b, i, j, k)) last = false;
+					if (!last) continue;
+					int ci = i, cj = j, ck = k, t = p;
+					for (;;) {
+						ci += C[t].di, cj += C[t].dj, ck += C[t].dk;
+						int prev = -1;
+						for (int u = t - 1; u >= 0; u--) if (C[u].F == C[p].F && inBoxH(C[u].b, ci, cj, ck)) { prev = u; break; }
+						if (prev < 0) break;
+						t = prev;
+					}
+					for (int q = 0; q < n; q++) if (C[q].F == C[p].F && inBoxH(C[q].b, ci, cj, ck)) return true;
+					for (const PostOp<T> &o : A)
+						if ((o.F == C[p].F || (C[p].G && o.F == C[p].G)) && inBoxH(o.b, ci, cj, ck)) return true;
+				}
 	}
-	return groups;
-}
-
-template<class T>
-int launchGroups(alis_dev_world *D, const std::vector<std::vector<Op<T>>> &groups)
-{
-	for (const auto &g : groups) {
-		OpGroup<T> G;
-		G.n = (int)g.size();
-		long most = 0;
-		for (int a = 0; a < G.n; a++) G.op[a] = g[a], most = std::max(most, g[a].b.count());
-		dim3 grid(blocksFor(most), G.n);
-		k_ops<T><<<grid, 256, 0, D->st>>>(G);
-		HIPCHECK(hipGetLastError());
-	}
-	return 0;
-}
-
-template<class T>
-std::vector<std::vector<Op<T>>> &ghostProgram(alis_dev_world *D, int h);
-template<> std::vector<std::vector<Op<float>>> &ghostProgram<float>(alis_dev_world *D, int h) { return D->ghostF[h]; }
-template<> std::vector<std::vector<Op<double>>> &ghostProgram<double>(alis_dev_world *D, int h) { return D->ghostD[h]; }
-
-template<class T>
-void buildGhosts(alis_dev_world *D)
-{
-	for (int h = 0; h < 2; h++) {
-		std::vector<Op<T>> ops;
-		for (const CopySpec &c : ghostSchedule(&D->W, h)) {
-			Op<T> o{};
-			o.kind = OP_COPY;
-			o.F = (T *)D->F[h][0][c.axis];
-			o.G = D->cplx ? (T *)D->F[h][1][c.axis] : nullptr;
-			o.LF = D->L;
-			o.b = c.b;
-			o.di = c.di, o.dj = c.dj, o.dk = c.dk;
-			o.c = (T)c.c, o.s = (T)c.s;
-			ops.push_back(o);
-		}
-		ghostProgram<T>(D, h) = groupOps(ops);
-	}
+	return false;
 }
 
 // Commit: allocate and upload everything of the descriptor.
@@ -844,6 +911,30 @@ int commitWorld(alis_dev_world *D)
 				}
 		D->obj.push_back(O);
 	}
+	if (!D->obj.empty()) {
+		std::vector<ObjDev<T>> tab(D->obj.size());
+		for (size_t n = 0; n < D->obj.size(); n++) {
+			const DevObj &O = D->obj[n];
+			ObjDev<T> &t = tab[n];
+			memset(&t, 0, sizeof t);
+			t.i0 = O.clip.i0, t.i1 = O.clip.i1, t.j0 = O.clip.j0, t.j1 = O.clip.j1, t.k0 = O.clip.k0, t.k1 = O.clip.k1;
+			t.LO = O.LO;
+			t.NJ = O.NJ, t.NK = O.NK;
+			t.scale = O.a[0] || O.a[1] || O.a[2];
+			for (int c = 0; c < 3; c++) {
+				t.a[c] = (T)O.a[c];
+				t.fill[c] = (const T *)O.fill[c];
+				for (int p = 0; p < MAXPOLE; p++) {
+					t.JJ[c][p] = (T)O.JJ[c][p], t.JK[c][p] = (T)O.JK[c][p], t.JE[c][p] = (T)O.JE[c][p];
+					t.KJ[c][p] = (T)O.KJ[c][p], t.KK[c][p] = (T)O.KK[c][p], t.KE[c][p] = (T)O.KE[c][p];
+					for (int m = 0; m <= D->cplx; m++) t.J[m][c][p] = (T *)O.J[m][c][p], t.K[m][c][p] = (T *)O.K[m][c][p];
+				}
+			}
+		}
+		if (devAlloc(D, &D->objTab, tab.size() * sizeof(ObjDev<T>))) return -1;
+		HIPCHECK(hipMemcpyAsync(D->objTab, tab.data(), tab.size() * sizeof(ObjDev<T>), hipMemcpyHostToDevice, D->st));
+		HIPCHECK(hipStreamSynchronize(D->st));
+	}
 
 	// sources
 	for (int h = 0; h < 2; h++)
@@ -873,7 +964,10 @@ int commitWorld(alis_dev_world *D)
 			D->src[h].push_back(s);
 		}
 
-	buildGhosts<T>(D);
+	for (int h = 0; h < 2; h++) {
+		D->pendSrc[h].assign((D->src[h].size() + 31) / 32, 0u);
+		D->pendWave[h].assign(D->src[h].size(), 0.0);
+	}
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
 }
@@ -900,43 +994,10 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 		P.b = (const T *)D->cp[h][ax][0], P.ca = (const T *)D->cp[h][ax][1], P.cb = (const T *)D->cp[h][ax][2];
 		P.L = S.L, P.lo = S.lo, P.hi = S.hi;
 	}
+	a.obj = (const ObjDev<T> *)D->objTab;
+	a.nobj = (int)D->obj.size();
+	a.m = m;
 	return a;
-}
-
-template<class T>
-AdeArgs<T> adeArgs(alis_dev_world *D, const DevObj &O, int m, int mode)
-{
-	AdeArgs<T> a{};
-	for (int c = 0; c < 3; c++) {
-		a.F[c] = (T *)D->F[0][m][c];
-		a.fill[c] = (const T *)O.fill[c];
-		a.a[c] = (T)O.a[c];
-		for (int p = 0; p < O.NK; p++) {
-			a.J[c][p] = (T *)O.J[m][c][p];
-			a.K[c][p] = (T *)O.K[m][c][p];
-			a.JJ[c][p] = (T)O.JJ[c][p], a.JK[c][p] = (T)O.JK[c][p], a.JE[c][p] = (T)O.JE[c][p];
-			a.KJ[c][p] = (T)O.KJ[c][p], a.KK[c][p] = (T)O.KK[c][p], a.KE[c][p] = (T)O.KE[c][p];
-		}
-	}
-	a.L = D->L, a.LO = O.LO;
-	a.i0 = O.clip.i0, a.j0 = O.clip.j0, a.k0 = O.clip.k0;
-	a.ni = O.clip.i1 - O.clip.i0 + 1, a.nj = O.clip.j1 - O.clip.j0 + 1, a.nk = O.clip.k1 - O.clip.k0 + 1;
-	a.NJ = O.NJ, a.NK = O.NK, a.mode = mode;
-	return a;
-}
-
-template<class T>
-int launchAde(alis_dev_world *D, int mode)
-{
-	for (int m = 0; m <= D->cplx; m++)
-		for (const DevObj &O : D->obj) {
-			if (mode == 0 && !(O.a[0] || O.a[1] || O.a[2])) continue;
-			long n = O.clip.count();
-			if (!n) continue;
-			k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, mode));
-			HIPCHECK(hipGetLastError());
-		}
-	return 0;
 }
 
 template<class T>
@@ -946,20 +1007,6 @@ int coreHalf(alis_dev_world *D, int h)
 	if (D->timing) {
 		HIPCHECK(hipEventCreate(&e0));
 		HIPCHECK(hipEventCreate(&e1));
-	}
-	if (h == 0) {
-		// reference order: all conductivity scalings, then all -J subtractions, per part m
-		for (int m = 0; m <= D->cplx; m++) {
-			for (const DevObj &O : D->obj) {
-				if (!(O.a[0] || O.a[1] || O.a[2])) continue;
-				long n = O.clip.count();
-				if (n) k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, 0));
-			}
-			for (const DevObj &O : D->obj) {
-				long n = O.clip.count();
-				if (n) k_ade<T><<<blocksFor(n), 256, 0, D->st>>>(adeArgs<T>(D, O, m, 1));
-			}
-		}
 	}
 	dim3 grid((D->kNUM + 63) / 64, (D->jNUM + 3) / 4, D->iNUM), block(64, 4);
 	if (e0) HIPCHECK(hipEventRecord(e0, D->st));
@@ -974,14 +1021,13 @@ int coreHalf(alis_dev_world *D, int h)
 		D->evPairs[h].push_back(e1);
 	}
 	HIPCHECK(hipGetLastError());
-	if (h == 1) return launchAde<T>(D, 2);
 	return 0;
 }
 
 // TF/SF strips of box s (src/update.c:205-272): F += dt/ds * G_aux * (1/eps) on the 12 face
 // strips, with auxiliary indices beyond its last row aliased as in the reference.
 template<class T>
-int tfsfHalf(alis_dev_world *D, int s, int h)
+int stripOps(alis_dev_world *D, int s, int h, std::vector<PostOp<T>> &ops)
 {
 	if (s >= (int)D->tf.size() || !D->tf[s].aux) return fail("TF/SF box not linked");
 	const alis_dev_world_desc *W = &D->W;
@@ -989,15 +1035,16 @@ int tfsfHalf(alis_dev_world *D, int s, int h)
 	alis_dev_world *A = D->tf[s].aux;
 	const alis_dev_world_desc *AW = &A->W;
 	static const int altY[6] = { 0, 1, 0, 1, 0, 1 }, altX[6] = { 1, 0, 0, 0, 1, 1 };
-	std::vector<Op<T>> ops;
+	const size_t first = ops.size();
 	const float *dtx = W->dt[h][0] - W->iMIN, *dty = W->dt[h][1] - W->jMIN, *dtz = W->dt[h][2] - W->kMIN;
 	auto strip = [&](int m, int fa, int ga, float dtds, int i0, int i1, int j0, int j1, int k0, int k1, int di, int dj, int dk) {
-		Op<T> o{};
+		PostOp<T> o{};
 		o.kind = OP_STRIP;
+		o.slot = s;
 		o.F = (T *)D->F[h][m][fa];
-		o.G = (T *)A->F[1-h][m][ga];
+		o.A = (const T *)A->F[1-h][m][ga];
 		o.C = h == 0 ? (const T *)D->C[fa] : nullptr;
-		o.LF = D->L, o.LG = A->L, o.LC = D->L;
+		o.LF = D->L, o.LA = A->L, o.LC = D->L;
 		o.b = Box{i0, i1, j0, j1, k0, k1};
 		o.di = di, o.dj = dj, o.dk = dk;
 		o.c = (T)dtds;
@@ -1064,7 +1111,8 @@ int tfsfHalf(alis_dev_world *D, int s, int h)
 	}
 	// every strip must stay inside both grids
 	const Box fb = fieldBox(D);
-	for (const Op<T> &o : ops) {
+	for (size_t n = first; n < ops.size(); n++) {
+		const PostOp<T> &o = ops[n];
 		if (o.b.count() == 0) continue;
 		if (o.b.i0 < fb.i0 || o.b.i1 > fb.i1 || o.b.j0 < fb.j0 || o.b.j1 > fb.j1 || o.b.k0 < fb.k0 || o.b.k1 > fb.k1)
 			return fail("TF/SF strip outside the grid");
@@ -1074,23 +1122,121 @@ int tfsfHalf(alis_dev_world *D, int s, int h)
 		if (ia0 < AW->iMIN - 1 || ia1 > AW->iMAX + 1 || ja0 < AW->jMIN - 1 || ja1 > AW->jMAX + 1 || ka0 < AW->kMIN - 1 || ka1 > AW->kMAX + 1)
 			return fail("TF/SF box reaches outside its auxiliary grid");
 	}
-	return launchGroups<T>(D, groupOps(ops));
+	return 0;
 }
 
+// Build the end-of-half-step programs of D: strips of every TF/SF box, sources of the
+// chain, ghost copies; split into launches where a copy chain would read a cell that its
+// launch writes.
 template<class T>
-int injectHalf(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell)
+int buildPost(alis_dev_world *D)
 {
-	if (index >= (int)D->src[h].size()) return fail("source index out of range");
+	if ((int)D->tf.size() > 32) return fail("more than 32 TF/SF boxes in one world");
+	std::vector<std::vector<PostOp<T>>> segOps[2];
+	std::vector<int> segAdd[2];
+	for (int h = 0; h < 2; h++) {
+		if ((int)D->src[h].size() > MAXSRC) return fail("more than 128 sources in one chain");
+		std::vector<PostOp<T>> adds, copies;
+		for (int s = 0; s < (int)D->tf.size(); s++)
+			if (stripOps<T>(D, s, h, adds)) return -1;
+		for (int q = 0; q < (int)D->src[h].size(); q++) {
+			const DevSrc &S = D->src[h][q];
+			PostOp<T> o{};
+			o.kind = OP_SOURCE;
+			o.slot = q;
+			o.F = (T *)D->F[h][S.m][S.axis];
+			o.LF = D->L;
+			o.C = (const T *)S.ampArr;
+			o.W = S.cellWave ? (const T *)S.waveDev : nullptr;
+			o.LC = S.LB;
+			o.b = S.b;
+			o.c = (T)S.amp;
+			adds.push_back(o);
+		}
+		for (const CopySpec &c : ghostSchedule(&D->W, h)) {
+			PostOp<T> o{};
+			o.kind = OP_COPY;
+			o.F = (T *)D->F[h][0][c.axis];
+			o.G = D->cplx ? (T *)D->F[h][1][c.axis] : nullptr;
+			o.LF = D->L;
+			o.b = c.b;
+			o.di = c.di, o.dj = c.dj, o.dk = c.dk;
+			o.c = (T)c.c, o.s = (T)c.s;
+			copies.push_back(o);
+		}
+		auto nonEmpty = [](std::vector<PostOp<T>> &v) {
+			v.erase(std::remove_if(v.begin(), v.end(), [](const PostOp<T> &o) { return o.b.count() == 0; }), v.end());
+		};
+		nonEmpty(adds), nonEmpty(copies);
+		if (copies.size() <= (size_t)MAXCHAIN && adds.size() + copies.size() <= (size_t)MAXPOST && !postConflict(adds, copies)) {
+			std::vector<PostOp<T>> all = adds;
+			all.insert(all.end(), copies.begin(), copies.end());
+			if (!all.empty()) segOps[h].push_back(all), segAdd[h].push_back((int)adds.size());
+			continue;
+		}
+		for (size_t a0 = 0; a0 < adds.size(); a0 += MAXPOST) {
+			std::vector<PostOp<T>> chunk(adds.begin() + a0, adds.begin() + std::min(adds.size(), a0 + MAXPOST));
+			segOps[h].push_back(chunk), segAdd[h].push_back((int)chunk.size());
+		}
+		std::vector<PostOp<T>> group;
+		const std::vector<PostOp<T>> none;
+		for (const PostOp<T> &o : copies) {
+			std::vector<PostOp<T>> trial = group;
+			trial.push_back(o);
+			if (!group.empty() && (trial.size() > (size_t)MAXCHAIN || postConflict(none, trial))) {
+				segOps[h].push_back(group), segAdd[h].push_back(0);
+				group.assign(1, o);
+			}
+			else group.swap(trial);
+		}
+		if (!group.empty()) segOps[h].push_back(group), segAdd[h].push_back(0);
+	}
+	for (int h = 0; h < 2; h++) {
+		D->post[h].clear();
+		for (size_t g = 0; g < segOps[h].size(); g++) {
+			std::vector<PostOp<T>> &v = segOps[h][g];
+			long off = 0;
+			for (PostOp<T> &o : v) o.off = off, off += (o.b.count() + 255) / 256 * 256;
+			PostSeg seg;
+			seg.nops = (int)v.size(), seg.nadd = segAdd[h][g], seg.total = off;
+			if (devAlloc(D, &seg.ops, v.size() * sizeof(PostOp<T>))) return -1;
+			HIPCHECK(hipMemcpyAsync(seg.ops, v.data(), v.size() * sizeof(PostOp<T>), hipMemcpyHostToDevice, D->st));
+			D->post[h].push_back(seg);
+		}
+	}
+	HIPCHECK(hipStreamSynchronize(D->st));
+	D->postBuilt = true;
+	return 0;
+}
+
+// Run the end of half step h: the recorded strips and sources, then the ghost copies.
+template<class T>
+int postHalf(alis_dev_world *D, int h)
+{
+	if (!D->postBuilt && buildPost<T>(D)) return -1;
+	PostArgs<T> a{};
+	a.strips = D->pendStrips[h];
+	for (size_t w = 0; w < D->pendSrc[h].size(); w++) a.src[w] = D->pendSrc[h][w];
+	for (size_t q = 0; q < D->pendWave[h].size(); q++) a.wave[q] = (T)D->pendWave[h][q];
+	for (const PostSeg &g : D->post[h]) {
+		if (!g.total) continue;
+		a.ops = (const PostOp<T> *)g.ops;
+		a.nops = g.nops, a.nadd = g.nadd, a.total = g.total;
+		k_post<T><<<blocksFor(g.total), 256, 0, D->st>>>(a);
+		HIPCHECK(hipGetLastError());
+	}
+	D->pendStrips[h] = 0;
+	std::fill(D->pendSrc[h].begin(), D->pendSrc[h].end(), 0u);
+	return 0;
+}
+
+// Record source `index` of chain h for the end of this half step (src/update.c:161-179);
+// per-cell waves are uploaded now, in stream order before the launch that uses them.
+template<class T>
+int injectRecord(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell)
+{
+	if (index < 0 || index >= (int)D->src[h].size()) return fail("source index out of range");
 	DevSrc &s = D->src[h][index];
-	Op<T> o{};
-	o.kind = OP_SOURCE;
-	o.F = (T *)D->F[h][s.m][s.axis];
-	o.LF = D->L;
-	o.C = (const T *)s.ampArr;
-	o.LC = s.LB;
-	o.b = s.b;
-	o.c = (T)s.amp;
-	o.s = (T)wave;
 	if (cellWave) {
 		if (!s.cellWave || ncell != s.ncell) return fail("per-cell wave does not match the source box");
 		HIPCHECK(hipEventSynchronize(s.waveDone));
@@ -1098,10 +1244,11 @@ int injectHalf(alis_dev_world *D, int h, int index, float wave, const float *cel
 		for (long c = 0; c < ncell; c++) pin[c] = (T)cellWave[c];
 		HIPCHECK(hipMemcpyAsync(s.waveDev, pin, ncell * sizeof(T), hipMemcpyHostToDevice, D->st));
 		HIPCHECK(hipEventRecord(s.waveDone, D->st));
-		o.W = (const T *)s.waveDev;
 	}
-	std::vector<std::vector<Op<T>>> g{{o}};
-	return launchGroups<T>(D, g);
+	else if (s.cellWave) return fail("source with per-cell phases needs per-cell waves");
+	D->pendSrc[h][index >> 5] |= 1u << (index & 31);
+	D->pendWave[h][index] = wave;
+	return 0;
 }
 
 template<class T>
@@ -1254,6 +1401,7 @@ int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
 	if (s < 0 || s >= (int)D->tf.size()) return fail("TF/SF index out of range");
 	if (aux->fp64 != D->fp64) return fail("TF/SF auxiliary world precision differs");
 	D->tf[s].aux = aux;
+	D->postBuilt = false;
 	// The strips read the auxiliary fields between its half steps, so both worlds run on
 	// one stream: the auxiliary world gives up its own and joins the main world's.
 	if (aux->st != D->st) {
@@ -1296,17 +1444,21 @@ int alis_dev_core(alis_dev_world *D, int h)
 
 int alis_dev_strips(alis_dev_world *D, int s, int h)
 {
-	return D->fp64 ? tfsfHalf<double>(D, s, h) : tfsfHalf<float>(D, s, h);
+	if (h < 0 || h > 1 || s < 0 || s >= (int)D->tf.size() || !D->tf[s].aux) return fail("TF/SF box not linked");
+	D->pendStrips[h] |= 1u << s;
+	return 0;
 }
 
 int alis_dev_inject(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell)
 {
-	return D->fp64 ? injectHalf<double>(D, h, index, wave, cellWave, ncell) : injectHalf<float>(D, h, index, wave, cellWave, ncell);
+	if (h < 0 || h > 1) return fail("no such source chain");
+	return D->fp64 ? injectRecord<double>(D, h, index, wave, cellWave, ncell) : injectRecord<float>(D, h, index, wave, cellWave, ncell);
 }
 
 int alis_dev_ghosts(alis_dev_world *D, int h)
 {
-	return D->fp64 ? launchGroups<double>(D, D->ghostD[h]) : launchGroups<float>(D, D->ghostF[h]);
+	if (h < 0 || h > 1) return fail("no such half step");
+	return D->fp64 ? postHalf<double>(D, h) : postHalf<float>(D, h);
 }
 
 int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1)

@@ -101,16 +101,19 @@ void alis_dev_release(alis_dev_world *D);
 int alis_dev_core(alis_dev_world *D, int h);
 
 /* Face-strip corrections of the s-th TF/SF box from its auxiliary world
- * (src/update.c:194-272); the auxiliary world must have been stepped first. */
+ * (src/update.c:194-272).  Recorded: applied by the next alis_dev_ghosts of the same half
+ * step, in the reference's order (strips of every box, then sources, then ghosts). */
 int alis_dev_strips(alis_dev_world *D, int s, int h);
 
 /* Add amplitude*wave to the box of the index-th source of the E (h=0) or H (h=1) chain
  * (src/update.c:161-179).  cellWave, when not NULL, gives one wave value per box cell
- * (k fastest) for sources with per-cell phase; otherwise `wave` applies to all cells. */
+ * (k fastest) for sources with per-cell phase; otherwise `wave` applies to all cells.
+ * Recorded like alis_dev_strips; sources not recorded in a half step are inactive. */
 int alis_dev_inject(alis_dev_world *D, int h, int index, float wave, const float *cellWave, long ncell);
 
-/* Ghost-layer copies of E (h=0) or H (h=1): periodic, Bloch, PEC, symmetry and hexagonal
- * boundaries (src/update.c:182-191 with the schedules at :284-297 and :311-339). */
+/* End of the E (h=0) or H (h=1) half step: the recorded strips and sources, then the
+ * ghost-layer copies -- periodic, Bloch, PEC, symmetry and hexagonal boundaries
+ * (src/update.c:182-191 with the schedules at :284-297 and :311-339). */
 int alis_dev_ghosts(alis_dev_world *D, int h);
 
 /* Copy device state to the host arrays of the descriptor: everything (fields, CPML slabs,
