@@ -140,6 +140,33 @@ def roofline(r, s, dft_s, steps):
     }
 
 
+def calibration_copy(hip, nbytes):
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes))
+    hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes))
+    hip.hipMemset(src, 0, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)      # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def measured_traffic(cells, fields):
+    """HBM bytes per E-curl launch measured with rocprofv3 counters on this engine source
+    (profiles/k_curl_nanoblock256_traffic.json, made by alis_amd/profile_traffic.py), or
+    None when that measurement was taken on another version of the engine or workload."""
+    f = ROOT / "profiles" / "k_curl_nanoblock256_traffic.json"
+    if not f.exists():
+        return None
+    import hashlib
+    m = json.loads(f.read_text())
+    src = (ROOT / "alis_amd" / "csrc" / "engine.hip").read_bytes()
+    if m.get("engine_sha256") != hashlib.sha256(src).hexdigest() or m.get("cells") != cells or m.get("fields") != fields:
+        return None
+    return m["E"]["traffic_bytes"]
+
+
 def cpu_baseline(cells, steps):
     exe = ROOT / "oracle" / "_ref" / "wl_ref"
     if not exe.exists():
@@ -170,16 +197,22 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a copy of known size that "
+                         "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent worlds would shard with no exchange; see the design notes)", file=sys.stderr)
         sys.exit(2)
     alis, w, hip = load()
+    if args.copy_mib:
+        calibration_copy(hip, args.copy_mib << 20)
     with quiet():
         r32 = run(alis, w, hip, "nanoblock", args.cells, 32, 64, args.warmup, args.steps)
     value = r32["cells"] * args.steps / r32["sec"] / 1e6
     rl = roofline(r32, 4, 8, args.steps)
+    rl["traffic"] = measured_traffic(r32["cells"], "fp32")
     line = {
         "metric": "Mcell-updates/s & %HBM peak (FP32 vs FP64 on GPU); p50/p99 solve latency",
         "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -197,7 +230,7 @@ def main():
         v64 = r64["cells"] * max(args.steps // 2, 1) / r64["sec"] / 1e6
         line["fp64"] = {"value": round(v64, 1), "ms_per_step": round(r64["sec"] / max(args.steps // 2, 1) * 1e3, 4),
                         "roofline_frac": roofline(r64, 8, 8, max(args.steps // 2, 1))["frac"], "fp32_over_fp64": round(value / v64, 2)}
-    line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps)
+    line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(line), flush=True)
 
 
