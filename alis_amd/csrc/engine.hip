@@ -119,15 +119,9 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // dealt out per XCD (see xcdTile), otherwise grid (k tiles, y tiles, x planes).
 __device__ inline int xcdTile(int wid, int nwg);
 
-template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
-__global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
+template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4>
+__device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, int bz)
 {
-	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
-	if (XCD) {
-		const int nkb = (a.nk + 63) / 64, njb = (a.nj + BY - 1) / BY;
-		const int tile = xcdTile((int)blockIdx.x, (int)gridDim.x);
-		bx = tile % nkb, by = (tile / nkb) % njb, bz = tile / (nkb * njb);
-	}
 	const int k = a.kMIN + bx * 64 + (int)threadIdx.x;
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
@@ -234,6 +228,37 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	}
 }
 
+template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
+__global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
+{
+	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+	if (XCD) {
+		const int nkb = (a.nk + 63) / 64, njb = (a.nj + BY - 1) / BY;
+		const int tile = xcdTile((int)blockIdx.x, (int)gridDim.x);
+		bx = tile % nkb, by = (tile / nkb) % njb, bz = tile / (nkb * njb);
+	}
+	curlCell<T, HASC, NT, NTG, BY>(a, bx, by, bz);
+}
+
+// Several worlds (and real/imaginary parts) in one launch: entry e owns workgroups
+// [start[e], start[e+1]) of a one-dimensional grid, tiled like k_curl.
+template<class T>
+__global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *args, const int *start, int n)
+{
+	int lo = 0, hi = n - 1;
+	const int b = (int)blockIdx.x;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) / 2;
+		if (start[mid] <= b) lo = mid;
+		else hi = mid - 1;
+	}
+	const CurlArgs<T> &a = args[lo];
+	const int t = b - start[lo], nkb = (a.nk + 63) / 64, njb = (a.nj + 3) / 4;
+	const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
+	if (a.C0) curlCell<T, true>(a, bx, by, bz);
+	else curlCell<T, false>(a, bx, by, bz);
+}
+
 // Workgroup order for the 8 XCDs of MI355X: the dispatcher deals consecutive workgroup ids
 // to XCDs in turn, so tile numbers are handed out as 8 contiguous ranges, one per XCD, and
 // tiles that share halo rows and planes share an L2.
@@ -320,12 +345,12 @@ template<class T> __device__ inline T addOne(const PostArgs<T> &a, const PostOp<
 	return f + amp * w;
 }
 
+// One workgroup (number blk within its program) of k_post; ops: LDS for the program.
 template<class T>
-__global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
+__device__ __forceinline__ void postBlock(const PostArgs<T> &a, PostOp<T> *ops, long blk)
 {
 	// the launch's operations in LDS; every operation starts on a workgroup boundary, so a
 	// workgroup belongs to exactly one operation
-	__shared__ PostOp<T> ops[MAXPOST];
 	{
 		const int words = a.nops * (int)sizeof(PostOp<T>) / 4;
 		const int *src = (const int *)a.ops;
@@ -333,9 +358,9 @@ __global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
 		for (int w = (int)threadIdx.x; w < words; w += (int)blockDim.x) dst[w] = src[w];
 	}
 	__syncthreads();
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long t = blk * blockDim.x + threadIdx.x;
 	int p = 0;
-	while (p + 1 < a.nops && (long)blockIdx.x * blockDim.x >= ops[p + 1].off) p++;
+	while (p + 1 < a.nops && blk * blockDim.x >= ops[p + 1].off) p++;
 	const PostOp<T> &o = ops[p];
 	if (!opOn(a, o)) return;
 	const int ni = o.b.i1 - o.b.i0 + 1, nj = o.b.j1 - o.b.j0 + 1, nk = o.b.k1 - o.b.k0 + 1;
@@ -391,6 +416,28 @@ __global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
 	const long d = o.LF.at(i, j, k);
 	o.F[d] = f;
 	if (o.G) o.G[d] = g;
+}
+
+template<class T>
+__global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
+{
+	__shared__ PostOp<T> ops[MAXPOST];
+	postBlock(a, ops, (long)blockIdx.x);
+}
+
+// The programs of several worlds in one launch: entry e owns workgroups [start[e], start[e+1]).
+template<class T>
+__global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *args, const long *start, int n)
+{
+	__shared__ PostOp<T> ops[MAXPOST];
+	int lo = 0, hi = n - 1;
+	const long b = (long)blockIdx.x;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) / 2;
+		if (start[mid] <= b) lo = mid;
+		else hi = mid - 1;
+	}
+	postBlock(args[lo], ops, b - start[lo]);
 }
 
 // Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
@@ -1344,6 +1391,162 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 	return 0;
 }
 
+// Make D use the stream of `to` (reference counted; the last user destroys a stream).
+int joinStream(alis_dev_world *D, alis_dev_world *to)
+{
+	if (D->st == to->st) return 0;
+	HIPCHECK(hipStreamSynchronize(D->st));
+	if (--*D->streamRefs == 0) {
+		HIPCHECK(hipStreamDestroy(D->st));
+		delete D->streamRefs;
+	}
+	D->st = to->st;
+	D->streamRefs = to->streamRefs;
+	++*to->streamRefs;
+	return 0;
+}
+
+// ---- batches: several worlds stepped in shared launches ----
+
+}  // namespace
+
+struct alis_dev_batch {
+	std::vector<alis_dev_world *> W;
+	bool fp64 = false;
+	hipStream_t st = nullptr;
+	void *curl[2] = {};         // CurlArgs<T>[] per half step
+	void *curlStart[2] = {};    // int[] workgroup starts
+	int curlN[2] = {}, curlBlocks[2] = {};
+	bool postReady = false;
+	struct Round {
+		std::vector<int> world;     // index into W
+		std::vector<int> seg;       // segment of that world
+		void *args = nullptr;       // PostArgs<T>[] (device)
+		void *pinned = nullptr;     // staging (host, pinned)
+		hipEvent_t done = nullptr;  // staging copy finished
+		void *start = nullptr;      // long[] workgroup starts (device)
+		long blocks = 0;
+	};
+	std::vector<Round> rounds[2];
+	std::vector<void *> allocs;
+};
+
+namespace {
+
+int batchAlloc(alis_dev_batch *B, void **p, size_t bytes)
+{
+	HIPCHECK(hipMalloc(p, bytes ? bytes : 16));
+	B->allocs.push_back(*p);
+	return 0;
+}
+
+template<class T>
+int batchBuildCurl(alis_dev_batch *B)
+{
+	for (int h = 0; h < 2; h++) {
+		std::vector<CurlArgs<T>> args;
+		std::vector<int> start;
+		int blocks = 0;
+		for (alis_dev_world *D : B->W)
+			for (int m = 0; m <= D->cplx; m++) {
+				args.push_back(curlArgs<T>(D, h, m));
+				start.push_back(blocks);
+				blocks += ((D->kNUM + 63) / 64) * ((D->jNUM + 3) / 4) * D->iNUM;
+			}
+		B->curlN[h] = (int)args.size(), B->curlBlocks[h] = blocks;
+		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
+		if (batchAlloc(B, &B->curlStart[h], start.size() * sizeof(int))) return -1;
+		HIPCHECK(hipMemcpyAsync(B->curl[h], args.data(), args.size() * sizeof(CurlArgs<T>), hipMemcpyHostToDevice, B->st));
+		HIPCHECK(hipMemcpyAsync(B->curlStart[h], start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, B->st));
+	}
+	HIPCHECK(hipStreamSynchronize(B->st));
+	return 0;
+}
+
+template<class T>
+int batchBuildPost(alis_dev_batch *B)
+{
+	for (alis_dev_world *D : B->W)
+		if (!D->postBuilt && buildPost<T>(D)) return -1;
+	for (int h = 0; h < 2; h++) {
+		size_t rounds = 0;
+		for (alis_dev_world *D : B->W) rounds = std::max(rounds, D->post[h].size());
+		B->rounds[h].assign(rounds, alis_dev_batch::Round{});
+		for (size_t r = 0; r < rounds; r++) {
+			alis_dev_batch::Round &R = B->rounds[h][r];
+			std::vector<long> start;
+			for (int w = 0; w < (int)B->W.size(); w++) {
+				alis_dev_world *D = B->W[w];
+				if (r >= D->post[h].size() || !D->post[h][r].total) continue;
+				R.world.push_back(w), R.seg.push_back((int)r);
+				start.push_back(R.blocks);
+				R.blocks += D->post[h][r].total / 256;
+			}
+			if (R.world.empty()) continue;
+			const size_t bytes = R.world.size() * sizeof(PostArgs<T>);
+			if (batchAlloc(B, &R.args, bytes)) return -1;
+			if (batchAlloc(B, &R.start, start.size() * sizeof(long))) return -1;
+			HIPCHECK(hipHostMalloc(&R.pinned, bytes, hipHostMallocDefault));
+			HIPCHECK(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
+			HIPCHECK(hipEventRecord(R.done, B->st));
+			HIPCHECK(hipMemcpyAsync(R.start, start.data(), start.size() * sizeof(long), hipMemcpyHostToDevice, B->st));
+		}
+	}
+	HIPCHECK(hipStreamSynchronize(B->st));
+	B->postReady = true;
+	return 0;
+}
+
+template<class T>
+int batchCore(alis_dev_batch *B, int h)
+{
+	alis_dev_world *D0 = B->W[0];
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (D0->timing) {
+		HIPCHECK(hipEventCreate(&e0));
+		HIPCHECK(hipEventCreate(&e1));
+		HIPCHECK(hipEventRecord(e0, B->st));
+	}
+	k_curl_batch<T><<<B->curlBlocks[h], dim3(64, 4), 0, B->st>>>((const CurlArgs<T> *)B->curl[h], (const int *)B->curlStart[h], B->curlN[h]);
+	HIPCHECK(hipGetLastError());
+	if (e1) {
+		HIPCHECK(hipEventRecord(e1, B->st));
+		D0->evPairs[h].push_back(e0);
+		D0->evPairs[h].push_back(e1);
+	}
+	return 0;
+}
+
+template<class T>
+int batchPost(alis_dev_batch *B, int h)
+{
+	if (!B->postReady && batchBuildPost<T>(B)) return -1;
+	for (alis_dev_batch::Round &R : B->rounds[h]) {
+		if (R.world.empty()) continue;
+		HIPCHECK(hipEventSynchronize(R.done));
+		PostArgs<T> *a = (PostArgs<T> *)R.pinned;
+		for (size_t e = 0; e < R.world.size(); e++) {
+			alis_dev_world *D = B->W[R.world[e]];
+			const PostSeg &g = D->post[h][R.seg[e]];
+			memset(&a[e], 0, sizeof(PostArgs<T>));
+			a[e].ops = (const PostOp<T> *)g.ops;
+			a[e].nops = g.nops, a[e].nadd = g.nadd, a[e].total = g.total;
+			a[e].strips = D->pendStrips[h];
+			for (size_t w = 0; w < D->pendSrc[h].size(); w++) a[e].src[w] = D->pendSrc[h][w];
+			for (size_t q = 0; q < D->pendWave[h].size(); q++) a[e].wave[q] = (T)D->pendWave[h][q];
+		}
+		HIPCHECK(hipMemcpyAsync(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), hipMemcpyHostToDevice, B->st));
+		HIPCHECK(hipEventRecord(R.done, B->st));
+		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const long *)R.start, (int)R.world.size());
+		HIPCHECK(hipGetLastError());
+	}
+	for (alis_dev_world *D : B->W) {
+		D->pendStrips[h] = 0;
+		std::fill(D->pendSrc[h].begin(), D->pendSrc[h].end(), 0u);
+	}
+	return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -1404,17 +1607,7 @@ int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
 	D->postBuilt = false;
 	// The strips read the auxiliary fields between its half steps, so both worlds run on
 	// one stream: the auxiliary world gives up its own and joins the main world's.
-	if (aux->st != D->st) {
-		HIPCHECK(hipStreamSynchronize(aux->st));
-		if (--*aux->streamRefs == 0) {
-			HIPCHECK(hipStreamDestroy(aux->st));
-			delete aux->streamRefs;
-		}
-		aux->st = D->st;
-		aux->streamRefs = D->streamRefs;
-		++*D->streamRefs;
-	}
-	return 0;
+	return joinStream(aux, D);
 }
 
 void alis_dev_release(alis_dev_world *D)
@@ -1591,6 +1784,50 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 	HIPCHECK(hipFree(P.stage));
 	P.live = false;
 	return 0;
+}
+
+int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out)
+{
+	*out = nullptr;
+	if (n < 1) return fail("empty batch");
+	alis_dev_batch *B = new alis_dev_batch;
+	B->W.assign(D, D + n);
+	B->fp64 = D[0]->fp64;
+	for (int w = 0; w < n; w++) {
+		if (D[w]->fp64 != B->fp64) { delete B; return fail("worlds of one batch must share their precision"); }
+		for (int v = 0; v < w; v++) if (D[v] == D[w]) { delete B; return fail("a world appears twice in a batch"); }
+		if (joinStream(D[w], D[0])) { delete B; return -1; }
+	}
+	B->st = D[0]->st;
+	int rc = B->fp64 ? batchBuildCurl<double>(B) : batchBuildCurl<float>(B);
+	if (rc) { alis_dev_batch_release(B); return rc; }
+	*out = B;
+	return 0;
+}
+
+int alis_dev_batch_core(alis_dev_batch *B, int h)
+{
+	if (h < 0 || h > 1) return fail("no such half step");
+	return B->fp64 ? batchCore<double>(B, h) : batchCore<float>(B, h);
+}
+
+int alis_dev_batch_post(alis_dev_batch *B, int h)
+{
+	if (h < 0 || h > 1) return fail("no such half step");
+	return B->fp64 ? batchPost<double>(B, h) : batchPost<float>(B, h);
+}
+
+void alis_dev_batch_release(alis_dev_batch *B)
+{
+	if (!B) return;
+	if (B->st) hipStreamSynchronize(B->st);
+	for (void *p : B->allocs) hipFree(p);
+	for (int h = 0; h < 2; h++)
+		for (alis_dev_batch::Round &R : B->rounds[h]) {
+			if (R.pinned) hipHostFree(R.pinned);
+			if (R.done) hipEventDestroy(R.done);
+		}
+	delete B;
 }
 
 int alis_dev_synchronize(alis_dev_world *D)

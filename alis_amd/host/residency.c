@@ -9,6 +9,35 @@
 #include "internal.h"
 
 static world *liveWorlds;
+static alis_dev_batch *cachedBatch;      /* the last batch of updateWorldsE/H and its worlds */
+static alis_dev_world **cachedList;
+static int cachedN;
+
+/* Drop the cached batch: called whenever a device world goes away, so that a later world at
+ * a reused address can never meet tables of a freed one. */
+static void dropBatch(void)
+{
+	if (cachedBatch) alis_dev_batch_release(cachedBatch);
+	cachedBatch = 0;
+	free(cachedList);
+	cachedList = 0;
+	cachedN = 0;
+}
+
+/* The batch of the committed worlds L[0..n-1] (created on first use, reused while the list
+ * stays the same). */
+alis_dev_batch *alisBatch(world *L, int n)
+{
+	int same = cachedBatch && cachedN == n;
+	for (int a = 0; same && a < n; a++) same = cachedList[a] == L[a]->dev;
+	if (same) return cachedBatch;
+	dropBatch();
+	alis_dev_world **D = malloc(n * sizeof *D);
+	for (int a = 0; a < n; a++) D[a] = L[a]->dev;
+	alisFatal("batch", alis_dev_batch_create(D, n, &cachedBatch));
+	cachedList = D, cachedN = n;
+	return cachedBatch;
+}
 static int nWorlds, capWorlds;
 static phaser *livePhasers;
 static int nPhasers, capPhasers;
@@ -50,6 +79,7 @@ void alisRegisterWorld(world W)
 void alisForgetWorld(world W)
 {
 	if (W->dev) {
+		dropBatch();
 		alis_dev_release(W->dev);
 		W->dev = 0;
 	}
@@ -135,6 +165,7 @@ static void releaseTree(world W)
 {
 	for (tfsf S = W->S; S; S = S->S) releaseTree(S->W);
 	if (W->dev) {
+		dropBatch();
 		alis_dev_release(W->dev);
 		W->dev = 0;
 	}

@@ -79,6 +79,48 @@ void updateH(world W)
 }
 
 
+/*
+ * Several worlds per call (an addition to the reference's API, which steps one world per
+ * call): the half step of every world in W[0..n-1] and of their TF/SF auxiliary worlds in
+ * shared launches -- one curl launch for all of them, one post launch per program round.
+ * Each world's arithmetic and order are those of updateE/updateH on it alone.
+ */
+static void collectTree(world W, world **L, int *n, int *cap)
+{
+	if (*n == *cap) *cap = *cap ? 2 * *cap : 16, *L = realloc(*L, *cap * sizeof(world));
+	(*L)[(*n)++] = W;
+	for (tfsf S = W->S; S; S = S->S) collectTree(S->W, L, n, cap);
+}
+
+static void halfStepMany(world *Ws, int n, int h)
+{
+	static world *L;
+	static int cap;
+	int m = 0;
+	for (int a = 0; a < n; a++) alisCommit(Ws[a]);
+	for (int a = 0; a < n; a++) collectTree(Ws[a], &L, &m, &cap);
+	alis_dev_batch *B = alisBatch(L, m);
+	alisFatal(h ? "H core" : "E core", alis_dev_batch_core(B, h));
+	for (int a = 0; a < m; a++) {
+		world W = L[a];
+		int s = 0;
+		for (tfsf S = W->S; S; S = S->S, s++) alisFatal("strips", alis_dev_strips(W->dev, s, h));
+		if (!h && W->SE) sourceHalfStep(W, W->SE, W->H->t, 0);
+		if (h && W->SH) sourceHalfStep(W, W->SH, W->E->t, 1);
+	}
+	alisFatal(h ? "H ghosts" : "E ghosts", alis_dev_batch_post(B, h));
+	for (int a = 0; a < m; a++) {
+		world W = L[a];
+		if (!h) W->t = W->E->t = ++W->n * W->dt;
+		else W->t = W->H->t = W->E->t + 0.5 * W->dt;
+		W->hostStale = 1;
+	}
+}
+
+void updateWorldsE(world *W, int n) { halfStepMany(W, n, 0); }
+void updateWorldsH(world *W, int n) { halfStepMany(W, n, 1); }
+
+
 /* Host-array ghost copy (src/update.c:182-191): F = c F' + s G', G = c G' - s F'. */
 void (copyBoundary)(float ***F, float ***G, float cosk, float sink, int iMin, int iMax, int jMin, int jMax, int kMin, int kMax, int iAlt, int jAlt, int kAlt)
 {

@@ -354,6 +354,10 @@ void worldPrecision(world W, int fieldBits, int dftBits);
 /* Copy every device-resident quantity of W (fields, ADE state, phasers) to the host
  * arrays of the reference layout.  Library observers call it themselves. */
 void syncWorld(world W);
+/* One E (H) half step of n worlds and their TF/SF auxiliary worlds in shared launches; the
+ * same results as updateE (updateH) on each world in turn. */
+void updateWorldsE(world *W, int n);
+void updateWorldsH(world *W, int n);
 
 #ifdef __cplusplus
 }

@@ -138,6 +138,19 @@ long alis_dev_phaser_size(alis_dev_world *D, int id);
 int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
 int alis_dev_phaser_remove(alis_dev_world *D, int id);
 
+/* Batches: several worlds -- with the TF/SF auxiliary worlds of each, which the caller
+ * lists too -- stepped in shared launches.  All worlds join the first one's stream.  A half
+ * step of a batch is alis_dev_batch_core (curl, CPML and ADE of every world in one launch),
+ * then the usual alis_dev_strips / alis_dev_inject records per world, then
+ * alis_dev_batch_post (each world's strips, sources and ghost copies; one launch per
+ * program round).  Worlds keep their own state; results equal stepping them one by one.
+ * No counterpart in the reference, which steps one world per call (src/update.c:275-341). */
+typedef struct alis_dev_batch alis_dev_batch;
+int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out);
+int alis_dev_batch_core(alis_dev_batch *B, int h);
+int alis_dev_batch_post(alis_dev_batch *B, int h);
+void alis_dev_batch_release(alis_dev_batch *B);
+
 /* Wait for all work of D. */
 int alis_dev_synchronize(alis_dev_world *D);
 

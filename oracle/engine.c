@@ -459,3 +459,40 @@ int alis_dev_synchronize(alis_dev_world *D) { (void)D; return 0; }
 int alis_dev_timing(alis_dev_world *D, int enable) { (void)D, (void)enable; return 0; }
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches) { (void)D, (void)h; *ms = 0, *launches = 0; return 0; }
 long alis_dev_bytes(alis_dev_world *D) { return D->bytes; }
+
+/* ---- batches: the worlds one after the other (the oracle applies strips and sources as
+ * they are recorded, so a batch half step is core of every world, then their ghosts) ---- */
+struct alis_dev_batch {
+	int n;
+	alis_dev_world **W;
+};
+
+int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out)
+{
+	if (n < 1) return fail("empty batch");
+	alis_dev_batch *B = calloc(1, sizeof *B);
+	B->n = n;
+	B->W = malloc(n * sizeof *B->W);
+	memcpy(B->W, D, n * sizeof *B->W);
+	*out = B;
+	return 0;
+}
+
+int alis_dev_batch_core(alis_dev_batch *B, int h)
+{
+	for (int a = 0; a < B->n; a++) if (alis_dev_core(B->W[a], h)) return -1;
+	return 0;
+}
+
+int alis_dev_batch_post(alis_dev_batch *B, int h)
+{
+	for (int a = 0; a < B->n; a++) if (alis_dev_ghosts(B->W[a], h)) return -1;
+	return 0;
+}
+
+void alis_dev_batch_release(alis_dev_batch *B)
+{
+	if (!B) return;
+	free(B->W);
+	free(B);
+}

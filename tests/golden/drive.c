@@ -13,8 +13,26 @@
 #include <string.h>
 
 static FILE *OUT;
-static world LAST;      /* the scenario's world and phaser array, for the raw dumps */
+static world LAST;      /* the (first) scenario's world and phaser array, for the raw dumps */
 static phaser LASTP;
+static world RUNW[8];   /* everything a run created, freed by releaseRun */
+static phaser RUNP[8];
+static int NRUN;
+
+/* Free the worlds of the last run.  Against the reference library they are left alive: its
+ * deletePhaser passes the array's end sentinel, not its start, to free() and aborts
+ * (src/phaser.c:135-138). */
+static void releaseRun(void)
+{
+#ifndef ALIS_REF
+	for (int a = 0; a < NRUN; a++) {
+		if (RUNP[a]) deletePhaser(RUNP[a]);
+		deleteWorld(RUNW[a]);
+	}
+#endif
+	NRUN = 0;
+	LAST = 0, LASTP = 0;
+}
 
 static uint64_t fnv(uint64_t h, const void *p, size_t n)
 {
@@ -217,7 +235,17 @@ static void putTrace(const char *tag, trace *T)
 
 /* ---- scenarios ---- */
 
-static void dipole(int steps)
+/* A scenario: its world, phaser array (or none) and probes. */
+typedef struct {
+	const char *name;
+	world W;
+	phaser P;
+	int np;
+	field *F[8];
+	float pts[8][3];
+	trace T;
+} scen;
+static void dipoleSetup(scen *c)
 {
 	dom D = {{60}, {60}, {60}};
 	res R = {5, {10}, {1240}};
@@ -225,21 +253,13 @@ static void dipole(int steps)
 	world W = createWorld(D, R, S, "");
 	pointDipole(W, Ez, 0, 0, 0, Pulse, 600, 300);
 	pointDipole(W, Hx, 20, -10, 30, Sine, 500, 2, 0.5);
-	dumpSetup("dipole", W);
 	field *F[6] = { Ez, Ez, Hy, Ex, Hz, Ey };
 	float pts[6][3] = {{0,0,0}, {30,-20,10}, {10,10,-40}, {-50,40,0}, {20,20,20}, {-30,0,50}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		sample(&T, W, F, pts, 6);
-	}
-	putTrace("dipole", &T);
-	dumpState("dipole", W);
-	LAST = W;
+	c->W = W, c->P = 0, c->np = 6;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void symband(int steps)
+static void symbandSetup(scen *c)
 {
 	dom D = {{150}, {150}, {150}};
 	res R = {10, {20}, {1240}, {{2, 60}, {2, 60}, {2, 60}}};
@@ -247,24 +267,13 @@ static void symband(int steps)
 	world W = createWorld(D, R, S, "");
 	pointDipole(W, Ex, 0, 0, 0, Band, 400, 800);
 	phaser P = createPhasers(W, 400, 800, 100);
-	dumpSetup("symband", W);
 	field *F[4] = { Ex, Ex, Hy, Ez };
 	float pts[4][3] = {{0,0,100}, {-40,60,0}, {20,-20,80}, {100,0,0}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		updatePhaser(W, P);
-		sample(&T, W, F, pts, 4);
-	}
-	sync(W);
-	putTrace("symband", &T);
-	dumpState("symband", W);
-	dumpPhasers("symband", P, 1);
-	LAST = W, LASTP = P;
+	c->W = W, c->P = P, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void nanoblock(int steps)
+static void nanoblockSetup(scen *c)
 {
 	dom DTF = {{36}, {36}, {36}};
 	dom DSF = {{48}, {48}, {48}};
@@ -277,24 +286,13 @@ static void nanoblock(int steps)
 	putObjects(W, Gold, Block, n(1.5), Substrate, Air);
 	planewave(W, DTF, Ppol, 0, 0, Pulse, 600, 400);
 	phaser P = createPhasers(W, 500, 700, 100, -42, 42, -42, 42, -42, 42);
-	dumpSetup("nanoblock", W);
 	field *F[5] = { Ex, Ex, Hy, Ez, Ey };
 	float pts[5][3] = {{0,0,12}, {0,0,0}, {10,-6,20}, {-20,4,-20}, {40,40,40}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		updatePhaser(W, P);
-		sample(&T, W, F, pts, 5);
-	}
-	sync(W);
-	putTrace("nanoblock", &T);
-	dumpState("nanoblock", W);
-	dumpPhasers("nanoblock", P, 1);
-	LAST = W, LASTP = P;
+	c->W = W, c->P = P, c->np = 5;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void oblique(int steps)
+static void obliqueSetup(scen *c)
 {
 	dom DTF = {{50}, {50}, {30}};
 	dom DSF = {{50}, {50}, {50}};
@@ -306,24 +304,13 @@ static void oblique(int steps)
 	putObjects(W, n(2), Block, n(1.5), Substrate, Air);
 	planewave(W, DTF, Ppol, 35, 0, Sine, 530, 3);
 	phaser P = createPhaser(W, 530, -300, 300, -300, 300);
-	dumpSetup("oblique", W);
 	field *F[4] = { Ex, iEx, Hy, iHz };
 	float pts[4][3] = {{0,0,40}, {20,10,0}, {-30,0,-40}, {10,-20,20}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		updatePhaser(W, P);
-		sample(&T, W, F, pts, 4);
-	}
-	sync(W);
-	putTrace("oblique", &T);
-	dumpState("oblique", W);
-	dumpPhasers("oblique", P, 1);
-	LAST = W, LASTP = P;
+	c->W = W, c->P = P, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void periodic(int steps)
+static void periodicSetup(scen *c)
 {
 	dom D = {{40}, {40}, {-40, 80}};
 	res R = {5, {10}, {1240}};
@@ -335,21 +322,13 @@ static void periodic(int steps)
 	object Layer = {Box, {{-INF, INF}, {-INF, INF}, {10, 20}}};
 	putObjects(W, InGaN, Layer, n(2.46), Slab, Ag, Mirror, n(1.4));
 	pointDipole(W, Ex, 0, 0, 20, Pulse, 450, 100);
-	dumpSetup("periodic", W);
 	field *F[4] = { Ex, Ey, Hy, Hz };
 	float pts[4][3] = {{0,0,20}, {20,10,40}, {-10,30,60}, {30,-30,5}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		sample(&T, W, F, pts, 4);
-	}
-	putTrace("periodic", &T);
-	dumpState("periodic", W);
-	LAST = W;
+	c->W = W, c->P = 0, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void hexagonal(int steps)
+static void hexagonalSetup(scen *c)
 {
 	float kx = 0.2, ky = 1;
 	dom D = {{50}, {0, 50*sqrtf(3)}, {60}};
@@ -362,21 +341,13 @@ static void hexagonal(int steps)
 	}};
 	putObjects(W, n(3.5), Slab, Air);
 	pointDipole(W, iHz, 20, 10*sqrtf(3), 0, Band, 750, 2000);
-	dumpSetup("hexagonal", W);
 	field *F[3] = { Hz, iHz, Ex };
 	float pts[3][3] = {{20, 0, 0}, {-20, 30, 10}, {0, 60, 30}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		sample(&T, W, F, pts, 3);
-	}
-	putTrace("hexagonal", &T);
-	dumpState("hexagonal", W);
-	LAST = W;
+	c->W = W, c->P = 0, c->np = 3;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
 }
-
-static void multipole(int steps)
+static void multipoleSetup(scen *c)
 {
 	dom D = {{50}, {50}, {50}};
 	res R = {2.5, {5}, {1240}};
@@ -390,45 +361,79 @@ static void multipole(int steps)
 	putObjects(W, Au, A, Ag, B, Si, C, Aniso, Dd, Air);
 	pointDipole(W, Ey, 0, 0, 0, Pulse, 500, 300);
 	pointDipole(W, Ez, 10, -10, 5, Pulse, 550, 200, 2, 45);
-	dumpSetup("multipole", W);
 	field *F[4] = { Ey, Ez, Hx, Jy };
 	float pts[4][3] = {{0,0,0}, {10,-10,5}, {-20,20,10}, {15,5,0}};
-	trace T = {0};
-	for (int n = 1; n <= steps; n++) {
-		updateE(W);
-		updateH(W);
-		sample(&T, W, F, pts, 4);
+	c->W = W, c->P = 0, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
+}
+static int setupScenario(scen *c, const char *name)
+{
+	memset(c, 0, sizeof *c);
+	c->name = name;
+	if (!strcmp(name, "dipole")) return dipoleSetup(c), 0;
+	if (!strcmp(name, "symband")) return symbandSetup(c), 0;
+	if (!strcmp(name, "nanoblock")) return nanoblockSetup(c), 0;
+	if (!strcmp(name, "oblique")) return obliqueSetup(c), 0;
+	if (!strcmp(name, "periodic")) return periodicSetup(c), 0;
+	if (!strcmp(name, "hexagonal")) return hexagonalSetup(c), 0;
+	if (!strcmp(name, "multipole")) return multipoleSetup(c), 0;
+	return fprintf(stderr, "unknown scenario %s\n", name), 2;
+}
+
+static void stepScenario(scen *c)
+{
+	if (c->P) updatePhaser(c->W, c->P);
+	sample(&c->T, c->W, c->F, c->pts, c->np);
+}
+
+static void dumpScenario(scen *c)
+{
+	if (c->P) sync(c->W);
+	putTrace(c->name, &c->T);
+	dumpState(c->name, c->W);
+	if (c->P) dumpPhasers(c->name, c->P, 1);
+}
+
+/* Run the scenarios of a comma-separated list.  One scenario steps with updateE/updateH;
+ * several step together with updateWorldsE/updateWorldsH (shared launches). */
+static int run(const char *names, int steps)
+{
+	scen c[8];
+	int n = 0;
+	char buf[256];
+	snprintf(buf, sizeof buf, "%s", names);
+	for (char *tok = strtok(buf, ","); tok; tok = strtok(0, ",")) {
+		if (n == 8) return fprintf(stderr, "too many scenarios\n"), 2;
+		if (setupScenario(&c[n], tok)) return 2;
+		static char label[8][32];
+		snprintf(label[n], sizeof label[n], "%s", tok);
+		c[n].name = label[n];
+		dumpSetup(c[n].name, c[n].W);
+		n++;
 	}
-	putTrace("multipole", &T);
-	dumpState("multipole", W);
-	LAST = W;
-}
-
-static int run(const char *name, int steps)
-{
-	if (!strcmp(name, "dipole")) dipole(steps);
-	else if (!strcmp(name, "symband")) symband(steps);
-	else if (!strcmp(name, "nanoblock")) nanoblock(steps);
-	else if (!strcmp(name, "oblique")) oblique(steps);
-	else if (!strcmp(name, "periodic")) periodic(steps);
-	else if (!strcmp(name, "hexagonal")) hexagonal(steps);
-	else if (!strcmp(name, "multipole")) multipole(steps);
-	else return fprintf(stderr, "unknown scenario %s\n", name), 2;
-	return 0;
-}
-
-/* Free the scenario's world.  Against the reference library the objects are left alive:
- * its deletePhaser passes the array's end sentinel, not its start, to free() and aborts
- * (src/phaser.c:135-138). */
-static void release(void)
-{
-#ifdef ALIS_REF
-	LAST = 0, LASTP = 0;
-	return;
+	world Ws[8];
+	for (int a = 0; a < n; a++) Ws[a] = c[a].W;
+	for (int s = 1; s <= steps; s++) {
+		if (n == 1) {
+			updateE(Ws[0]);
+			updateH(Ws[0]);
+		}
+		else {
+#ifndef ALIS_REF
+			updateWorldsE(Ws, n);
+			updateWorldsH(Ws, n);
+#else
+			for (int a = 0; a < n; a++) updateE(Ws[a]), updateH(Ws[a]);
 #endif
-	if (LASTP) deletePhaser(LASTP);
-	if (LAST) deleteWorld(LAST);
-	LAST = 0, LASTP = 0;
+		}
+		for (int a = 0; a < n; a++) stepScenario(&c[a]);
+	}
+	for (int a = 0; a < n; a++) dumpScenario(&c[a]);
+	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P;
+	NRUN = n;
+	LAST = c[0].W, LASTP = c[0].P;
+	return 0;
 }
 
 int alis_drive(const char *name, int steps, const char *path)
@@ -438,7 +443,7 @@ int alis_drive(const char *name, int steps, const char *path)
 	int rc = run(name, steps);
 	fclose(OUT);
 	OUT = 0;
-	release();
+	releaseRun();
 	return rc;
 }
 
@@ -484,7 +489,7 @@ int alis_drive_raw(const char *name, int steps, const char *path)
 		free(pb);
 	}
 	fclose(f);
-	release();
+	releaseRun();
 	return rc;
 }
 

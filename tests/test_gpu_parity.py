@@ -113,3 +113,25 @@ def test_full_size_fp64_tracks_fp32():
     s32 = _gpu_checksum("nanoblock", 256, 32, 300)
     s64 = _gpu_checksum("nanoblock", 256, 64, 300)
     assert s32 > 0 and abs(s64 - s32) <= 1e-4 * s32
+
+
+def _split(got, names):
+    return {n: {k: v for k, v in got.items() if k.split(".")[0].split("#")[0] == n} for n in names}
+
+
+@pytest.mark.parametrize("precision", ["ref", "fp32"])
+def test_batch_equals_alone(precision):
+    """Continuous batching's invariant: every scenario stepped in one batch with all the
+    others (shared curl and post launches, TF/SF auxiliary worlds included) gives exactly
+    its result alone -- the reference's fixture in float accumulation, and the same bits as
+    its solo GPU run in the default precision."""
+    names = sorted(SCENARIOS)
+    steps = SCENARIOS[names[0]]
+    print(f"[gpu] batch of {len(names)} scenarios, {precision}", flush=True)
+    mix = _split(run_scenario("hip", ",".join(names), steps, precision), names)
+    for n in names:
+        alone = run_scenario("hip", n, steps, precision)
+        assert mix[n] == alone, n
+        if precision == "ref":
+            bad = compare(golden(n), mix[n])
+            assert not bad, n + ": " + "\n".join(bad[:10])

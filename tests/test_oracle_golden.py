@@ -32,3 +32,17 @@ def test_fixture_catches_planted_mistake(tmp_path):
     h, s = got[key]
     got[key] = [h, s[:-1] + ("1" if s[-1] != "1" else "2")]
     assert compare(ref, got)
+
+
+def _split(got, names):
+    return {n: {k: v for k, v in got.items() if k.split(".")[0].split("#")[0] == n} for n in names}
+
+
+def test_oracle_batch_matches_reference():
+    """All scenarios stepped together (updateWorldsE/updateWorldsH over the batch ABI) give
+    each scenario's own reference output."""
+    names = sorted(SCENARIOS)
+    got = _split(run_scenario("oracle", ",".join(names), SCENARIOS[names[0]], "ref"), names)
+    for n in names:
+        bad = compare(golden(n), got[n])
+        assert not bad, n + ": " + "\n".join(bad[:10])
