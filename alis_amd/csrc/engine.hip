@@ -119,6 +119,9 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // dealt out per XCD (see xcdTile), otherwise grid (k tiles, y tiles, x planes).
 __device__ inline int xcdTile(int wid, int nwg);
 
+template<class T, bool HASC, bool NT = true, bool NTG = false>
+__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k);
+
 template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4>
 __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, int bz)
 {
@@ -126,6 +129,13 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
+	curlAt<T, HASC, NT, NTG>(a, i, j, k);
+}
+
+// The update of cell (i, j, k).
+template<class T, bool HASC, bool NT, bool NTG>
+__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
+{
 
 	const long id = a.L.at(i, j, k);
 	const long oI = a.alt * a.L.si, oJ = a.alt * a.L.sj, oK = a.alt;
@@ -240,23 +250,36 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	curlCell<T, HASC, NT, NTG, BY>(a, bx, by, bz);
 }
 
-// Several worlds (and real/imaginary parts) in one launch: entry e owns workgroups
-// [start[e], start[e+1]) of a one-dimensional grid, tiled like k_curl.
-template<class T>
-__global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *args, const int *start, int n)
+// The entry owning workgroup b of a batch launch: start[] is increasing; every wave finds it
+// with one load per lane and a ballot (no chain of dependent loads).
+__device__ __forceinline__ int batchEntry(const int *start, int n, int b)
 {
-	int lo = 0, hi = n - 1;
-	const int b = (int)blockIdx.x;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) / 2;
-		if (start[mid] <= b) lo = mid;
-		else hi = mid - 1;
+	int e = 0;
+	for (int c = 0; c < n; c += 64) {
+		const int q = c + (int)__lane_id();
+		const unsigned long long m = __ballot(q < n && start[q] <= b);
+		if (!m) break;
+		e = c + __popcll(m) - 1;
+		if (__popcll(m) < 64) break;
 	}
-	const CurlArgs<T> &a = args[lo];
-	const int t = b - start[lo], nkb = (a.nk + 63) / 64, njb = (a.nj + 3) / 4;
-	const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
-	if (a.C0) curlCell<T, true>(a, bx, by, bz);
-	else curlCell<T, false>(a, bx, by, bz);
+	return __builtin_amdgcn_readfirstlane(e);
+}
+
+// Several worlds (and real/imaginary parts) in one launch: entry e owns workgroups
+// [start[e], start[e+1]) of a one-dimensional grid; each workgroup updates TPB consecutive
+// tiles of its entry (tiles as in k_curl).
+template<class T, int TPB>
+__global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
+{
+	const int b = (int)blockIdx.x;
+	const int e = batchEntry(start, n, b);
+	const CurlArgs<T> &a = args[e];
+	const int nkb = (a.nk + 63) / 64, njb = (a.nj + 3) / 4, tiles = nkb * njb * a.ni;
+	for (int t = (b - start[e]) * TPB, t1 = min(t + TPB, tiles); t < t1; t++) {
+		const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
+		if (a.C0) curlCell<T, true>(a, bx, by, bz);
+		else curlCell<T, false>(a, bx, by, bz);
+	}
 }
 
 // Workgroup order for the 8 XCDs of MI355X: the dispatcher deals consecutive workgroup ids
@@ -307,6 +330,7 @@ template<class T> struct PostOp {
 constexpr int MAXSRC = 128;     // sources per chain (E or H) of one world
 constexpr int MAXCHAIN = 24;    // copies per launch
 constexpr int MAXPOST = 48;     // operations per launch (staged in LDS)
+constexpr int CURL_TPB = 2;     // tiles per workgroup of a batched curl launch (measured best of 1..16)
 
 template<class T> struct PostArgs {
 	const PostOp<T> *ops;
@@ -427,17 +451,12 @@ __global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
 
 // The programs of several worlds in one launch: entry e owns workgroups [start[e], start[e+1]).
 template<class T>
-__global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *args, const long *start, int n)
+__global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
 {
 	__shared__ PostOp<T> ops[MAXPOST];
-	int lo = 0, hi = n - 1;
-	const long b = (long)blockIdx.x;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) / 2;
-		if (start[mid] <= b) lo = mid;
-		else hi = mid - 1;
-	}
-	postBlock(args[lo], ops, b - start[lo]);
+	const int b = (int)blockIdx.x;
+	const int e = batchEntry(start, n, b);
+	postBlock(args[e], ops, (long)(b - start[e]));
 }
 
 // Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
@@ -1451,7 +1470,7 @@ int batchBuildCurl(alis_dev_batch *B)
 			for (int m = 0; m <= D->cplx; m++) {
 				args.push_back(curlArgs<T>(D, h, m));
 				start.push_back(blocks);
-				blocks += ((D->kNUM + 63) / 64) * ((D->jNUM + 3) / 4) * D->iNUM;
+				blocks += (((D->kNUM + 63) / 64) * ((D->jNUM + 3) / 4) * D->iNUM + CURL_TPB - 1) / CURL_TPB;
 			}
 		B->curlN[h] = (int)args.size(), B->curlBlocks[h] = blocks;
 		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
@@ -1474,22 +1493,22 @@ int batchBuildPost(alis_dev_batch *B)
 		B->rounds[h].assign(rounds, alis_dev_batch::Round{});
 		for (size_t r = 0; r < rounds; r++) {
 			alis_dev_batch::Round &R = B->rounds[h][r];
-			std::vector<long> start;
+			std::vector<int> start;
 			for (int w = 0; w < (int)B->W.size(); w++) {
 				alis_dev_world *D = B->W[w];
 				if (r >= D->post[h].size() || !D->post[h][r].total) continue;
 				R.world.push_back(w), R.seg.push_back((int)r);
-				start.push_back(R.blocks);
+				start.push_back((int)R.blocks);
 				R.blocks += D->post[h][r].total / 256;
 			}
 			if (R.world.empty()) continue;
 			const size_t bytes = R.world.size() * sizeof(PostArgs<T>);
 			if (batchAlloc(B, &R.args, bytes)) return -1;
-			if (batchAlloc(B, &R.start, start.size() * sizeof(long))) return -1;
+			if (batchAlloc(B, &R.start, start.size() * sizeof(int))) return -1;
 			HIPCHECK(hipHostMalloc(&R.pinned, bytes, hipHostMallocDefault));
 			HIPCHECK(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
 			HIPCHECK(hipEventRecord(R.done, B->st));
-			HIPCHECK(hipMemcpyAsync(R.start, start.data(), start.size() * sizeof(long), hipMemcpyHostToDevice, B->st));
+			HIPCHECK(hipMemcpyAsync(R.start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, B->st));
 		}
 	}
 	HIPCHECK(hipStreamSynchronize(B->st));
@@ -1507,7 +1526,7 @@ int batchCore(alis_dev_batch *B, int h)
 		HIPCHECK(hipEventCreate(&e1));
 		HIPCHECK(hipEventRecord(e0, B->st));
 	}
-	k_curl_batch<T><<<B->curlBlocks[h], dim3(64, 4), 0, B->st>>>((const CurlArgs<T> *)B->curl[h], (const int *)B->curlStart[h], B->curlN[h]);
+	k_curl_batch<T, CURL_TPB><<<B->curlBlocks[h], dim3(64, 4), 0, B->st>>>((const CurlArgs<T> *)B->curl[h], (const int *)B->curlStart[h], B->curlN[h]);
 	HIPCHECK(hipGetLastError());
 	if (e1) {
 		HIPCHECK(hipEventRecord(e1, B->st));
@@ -1537,7 +1556,7 @@ int batchPost(alis_dev_batch *B, int h)
 		}
 		HIPCHECK(hipMemcpyAsync(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), hipMemcpyHostToDevice, B->st));
 		HIPCHECK(hipEventRecord(R.done, B->st));
-		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const long *)R.start, (int)R.world.size());
+		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
 		HIPCHECK(hipGetLastError());
 	}
 	for (alis_dev_world *D : B->W) {

@@ -9,35 +9,48 @@
 #include "internal.h"
 
 static world *liveWorlds;
-static alis_dev_batch *cachedBatch;      /* the last batch of updateWorldsE/H and its worlds */
-static alis_dev_world **cachedList;
-static int cachedN;
+/* Batches of updateWorldsE/H (and of updateE/H on worlds with TF/SF boxes), most recently
+ * used first, keyed by their device worlds. */
+#define NBATCH 8
+static struct { alis_dev_batch *B; alis_dev_world **D; int n; } batches[NBATCH];
 
-/* Drop the cached batch: called whenever a device world goes away, so that a later world at
- * a reused address can never meet tables of a freed one. */
+/* Drop every cached batch: called whenever a device world goes away, so that a later world
+ * at a reused address can never meet tables of a freed one. */
 static void dropBatch(void)
 {
-	if (cachedBatch) alis_dev_batch_release(cachedBatch);
-	cachedBatch = 0;
-	free(cachedList);
-	cachedList = 0;
-	cachedN = 0;
+	for (int a = 0; a < NBATCH; a++) {
+		if (batches[a].B) alis_dev_batch_release(batches[a].B);
+		free(batches[a].D);
+		batches[a].B = 0, batches[a].D = 0, batches[a].n = 0;
+	}
 }
 
-/* The batch of the committed worlds L[0..n-1] (created on first use, reused while the list
- * stays the same). */
+/* The batch of the committed worlds L[0..n-1] (created on first use, then reused). */
 alis_dev_batch *alisBatch(world *L, int n)
 {
-	int same = cachedBatch && cachedN == n;
-	for (int a = 0; same && a < n; a++) same = cachedList[a] == L[a]->dev;
-	if (same) return cachedBatch;
-	dropBatch();
-	alis_dev_world **D = malloc(n * sizeof *D);
-	for (int a = 0; a < n; a++) D[a] = L[a]->dev;
-	alisFatal("batch", alis_dev_batch_create(D, n, &cachedBatch));
-	cachedList = D, cachedN = n;
-	return cachedBatch;
+	int hit = -1;
+	for (int a = 0; a < NBATCH && hit < 0; a++) {
+		if (!batches[a].B || batches[a].n != n) continue;
+		int same = 1;
+		for (int b = 0; same && b < n; b++) same = batches[a].D[b] == L[b]->dev;
+		if (same) hit = a;
+	}
+	if (hit < 0) {
+		if (batches[NBATCH-1].B) alis_dev_batch_release(batches[NBATCH-1].B), free(batches[NBATCH-1].D);
+		hit = NBATCH - 1;
+		batches[hit].D = malloc(n * sizeof(alis_dev_world *));
+		for (int b = 0; b < n; b++) batches[hit].D[b] = L[b]->dev;
+		batches[hit].n = n;
+		alisFatal("batch", alis_dev_batch_create(batches[hit].D, n, &batches[hit].B));
+	}
+	if (hit) {
+		typeof(batches[0]) e = batches[hit];
+		memmove(&batches[1], &batches[0], hit * sizeof batches[0]);
+		batches[0] = e;
+	}
+	return batches[0].B;
 }
+
 static int nWorlds, capWorlds;
 static phaser *livePhasers;
 static int nPhasers, capPhasers;

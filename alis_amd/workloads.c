@@ -10,12 +10,20 @@
  *              is the grid edge including CPML (256 for the 256^3 workload).
  *   dipole     Ez point dipole (600 nm pulse) in vacuum: dx = 10 nm, Courant 0.5, 10-cell
  *              CPML, 8 fixed probes; `cells` = grid edge including CPML (64).
+ *   kpoints    a band sweep: `cells` worlds (k-points along Gamma-X-M), each a photonic
+ *              crystal slab cell of 64 x 64 x 31 cells plus an 8-cell CPML above and below
+ *              (dx = 10 nm; Bloch-periodic x and y with period 640 nm = 2a, a = 320 nm; an
+ *              eps = 12 slab 0.5a thick with air holes of radius 0.3a on the square lattice
+ *              of period a; complex fields), 8 point dipoles per world at positions and
+ *              phases drawn from a splitmix64 stream (seed 2024).  All worlds step together
+ *              with updateWorldsE/updateWorldsH.
  *
  * One step is updateE + updateH (+ updatePhaser for nanoblock), src/update.c:275-341.
  */
 #include "alis.h"
 #include <string.h>
 #include <time.h>
+#include <stdint.h>
 
 typedef struct {
 	world W;
@@ -23,7 +31,43 @@ typedef struct {
 	int probes;
 	float pts[8][3];
 	double probeSum;
+	int nw;                     /* worlds stepped together (W = Ws[0]) */
+	world *Ws;
 } wl;
+
+static uint64_t splitmix(uint64_t *s)
+{
+	uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+	return z ^ (z >> 31);
+}
+
+static double uniform(uint64_t *s) { return (splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* One k-point cell of the band sweep: q-th of nq points along Gamma-X-M. */
+static world kpointWorld(int q, int nq, uint64_t *seed)
+{
+	const float a = 320, L = 2 * a;
+	float t = (q + 0.5f) / nq * 2, kx, ky;                 /* path parameter 0..2 */
+	if (t < 1) kx = 0.5f * t, ky = 0;
+	else kx = 0.5f, ky = 0.5f * (t - 1);
+	dom D = {{-L/2, L/2}, {-L/2, L/2}, {-150, 160}};     /* period L: node iMAX is node iMIN */
+	res R = {5, {10}, {1240}};
+	sur S = {{BBC, ky >= 0 && kx > 0 ? a/kx : INF}, {ky > 0 ? BBC : PBC, ky > 0 ? a/ky : INF}, {PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	object Slab = {Difference, {2}, objects {
+		{Box, {{-INF, INF}, {-INF, INF}, {-80, 80}}},
+		{Lattice, {{a}, {a}}, objects {{RodZ, {{-96, 96}, {-96, 96}, {-INF, INF}}}}},
+	}};
+	putObjects(W, n(sqrtf(12)), Slab, Air);
+	for (int d = 0; d < 8; d++) {
+		float x = (float)(uniform(seed) * (L - 40) - L/2 + 20), y = (float)(uniform(seed) * (L - 40) - L/2 + 20);
+		float z = (float)(uniform(seed) * 120 - 60), ph = (float)(uniform(seed) * 360);
+		pointDipole(W, d % 2 ? Ez : Hz, x, y, z, Band, 1000, 2500, 1, ph);
+	}
+	return W;
+}
 
 /* Domain extents giving `cells` grid nodes including two CPML layers of pml+1 cells:
  * nodes iMin..iMax = -(d-2)/2 .. d/2 with d = cells - 2(pml+1), src/world.c:229-234. */
@@ -64,12 +108,20 @@ void *wl_create(const char *name, int cells, int fieldBits, int dftBits)
 		memcpy(c->pts, pts, sizeof pts);
 		c->probes = 8;
 	}
+	else if (!strcmp(name, "kpoints")) {
+		uint64_t seed = 2024;
+		c->nw = cells;
+		c->Ws = calloc(cells, sizeof(world));
+		for (int q = 0; q < cells; q++) c->Ws[q] = kpointWorld(q, cells, &seed);
+		c->W = c->Ws[0];
+	}
 	else {
 		free(c);
 		return 0;
 	}
+	if (!c->nw) c->nw = 1, c->Ws = calloc(1, sizeof(world)), c->Ws[0] = c->W;
 #ifndef ALIS_REF
-	worldPrecision(c->W, fieldBits, dftBits);
+	for (int q = 0; q < c->nw; q++) worldPrecision(c->Ws[q], fieldBits, dftBits);
 #else
 	(void)fieldBits, (void)dftBits;
 #endif
@@ -80,8 +132,18 @@ void wl_step(void *ctx, int n)
 {
 	wl *c = ctx;
 	for (int s = 0; s < n; s++) {
-		updateE(c->W);
-		updateH(c->W);
+		if (c->nw > 1) {
+#ifndef ALIS_REF
+			updateWorldsE(c->Ws, c->nw);
+			updateWorldsH(c->Ws, c->nw);
+#else
+			for (int q = 0; q < c->nw; q++) updateE(c->Ws[q]), updateH(c->Ws[q]);
+#endif
+		}
+		else {
+			updateE(c->W);
+			updateH(c->W);
+		}
 		if (c->P) updatePhaser(c->W, c->P);
 		for (int p = 0; p < c->probes; p++) {
 			c->probeSum += get(c->W, Ez, c->pts[p][0], c->pts[p][1], c->pts[p][2]);
@@ -90,11 +152,22 @@ void wl_step(void *ctx, int n)
 	}
 }
 
-/* Grid nodes of the main world including CPML (the cell count of one step). */
+/* Grid nodes of the stepped worlds including CPML (the cell count of one step; a complex
+ * cell of a Bloch world counts once). */
 long wl_cells(void *ctx)
 {
 	wl *c = ctx;
-	return (long)c->W->iNUM * c->W->jNUM * c->W->kNUM;
+	long n = 0;
+	for (int q = 0; q < c->nw; q++) n += (long)c->Ws[q]->iNUM * c->Ws[q]->jNUM * c->Ws[q]->kNUM;
+	return n;
+}
+
+/* Worlds stepped together and whether they carry complex (Bloch) fields. */
+int wl_worlds(void *ctx, int *complexField)
+{
+	wl *c = ctx;
+	*complexField = c->W->complexField;
+	return c->nw;
 }
 
 /* Phaser frequencies and surface cells (for the DFT byte count). */
@@ -132,16 +205,22 @@ double wl_checksum(void *ctx)
 {
 	wl *c = ctx;
 #ifndef ALIS_REF
-	syncWorld(c->W);
 #endif
 	double s = c->probeSum;
-	world W = c->W;
-	float ***F[6] = { W->E->x, W->E->y, W->E->z, W->H->x, W->H->y, W->H->z };
-	for (int a = 0; a < 6; a++)
-		for (int i = W->iMIN; i <= W->iMAX; i++)
-			for (int j = W->jMIN; j <= W->jMAX; j++)
-				for (int k = W->kMIN; k <= W->kMAX; k++)
-					s += (double)F[a][i][j][k] * F[a][i][j][k];
+	for (int q = 0; q < c->nw; q++) {
+		world W = c->Ws[q];
+#ifndef ALIS_REF
+		syncWorld(W);
+#endif
+		for (int m = 0; m <= W->complexField; m++) {
+			float ***F[6] = { W->E[m].x, W->E[m].y, W->E[m].z, W->H[m].x, W->H[m].y, W->H[m].z };
+			for (int a = 0; a < 6; a++)
+				for (int i = W->iMIN; i <= W->iMAX; i++)
+					for (int j = W->jMIN; j <= W->jMAX; j++)
+						for (int k = W->kMIN; k <= W->kMAX; k++)
+							s += (double)F[a][i][j][k] * F[a][i][j][k];
+		}
+	}
 	return s;
 }
 
@@ -149,7 +228,8 @@ void wl_free(void *ctx)
 {
 	wl *c = ctx;
 	if (c->P) deletePhaser(c->P);
-	deleteWorld(c->W);
+	for (int q = 0; q < c->nw; q++) deleteWorld(c->Ws[q]);
+	free(c->Ws);
 	free(c);
 }
 
@@ -158,6 +238,13 @@ void wl_free(void *ctx)
 void *wl_device(void *ctx)
 {
 	return ((wl *)ctx)->W->dev;
+}
+
+/* Device world of the q-th world stepped together. */
+void *wl_device_of(void *ctx, int q)
+{
+	wl *c = ctx;
+	return q >= 0 && q < c->nw ? c->Ws[q]->dev : 0;
 }
 #endif
 

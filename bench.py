@@ -64,6 +64,7 @@ def load():
     w.wl_free.argtypes = [ctypes.c_void_p]
     w.wl_checksum.restype = ctypes.c_double
     w.wl_checksum.argtypes = [ctypes.c_void_p]
+    w.wl_worlds.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     alis.alis_dev_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     alis.alis_dev_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
     alis.alis_dev_synchronize.argtypes = [ctypes.c_void_p]
@@ -113,14 +114,18 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps):
     nobj = w.wl_geometry(ctx, num, pml)
     surf = ctypes.c_long()
     nf = w.wl_phaser_info(ctx, ctypes.byref(surf))
+    cplx = ctypes.c_int()
+    nw = w.wl_worlds(ctx, ctypes.byref(cplx))
     out = {"cells": w.wl_cells(ctx), "sec": t1 - t0, "kern": kern, "num": list(num), "pml": list(pml),
-           "nobj": nobj, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev)}
+           "nobj": nobj, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev),
+           "worlds": nw, "parts": 1 + cplx.value}
     w.wl_free(ctx)
     return out
 
 
 def roofline(r, s, dft_s, steps):
     be, bh = curl_bytes(r["num"], r["pml"], s, r["nobj"] > 0)
+    be, bh = be * r["worlds"] * r["parts"], bh * r["worlds"] * r["parts"]    # one launch covers all
     (mse, ne), (msh, nh) = r["kern"]
     te, th = mse / max(ne, 1) * 1e-3, msh / max(nh, 1) * 1e-3
     dom = ("E", be, te, ne) if mse >= msh else ("H", bh, th, nh)
@@ -197,6 +202,7 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kpoints", type=int, default=64, help="worlds of the batched k-point sweep (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a copy of known size that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -230,6 +236,24 @@ def main():
         v64 = r64["cells"] * max(args.steps // 2, 1) / r64["sec"] / 1e6
         line["fp64"] = {"value": round(v64, 1), "ms_per_step": round(r64["sec"] / max(args.steps // 2, 1) * 1e3, 4),
                         "roofline_frac": roofline(r64, 8, 8, max(args.steps // 2, 1))["frac"], "fp32_over_fp64": round(value / v64, 2)}
+    if args.kpoints > 0:
+        ksteps = max(args.steps // 2, 1)
+        with quiet():
+            rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
+            r1 = run(alis, w, hip, "kpoints", 1, 64, 64, args.warmup, ksteps)
+        vb = rb["cells"] * ksteps / rb["sec"] / 1e6
+        v1 = r1["cells"] * ksteps / r1["sec"] / 1e6
+        rlb = roofline(rb, 4, 8, ksteps)
+        line["batched"] = {
+            "workload": f"{args.kpoints} k-point worlds along Gamma-X-M, each 65x65x50 (64x64x31 cells + 8-cell CPML in z), "
+                        "Bloch-periodic x/y, complex fields, eps=12 PhC slab with air holes, 8 random-phase dipoles",
+            "value": round(vb, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rb["sec"] / ksteps * 1e3, 4),
+            "roofline": {"bound": "hbm", "achieved": rlb["achieved"], "peak": rlb["peak"], "unit": "GB/s", "frac": rlb["frac"],
+                         "kernel": "k_curl_batch (E half step of all worlds, both parts)", "bytes_per_launch": rlb["bytes_per_launch"],
+                         "launch_ms": rlb["launch_ms"]},
+            "fp64_single_world": {"value": round(v1, 1), "ms_per_step": round(r1["sec"] / ksteps * 1e3, 4)},
+            "batched_fp32_over_single_fp64": round(vb / v1, 2),
+        }
     line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(line), flush=True)
 
