@@ -629,6 +629,7 @@ struct DevPhaser {
 	long ncell = 0;
 	void *acc = nullptr;        // TA
 	float *stage = nullptr;     // device float buffer for host syncs
+	size_t bytes = 0;           // device bytes of acc and stage
 };
 
 struct alis_dev_world {
@@ -700,9 +701,11 @@ int ensureScratch(alis_dev_world *D, long elems)
 	if (D->scratch) {
 		HIPCHECK(hipStreamSynchronize(D->st));
 		HIPCHECK(hipFree(D->scratch));
+		D->bytes -= D->scratchElems * sizeof(double);
 	}
 	HIPCHECK(hipMalloc(&D->scratch, elems * sizeof(double)));
 	D->scratchElems = elems;
+	D->bytes += elems * sizeof(double);
 	return 0;
 }
 
@@ -1719,6 +1722,8 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 	const size_t accBytes = (size_t)std::max(n, 1L) * (D->dft64 ? 8 : 4);
 	HIPCHECK(hipMalloc(&Q.acc, accBytes));
 	HIPCHECK(hipMalloc((void **)&Q.stage, (size_t)std::max(n, 1L) * sizeof(float)));
+	Q.bytes = accBytes + (size_t)std::max(n, 1L) * sizeof(float);
+	D->bytes += Q.bytes;
 	// start from the host accumulators (normally zero)
 	std::vector<float> host(std::max(n, 1L));
 	for (int q = 0; q < Q.nf; q++)
@@ -1801,6 +1806,7 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 	HIPCHECK(hipStreamSynchronize(D->st));
 	HIPCHECK(hipFree(P.acc));
 	HIPCHECK(hipFree(P.stage));
+	D->bytes -= P.bytes;
 	P.live = false;
 	return 0;
 }
@@ -1824,22 +1830,35 @@ int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out)
 	return 0;
 }
 
+// The worlds of a batch may have joined another stream since the batch was built (a later
+// batch led by another world): bring them back onto the first world's stream, which the
+// batch launches on.  joinStream waits for the work on the stream a world leaves.
+static int batchStream(alis_dev_batch *B)
+{
+	for (alis_dev_world *W : B->W)
+		if (W->st != B->W[0]->st && joinStream(W, B->W[0])) return -1;
+	B->st = B->W[0]->st;
+	return 0;
+}
+
 int alis_dev_batch_core(alis_dev_batch *B, int h)
 {
 	if (h < 0 || h > 1) return fail("no such half step");
+	if (batchStream(B)) return -1;
 	return B->fp64 ? batchCore<double>(B, h) : batchCore<float>(B, h);
 }
 
 int alis_dev_batch_post(alis_dev_batch *B, int h)
 {
 	if (h < 0 || h > 1) return fail("no such half step");
+	if (batchStream(B)) return -1;
 	return B->fp64 ? batchPost<double>(B, h) : batchPost<float>(B, h);
 }
 
 void alis_dev_batch_release(alis_dev_batch *B)
 {
 	if (!B) return;
-	if (B->st) hipStreamSynchronize(B->st);
+	for (alis_dev_world *W : B->W) hipStreamSynchronize(W->st);   // the batch's launches ran on one of these
 	for (void *p : B->allocs) hipFree(p);
 	for (int h = 0; h < 2; h++)
 		for (alis_dev_batch::Round &R : B->rounds[h]) {

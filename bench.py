@@ -194,6 +194,30 @@ def cpu_baseline(cells, steps):
                       f"{steps} steps after 2 warm-up steps, OMP_NUM_THREADS={threads}"}
 
 
+def serving(alis, args):
+    """The serving load: seeded Poisson arrivals of scattering requests (grid edges 64-192
+    with 8-cell CPML, a Drude-Lorentz sphere of 10-40 cells, 1000-6000 steps, 16-wavelength
+    phaser), served with continuous batching; latency = finish - arrival."""
+    sys.path.insert(0, str(ROOT))
+    from alis_amd import serve
+    os.environ["ALIS_PRECISION"] = "fp32"
+    reqs = serve.requests(args.serve_seed, args.serve, args.serve_rate, alis)
+    with quiet():
+        res = serve.run(reqs, args.serve_budget_gib * 2**30, args.serve_slots, args.serve_tick, alis)
+    s = serve.summary(reqs, res)
+    return {
+        "workload": f"{args.serve} requests, Poisson arrivals {args.serve_rate:g}/s (seed {args.serve_seed}), grid edge "
+                    "{64,96,128,192} incl. 8-cell CPML, Drude-Lorentz sphere r 10-40 cells, TF/SF plane wave "
+                    "400-900 nm, 1000-6000 steps, 16-wavelength phaser; FP32 fields / FP64 DFT",
+        "slots": args.serve_slots, "steps_per_tick": args.serve_tick, "budget_gib": args.serve_budget_gib,
+        "p50_latency_s": round(s["p50_latency_s"], 3), "p99_latency_s": round(s["p99_latency_s"], 3),
+        "p50_queue_s": round(s["p50_queue_s"], 3), "makespan_s": round(s["makespan_s"], 3),
+        "value": round(s["mcell_updates_per_s"], 1), "unit": "Mcell-updates/s",
+        "max_batch": s["max_batch"], "max_request_gib": round(s["peak_request_bytes"] / 2**30, 3),
+        "offered_gcell_updates": round(sum(r.cellSteps for r in res) / 1e9, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +227,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kpoints", type=int, default=64, help="worlds of the batched k-point sweep (0: skip)")
+    ap.add_argument("--serve", type=int, default=24, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second (Poisson)")
+    ap.add_argument("--serve-seed", type=int, default=2024)
+    ap.add_argument("--serve-slots", type=int, default=8, help="most requests on the device at once")
+    ap.add_argument("--serve-tick", type=int, default=50, help="most steps between admissions")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="device memory for admitted requests")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a copy of known size that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -254,6 +284,8 @@ def main():
             "fp64_single_world": {"value": round(v1, 1), "ms_per_step": round(r1["sec"] / ksteps * 1e3, 4)},
             "batched_fp32_over_single_fp64": round(vb / v1, 2),
         }
+    if args.serve > 0:
+        line["serving"] = serving(alis, args)
     line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(line), flush=True)
 

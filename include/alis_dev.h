@@ -144,6 +144,7 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id);
  * then the usual alis_dev_strips / alis_dev_inject records per world, then
  * alis_dev_batch_post (each world's strips, sources and ghost copies; one launch per
  * program round).  Worlds keep their own state; results equal stepping them one by one.
+ * A world may belong to several batches; release a batch before any of its worlds.
  * No counterpart in the reference, which steps one world per call (src/update.c:275-341). */
 typedef struct alis_dev_batch alis_dev_batch;
 int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out);
@@ -159,7 +160,8 @@ int alis_dev_synchronize(alis_dev_world *D);
 int alis_dev_timing(alis_dev_world *D, int enable);
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches);
 
-/* Device-resident bytes of D. */
+/* Device-resident bytes of D: fields, coefficients, CPML and ADE state, source tables,
+ * phaser accumulators and staging (not the TF/SF auxiliary worlds, which have their own). */
 long alis_dev_bytes(alis_dev_world *D);
 
 #ifdef __cplusplus

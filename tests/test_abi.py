@@ -1,6 +1,6 @@
 """The drop-in boundary: the built libraries load and export every function that
-include/alis.h (the reference's public API, src/alis_c.h:132-334) and include/alis_dev.h
-(the engine ABI) declare; the product library depends on no oracle code; without a GPU the
+include/alis.h (the reference's public API, src/alis_c.h:132-334), include/alis_dev.h
+(the engine ABI) and include/alis_serve.h (the serving loop) declare; the product library depends on no oracle code; without a GPU the
 product stops with an error instead of computing on the CPU."""
 import ctypes
 import os
@@ -47,9 +47,10 @@ def test_headers_parse():
         assert n in api, n
     assert {"alis_dev_commit", "alis_dev_core", "alis_dev_strips", "alis_dev_ghosts", "alis_dev_phaser_update"} <= dev
     assert len(api) > 150 and len(dev) >= 20
+    assert declared("alis_serve.h") == {"alisServe", "alisServeMaxRadius", "alisServeRequests"}
 
 
-@pytest.mark.parametrize("header", ["alis.h", "alis_dev.h"])
+@pytest.mark.parametrize("header", ["alis.h", "alis_dev.h", "alis_serve.h"])
 def test_library_exports_declared_symbols(header):
     lib = ctypes.CDLL(str(LIB))
     missing = sorted(n for n in declared(header) if not hasattr(lib, n))
