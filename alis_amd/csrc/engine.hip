@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cmath>
 #include <algorithm>
+#include <rocblas/rocblas.h>
 
 #include "alis_dev.h"
 
@@ -661,6 +662,9 @@ struct alis_dev_world {
 	std::vector<DevPhaser> phs;
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
+	rocblas_handle blas = nullptr;   // far-field products, created on first use
+	void *ffWork = nullptr;          // far-field workspace
+	size_t ffWorkBytes = 0;
 	void *scratch = nullptr;    // pack/unpack buffer (double-sized)
 	long scratchElems = 0;
 	std::vector<void *> allocs;
@@ -1645,6 +1649,8 @@ void alis_dev_release(alis_dev_world *D)
 	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
+	if (D->ffWork) hipFree(D->ffWork);
+	if (D->blas) rocblas_destroy_handle(D->blas);
 	if (D->st && --*D->streamRefs == 0) {
 		hipStreamDestroy(D->st);
 		delete D->streamRefs;
@@ -1808,6 +1814,263 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 	HIPCHECK(hipFree(P.stage));
 	D->bytes -= P.bytes;
 	P.live = false;
+	return 0;
+}
+
+// ---- near-to-far surface sums (src/farfld.c:248-459) ----
+//
+// For a face F[u][w] (u its first axis, w its last, k-fastest like the accumulators) and a
+// batch of columns, S(col) = sum_v alpha_v(col) sum_p F[.][.] beta_p(col): beta runs along
+// the face's in-plane phase axis p and is contracted with F by a complex GEMM (rocBLAS),
+// alpha runs along the other axis v and is applied by a reduction kernel.  z faces: p = y,
+// v = x, both phase vectors.  x faces: p = y, v = z; y faces: p = x, v = z, where alpha is
+// the plane-wave profile T(z) and depends on the polarisation.  A beta matrix is shared by
+// the two components of a block and the two faces of an orientation.
+
+namespace {
+
+struct FFPhase {               // one in-plane phase vector along an axis
+	const double *d, *pos;     // cell widths and positions (device)
+	int n, half, sym, useKy;
+	double cosk;
+};
+
+// Weight of cell p: E widths at nodes, trapezoid ends, mirror image subtracted (half = 0),
+// or H widths at half nodes, first cell left out, mirror image added (half = 1).
+__device__ inline double2 ffPhaseAt(const FFPhase &a, int p, double k)
+{
+	double w = a.half ? (p ? 1.0 : 0.0) : (p == 0 ? 0.5 : 0.0) + (p == a.n - 1 ? 0.5 : 0.0) + (p > 0 && p < a.n - 1 ? 1.0 : 0.0);
+	double sn, cs;
+	sincos(k * a.pos[p], &sn, &cs);
+	double re = a.d[p] * cs, im = a.d[p] * sn;
+	if (a.sym) {
+		const double sg = a.half ? 1.0 : -1.0;
+		const double mr = re * a.cosk, mi = -im * a.cosk;      // conj(v) cosk
+		re += sg * mr, im += sg * mi;
+	}
+	return make_double2(w * re, w * im);
+}
+
+// beta[p][col] for columns c0..c0+nc-1 (col fastest).
+__global__ void k_ff_beta(double2 *beta, FFPhase a, const alis_dev_ff_col *cols, int nc)
+{
+	const int col = blockIdx.x * blockDim.x + threadIdx.x, p = blockIdx.y;
+	if (col >= nc) return;
+	const double k = a.useKy ? cols[col].ky : cols[col].kx;
+	beta[(size_t)p * nc + col] = ffPhaseAt(a, p, k);
+}
+
+struct FFReduce {
+	const double2 *C;          // nm matrices [v][col]
+	int nm, nv, nc;
+	int out[4];                // output slot kind*4+comp of each matrix
+	int tcomp[4];              // T component of each matrix (profile weights)
+	int profile, half;         // weights: phase vector (z faces) or profile (x, y faces)
+	FFPhase ph;                // phase weights
+	const double *dz;          // profile weights: cell widths along z
+	const double2 *T;          // [slot][6][nz]
+	int nz, kLo, k0, symZ;
+	double coskz;
+};
+
+__device__ inline double2 cmad(double2 acc, double2 a, double2 b)
+{
+	acc.x += a.x * b.x - a.y * b.y;
+	acc.y += a.x * b.y + a.y * b.x;
+	return acc;
+}
+
+// S[col][kind][comp][pol] = sum_v alpha_v(col, pol) C_m[v][col].
+__global__ void k_ff_reduce(double2 *S, FFReduce r, const alis_dev_ff_col *cols)
+{
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	if (col >= r.nc) return;
+	const alis_dev_ff_col &c = cols[col];
+	if (!r.profile) {
+		double2 acc[4] = {};
+		const double k = r.ph.useKy ? c.ky : c.kx;
+		for (int v = 0; v < r.nv; v++) {
+			const double2 a = ffPhaseAt(r.ph, v, k);
+			for (int m = 0; m < r.nm; m++) acc[m] = cmad(acc[m], a, r.C[((size_t)m * r.nv + v) * r.nc + col]);
+		}
+		for (int m = 0; m < r.nm; m++) S[((size_t)col * 24 + r.out[m]) * 2] = acc[m];
+		return;
+	}
+	for (int pol = 0; pol < 2; pol++)
+		for (int m = 0; m < r.nm; m++) {
+			const int src = c.src[pol][r.tcomp[m]];
+			const double fac = c.fac[pol][r.tcomp[m]];
+			double2 acc = make_double2(0.0, 0.0);
+			if (src >= 0 && fac != 0.0) {
+				const double2 *Tb = r.T + ((size_t)c.slot[pol] * 6 + src) * r.nz - r.kLo;   // index by absolute k
+				for (int v = 0; v < r.nv; v++) {
+					if (r.half && !v) continue;
+					const int k = r.k0 + v;
+					double2 t = Tb[k];
+					if (r.symZ) {
+						const double2 m2 = Tb[r.half ? 1 - k : -k];
+						const double sg = r.half ? r.coskz : -r.coskz;
+						t.x += sg * m2.x, t.y += sg * m2.y;
+					}
+					const double w = r.half ? 1.0 : (v == 0 ? 0.5 : 0.0) + (v == r.nv - 1 ? 0.5 : 0.0) + (v > 0 && v < r.nv - 1 ? 1.0 : 0.0);
+					const double s = fac * w * r.dz[v];
+					acc = cmad(acc, make_double2(s * t.x, s * t.y), r.C[((size_t)m * r.nv + v) * r.nc + col]);
+				}
+			}
+			S[((size_t)col * 24 + r.out[m]) * 2 + pol] = acc;
+		}
+}
+
+__global__ void k_ff_widen(const float *src, double *dst, long n)
+{
+	const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (a < n) dst[a] = src[a];
+}
+
+int ffWorkspace(alis_dev_world *D, size_t bytes)
+{
+	if (bytes <= D->ffWorkBytes) return 0;
+	if (D->ffWork) {
+		HIPCHECK(hipStreamSynchronize(D->st));
+		HIPCHECK(hipFree(D->ffWork));
+		D->bytes -= D->ffWorkBytes;
+	}
+	D->ffWork = nullptr, D->ffWorkBytes = 0;
+	HIPCHECK(hipMalloc(&D->ffWork, bytes));
+	D->ffWorkBytes = bytes;
+	D->bytes += bytes;
+	return 0;
+}
+
+#define BLASCHECK(x) do { rocblas_status st_ = (x); if (st_ != rocblas_status_success) return fail(std::string("rocBLAS: ") + rocblas_status_to_string(st_)); } while (0)
+
+}  // namespace
+
+int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *g, int n,
+                      const alis_dev_ff_col *cols, double *S)
+{
+	if (id < 0 || id >= (int)D->phs.size() || !D->phs[id].live) return fail("no such phaser");
+	DevPhaser &Q = D->phs[id];
+	if (q < 0 || q >= Q.nf) return fail("phaser frequency out of range");
+	if (n < 0) return fail("negative column count");
+	memset(S, 0, (size_t)n * 96 * sizeof(double));
+	if (!n) return 0;
+	const int ni = Q.iMax - Q.iMin + 1, nj = Q.jMax - Q.jMin + 1, nk = Q.kMax - Q.kMin + 1;
+	bool side = false;
+	for (int f = 0; f < Q.nface; f++) side |= Q.face[f].kind < 4;
+	if (side && (!g->T || g->nz < 1)) return fail("far field: x and y faces need the plane-wave profiles");
+	if (side) {
+		// the reduction reads T at k0..k1 and, with a z symmetry plane, at -k and 1-k
+		const int lo = g->symZ ? std::min(Q.kMin, -Q.kMax) : Q.kMin, hi = g->symZ ? std::max(Q.kMax, 1 - Q.kMin) : Q.kMax;
+		if (g->kLo > lo || g->kLo + g->nz - 1 < hi) return fail("far field: profiles do not cover the phaser box");
+		if (g->nslot < 1) return fail("far field: no profile slots");
+		for (int c = 0; c < n; c++)
+			for (int p = 0; p < 2; p++) {
+				if (cols[c].slot[p] < 0 || cols[c].slot[p] >= g->nslot) return fail("far field: profile slot out of range");
+				for (int t = 0; t < 6; t++) if (cols[c].src[p][t] >= 6) return fail("far field: no such profile component");
+			}
+	}
+
+	// columns per chunk: the C matrices (4 per block) stay within 256 MiB
+	const int nvMax = std::max(std::max(ni, nj), nk), npMax = std::max(ni, nj);
+	int NC = (int)std::min<long>(8192, (256L << 20) / (4L * nvMax * 16));
+	NC = std::max(64, NC / 64 * 64);
+	NC = std::min(NC, n);
+	// workspace (bytes, 256-aligned pieces)
+	auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t tabB = al((size_t)(4 * ni + 4 * nj + 2 * nk) * 8);
+	const size_t tB = side ? al((size_t)g->nslot * 6 * g->nz * 16) : 0;
+	const size_t colB = al((size_t)n * sizeof(alis_dev_ff_col));
+	const size_t fB = D->dft64 ? 0 : al((size_t)4 * Q.ncell * 16);
+	const size_t sB = al((size_t)n * 96 * 8);
+	const size_t betaB = al((size_t)npMax * NC * 16), cB = al((size_t)4 * nvMax * NC * 16);
+	if (ffWorkspace(D, tabB + tB + colB + fB + sB + betaB + cB)) return -1;
+	char *w = (char *)D->ffWork;
+	double *tab = (double *)w;                          w += tabB;
+	double2 *Tdev = (double2 *)w;                       w += tB;
+	alis_dev_ff_col *cdev = (alis_dev_ff_col *)w;       w += colB;
+	double *Fw = (double *)w;                           w += fB;
+	double2 *Sdev = (double2 *)w;                       w += sB;
+	double2 *beta = (double2 *)w;                       w += betaB;
+	double2 *Cm = (double2 *)w;
+
+	// tables: dxE dxH xN xH (ni) dyE dyH yN yH (nj) dzE dzH (nk)
+	std::vector<double> t;
+	t.reserve(4 * ni + 4 * nj + 2 * nk);
+	for (const double *a : {g->dxE, g->dxH, g->xN, g->xH}) t.insert(t.end(), a, a + ni);
+	for (const double *a : {g->dyE, g->dyH, g->yN, g->yH}) t.insert(t.end(), a, a + nj);
+	for (const double *a : {g->dzE, g->dzH}) t.insert(t.end(), a, a + nk);
+	HIPCHECK(hipMemcpyAsync(tab, t.data(), t.size() * 8, hipMemcpyHostToDevice, D->st));
+	if (side) HIPCHECK(hipMemcpyAsync(Tdev, g->T, (size_t)g->nslot * 6 * g->nz * 16, hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemcpyAsync(cdev, cols, (size_t)n * sizeof(alis_dev_ff_col), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemsetAsync(Sdev, 0, (size_t)n * 96 * 8, D->st));
+	const double *dxE = tab, *dxH = dxE + ni, *xN = dxH + ni, *xH = xN + ni;
+	const double *dyE = xH + ni, *dyH = dyE + nj, *yN = dyH + nj, *yH = yN + nj;
+	const double *dzE = yH + nj, *dzH = dzE + nk;
+
+	// the accumulators of frequency q as doubles: [comp][cell][re,im]
+	const double2 *F;
+	if (D->dft64) F = (const double2 *)Q.acc + (size_t)q * 4 * Q.ncell;
+	else {
+		const long m = 4 * Q.ncell * 2;
+		k_ff_widen<<<blocksFor(m), 256, 0, D->st>>>((const float *)Q.acc + (size_t)q * 4 * Q.ncell * 2, Fw, m);
+		F = (const double2 *)Fw;
+	}
+
+	if (!D->blas) BLASCHECK(rocblas_create_handle(&D->blas));
+	BLASCHECK(rocblas_set_stream(D->blas, D->st));
+	const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+
+	const FFPhase XN = {dxE, xN, ni, 0, g->symX, 0, g->coskx}, XH = {dxH, xH, ni, 1, g->symX, 0, g->coskx};
+	const FFPhase YN = {dyE, yN, nj, 0, g->symY, 1, g->cosky}, YH = {dyH, yH, nj, 1, g->symY, 1, g->cosky};
+	struct Block { int kinds[2]; int comps[2]; FFPhase beta; int profile, half; FFPhase alpha; int tcomp[2]; bool trans; int np, nv; };
+	const Block blocks[6] = {
+		// z faces F[i][j]: contract over j
+		{{4, 5}, {0, 3}, YN, 0, 0, XH, {0, 0}, false, nj, ni},
+		{{4, 5}, {1, 2}, YH, 0, 0, XN, {0, 0}, false, nj, ni},
+		// x faces F[j][k]: contract over j; T: comp 0 Hz, 3 Ey (Z_N), 1 Hy, 2 Ez (Z_H)
+		{{0, 1}, {0, 3}, YH, 1, 0, XN, {5, 1}, true, nj, nk},
+		{{0, 1}, {1, 2}, YN, 1, 1, XN, {4, 2}, true, nj, nk},
+		// y faces F[i][k]: contract over i; T: comp 0 Hx, 3 Ez (Z_H), 1 Hz, 2 Ex (Z_N)
+		{{2, 3}, {0, 3}, XN, 1, 1, XN, {3, 2}, true, ni, nk},
+		{{2, 3}, {1, 2}, XH, 1, 0, XN, {5, 0}, true, ni, nk},
+	};
+	for (int c0 = 0; c0 < n; c0 += NC) {
+		const int nc = std::min(NC, n - c0);
+		for (const Block &b : blocks) {
+			FFReduce r{};
+			r.C = Cm, r.nv = b.nv, r.nc = nc;
+			for (int f = 0; f < Q.nface; f++) {
+				const PhsFaceGeom &fg = Q.face[f];
+				if (fg.kind != b.kinds[0] && fg.kind != b.kinds[1]) continue;
+				for (int cc = 0; cc < 2; cc++) {
+					const int m = r.nm++;
+					r.out[m] = fg.kind * 4 + b.comps[cc];
+					r.tcomp[m] = b.tcomp[cc];
+					const double2 *Fm = F + (size_t)b.comps[cc] * Q.ncell + fg.off;
+					double2 *Cout = Cm + (size_t)m * b.nv * nc;
+					if (m == 0) {
+						dim3 grid(blocksFor(nc), b.np);
+						k_ff_beta<<<grid, 256, 0, D->st>>>(beta, b.beta, cdev + c0, nc);
+						HIPCHECK(hipGetLastError());
+					}
+					// C[v][col] = sum_p F[.][.] beta[p][col]  (column-major: C^T = beta^T op(F))
+					BLASCHECK(rocblas_zgemm(D->blas, rocblas_operation_none, b.trans ? rocblas_operation_transpose : rocblas_operation_none,
+					                        nc, b.nv, b.np, &one, (const rocblas_double_complex *)beta, nc,
+					                        (const rocblas_double_complex *)Fm, b.trans ? b.nv : b.np, &zero,
+					                        (rocblas_double_complex *)Cout, nc));
+				}
+			}
+			if (!r.nm) continue;
+			r.profile = b.profile, r.half = b.half, r.ph = b.alpha;
+			r.dz = b.half ? dzH : dzE;
+			r.T = Tdev, r.nz = g->nz, r.kLo = g->kLo, r.k0 = Q.kMin, r.symZ = g->symZ, r.coskz = g->coskz;
+			k_ff_reduce<<<blocksFor(nc), 256, 0, D->st>>>(Sdev + (size_t)c0 * 48, r, cdev + c0);
+			HIPCHECK(hipGetLastError());
+		}
+	}
+	HIPCHECK(hipMemcpyAsync(S, Sdev, (size_t)n * 96 * 8, hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
 }
 

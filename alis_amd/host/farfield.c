@@ -254,235 +254,249 @@ static profile profileFor(phaser P, tfield pol(phaser,float,float), double theta
 }
 
 
-/* Surface sums of one face of each orientation (src/farfld.c:248-459).  v/g/q address the
- * double accumulators (components E1 E2 H1 H2 of the face). */
-typedef struct { const double *v; const phaserGeom *g; int q; } acc;
+/*
+ * Far field at many directions (src/farfld.c:462-514 for each).  The directions become
+ * columns -- a polar angle within a degree of 90 or 270 is the mean of the two angles one
+ * degree either side -- and the plane-wave profiles T are computed once per distinct polar
+ * angle (the azimuth only rotates their components).  The engine evaluates the surface sums
+ * of every column (alis_dev_farfield: the face integrals of src/farfld.c:248-459 without the
+ * factors that are constant over a face); the far field of each column is assembled here:
+ * face signs, the in-plane phase of the x and y faces, the T values at the z faces, the
+ * periodic array factor and the normalisation.
+ */
+enum { TEX, TEY, TEZ, THX, THY, THZ };
 
-/* Zeroed complex array indexed lo..hi; release with free(a + lo). */
-static dcx *dcxSpan(int lo, int hi) { dcx *a = calloc(hi >= lo ? hi-lo+1 : 1, sizeof(dcx)); return a - lo; }
+typedef struct {
+	int n, cap, nz;
+	double *key;        /* polar angle of a pair of standard slots */
+	double *A;          /* per slot: T.A */
+	dcx *T;             /* per slot: 6 components x nz, k = kMIN .. kMax+1 */
+} ffSlots;
 
-static dcx F4(const acc *a, int kind, int comp, int u, int w) { return faceValue(a->v, a->g, a->q, kind, comp, u, w); }
-
-static void addC(dcx *s, dcx *c, dcx x)
+static int ffNewSlot(ffSlots *L)
 {
-	double sr = creal(*s), cr = creal(*c), si = cimag(*s), ci = cimag(*c);
-	neumaierAdd(&sr, &cr, creal(x));
-	neumaierAdd(&si, &ci, cimag(x));
-	*s = sr + I * si, *c = cr + I * ci;
+	if (L->n == L->cap) {
+		L->cap = L->cap ? 2 * L->cap : 64;
+		L->key = realloc(L->key, L->cap * sizeof(double));
+		L->A = realloc(L->A, L->cap * sizeof(double));
+		L->T = realloc(L->T, (size_t)L->cap * 6 * L->nz * sizeof(dcx));
+	}
+	memset(L->T + (size_t)L->n * 6 * L->nz, 0, 6 * L->nz * sizeof(dcx));
+	L->key[L->n] = NAN, L->A[L->n] = 0;
+	return L->n++;
 }
 
-static dcx faceX(phaser P, const acc *a, int kind, int i, profile *T, double kx, double ky)
+static void ffStore(phaser P, ffSlots *L, int slot, int comp, const dcx *a)
 {
-	world W = P->W;
-	if (!a->g->on[kind]) return 0;
-	int j0 = P->jMin, j1 = P->jMax, k0 = P->kMin, k1 = P->kMax;
-	dcx *Y = dcxSpan(j0, j1), *Z = dcxSpan(k0, k1);
-	dcx EyHz = 0, HzEy = 0, EzHy = 0, HyEz = 0, c = 0;
-	if (T->Hz || T->Ey) {
-		for (int j = j0+1; j <= j1; j++) Y[j] = W->CH->dy[j] * cexp(I*ky*W->jtoy(W, j-0.5));
-		if (W->yMinSur==SYM) for (int j = j0+1; j <= j1; j++) Y[j] += conj(Y[j]) * W->cosky;
-	}
-	for (int pass = 0; pass < 2; pass++) {
-		dcx *Tp = pass ? T->Ey : T->Hz;
-		if (!Tp) continue;
-		for (int k = k0; k <= k1; k++) Z[k] = W->CE->dz[k] * Tp[k];
-		if (W->zMinSur==SYM) for (int k = k0; k <= k1; k++) Z[k] -= W->CE->dz[k] * Tp[-k] * W->coskz;
-		int comp = pass ? 3 : 0;   /* E1 = Ey with T.Hz, H2 = Hz with T.Ey */
-		dcx s = 0; c = 0;
-		for (int j = j0+1; j <= j1; j++) {
-			dcx r = 0.5 * (F4(a, kind, comp, j-j0, 0) * Z[k0] + F4(a, kind, comp, j-j0, k1-k0) * Z[k1]);
-			for (int k = k0+1; k < k1; k++) r += F4(a, kind, comp, j-j0, k-k0) * Z[k];
-			addC(&s, &c, Y[j] * r);
-		}
-		if (pass) HzEy = s + c; else EyHz = s + c;
-	}
-	if (T->Hy || T->Ez) {
-		for (int j = j0; j <= j1; j++) Y[j] = W->CE->dy[j] * cexp(I*ky*W->jtoy(W, j));
-		if (W->yMinSur==SYM) for (int j = j0; j <= j1; j++) Y[j] -= conj(Y[j]) * W->cosky;
-	}
-	for (int pass = 0; pass < 2; pass++) {
-		dcx *Tp = pass ? T->Ez : T->Hy;
-		if (!Tp) continue;
-		for (int k = k0+1; k <= k1; k++) Z[k] = W->CH->dz[k] * Tp[k];
-		if (W->zMinSur==SYM) for (int k = k0+1; k <= k1; k++) Z[k] += W->CH->dz[k] * Tp[1-k] * W->coskz;
-		int comp = pass ? 2 : 1;   /* E2 = Ez with T.Hy, H1 = Hy with T.Ez */
-		dcx s = 0; c = 0;
-		for (int k = k0+1; k <= k1; k++) {
-			dcx r = 0.5 * (F4(a, kind, comp, 0, k-k0) * Y[j0] + F4(a, kind, comp, j1-j0, k-k0) * Y[j1]);
-			for (int j = j0+1; j < j1; j++) r += F4(a, kind, comp, j-j0, k-k0) * Y[j];
-			addC(&s, &c, Z[k] * r);
-		}
-		if (pass) HyEz = s + c; else EzHy = s + c;
-	}
-	free(Y + j0), free(Z + k0);
-	double x = W->itox(W, i);
-	dcx Sum = (EyHz-HzEy-EzHy+HyEz) * cexp(I*kx*x);
-	if (W->xMinSur==SYM) Sum += (EyHz+HzEy-EzHy-HyEz) * cexp(-I*kx*x) * W->coskx;
-	return Sum;
+	if (a) memcpy(L->T + ((size_t)slot * 6 + comp) * L->nz, a + P->kMIN, L->nz * sizeof(dcx));
 }
 
-static dcx faceY(phaser P, const acc *a, int kind, int j, profile *T, double kx, double ky)
+/* Far field of P at directions (theta[a], phi[a]) in degrees: for pol Ppol or Spol both
+ * polarisations (out0 P, out1 S), for another profile function its own (out0). */
+static void farFieldMany(phaser P, tfield pol(phaser,float,float), int n, const double *theta, const double *phi,
+                         dcx *out0, dcx *out1)
 {
 	world W = P->W;
-	if (!a->g->on[kind]) return 0;
-	int i0 = P->iMin, i1 = P->iMax, k0 = P->kMin, k1 = P->kMax;
-	dcx *X = dcxSpan(i0, i1), *Z = dcxSpan(k0, k1);
-	dcx EzHx = 0, HxEz = 0, ExHz = 0, HzEx = 0, c = 0;
-	if (T->Hx || T->Ez) {
-		for (int i = i0; i <= i1; i++) X[i] = W->CE->dx[i] * cexp(I*kx*W->itox(W, i));
-		if (W->xMinSur==SYM) for (int i = i0; i <= i1; i++) X[i] -= conj(X[i]) * W->coskx;
+	const int standard = pol == Ppol || pol == Spol;
+	for (int a = 0; a < n; a++) {
+		out0[a] = 0;
+		if (out1) out1[a] = 0;
 	}
-	for (int pass = 0; pass < 2; pass++) {
-		dcx *Tp = pass ? T->Ez : T->Hx;
-		if (!Tp) continue;
-		for (int k = k0+1; k <= k1; k++) Z[k] = W->CH->dz[k] * Tp[k];
-		if (W->zMinSur==SYM) for (int k = k0+1; k <= k1; k++) Z[k] += W->CH->dz[k] * Tp[1-k] * W->coskz;
-		int comp = pass ? 3 : 0;   /* E1 = Ez with T.Hx, H2 = Hx with T.Ez */
-		dcx s = 0; c = 0;
-		for (int k = k0+1; k <= k1; k++) {
-			dcx r = 0.5 * (F4(a, kind, comp, 0, k-k0) * X[i0] + F4(a, kind, comp, i1-i0, k-k0) * X[i1]);
-			for (int i = i0+1; i < i1; i++) r += F4(a, kind, comp, i-i0, k-k0) * X[i];
-			addC(&s, &c, Z[k] * r);
-		}
-		if (pass) HxEz = s + c; else EzHx = s + c;
-	}
-	if (T->Hz || T->Ex) {
-		for (int i = i0+1; i <= i1; i++) X[i] = W->CH->dx[i] * cexp(I*kx*W->itox(W, i-0.5));
-		if (W->xMinSur==SYM) for (int i = i0+1; i <= i1; i++) X[i] += conj(X[i]) * W->coskx;
-	}
-	for (int pass = 0; pass < 2; pass++) {
-		dcx *Tp = pass ? T->Ex : T->Hz;
-		if (!Tp) continue;
-		for (int k = k0; k <= k1; k++) Z[k] = W->CE->dz[k] * Tp[k];
-		if (W->zMinSur==SYM) for (int k = k0; k <= k1; k++) Z[k] -= W->CE->dz[k] * Tp[-k] * W->coskz;
-		int comp = pass ? 2 : 1;   /* E2 = Ex with T.Hz, H1 = Hz with T.Ex */
-		dcx s = 0; c = 0;
-		for (int i = i0+1; i <= i1; i++) {
-			dcx r = 0.5 * (F4(a, kind, comp, i-i0, 0) * Z[k0] + F4(a, kind, comp, i-i0, k1-k0) * Z[k1]);
-			for (int k = k0+1; k < k1; k++) r += F4(a, kind, comp, i-i0, k-k0) * Z[k];
-			addC(&s, &c, X[i] * r);
-		}
-		if (pass) HzEx = s + c; else ExHz = s + c;
-	}
-	free(X + i0), free(Z + k0);
-	double y = W->jtoy(W, j);
-	dcx Sum = (EzHx-HxEz-ExHz+HzEx) * cexp(I*ky*y);
-	if (W->yMinSur==SYM) Sum += (EzHx+HxEz-ExHz-HzEx) * cexp(-I*ky*y) * W->cosky;
-	return Sum;
-}
+	phaser B = phaserBase(P);
+	if (!B || !n) return;
+	if (!P->dE) farfieldCoeffs(P, W);
+	alisPhaserDevice(B);
 
-static dcx faceZ(phaser P, const acc *a, int kind, int k, profile *T, double kx, double ky)
-{
-	world W = P->W;
-	if (!a->g->on[kind]) return 0;
-	int i0 = P->iMin, i1 = P->iMax, j0 = P->jMin, j1 = P->jMax;
-	dcx *X = dcxSpan(i0, i1), *Y = dcxSpan(j0, j1);
-	dcx Sum = 0, c = 0;
-	int sym = W->zMinSur==SYM;
-	if (T->Hy || T->Ex) {
-		for (int i = i0+1; i <= i1; i++) X[i] = W->CH->dx[i] * cexp(I*kx*W->itox(W, i-0.5));
-		if (W->xMinSur==SYM) for (int i = i0+1; i <= i1; i++) X[i] += conj(X[i]) * W->coskx;
-		for (int j = j0; j <= j1; j++) Y[j] = W->CE->dy[j] * cexp(I*ky*W->jtoy(W, j));
-		if (W->yMinSur==SYM) for (int j = j0; j <= j1; j++) Y[j] -= conj(Y[j]) * W->cosky;
-		for (int pass = 0; pass < 2; pass++) {
-			dcx *Tp = pass ? T->Ex : T->Hy;
-			if (!Tp) continue;
-			int comp = pass ? 3 : 0;   /* E1 = Ex with T.Hy, H2 = Hy with T.Ex */
-			dcx s = 0; c = 0;
-			for (int i = i0+1; i <= i1; i++) {
-				dcx r = 0.5 * (F4(a, kind, comp, i-i0, 0) * Y[j0] + F4(a, kind, comp, i-i0, j1-j0) * Y[j1]);
-				for (int j = j0+1; j < j1; j++) r += F4(a, kind, comp, i-i0, j-j0) * Y[j];
-				addC(&s, &c, X[i] * r);
-			}
-			dcx S = s + c;
-			if (!pass) {
-				Sum += S * 0.5 * (Tp[k]+Tp[k+1]);
-				if (sym) Sum += S * 0.5 * (Tp[1-k]+Tp[-k]) * W->coskz;
+	/* columns */
+	typedef struct { double theta, phi, w; int out; } dir;
+	dir *D = malloc(2 * n * sizeof *D);
+	alis_dev_ff_col *C = calloc(2 * n, sizeof *C);
+	int m = 0;
+	for (int a = 0; a < n; a++) {
+		double th = theta[a], ph = phi[a];
+		if (th == INF || ph == INF) continue;
+		while (th < 0) th += 360;
+		while (th >= 360) th -= 360;
+		while (ph < 0) ph += 360;
+		while (ph >= 360) ph -= 360;
+		double ths[2] = {th, th}, w = 1;
+		int nt = 1;
+		if (89 < th && th < 91) ths[0] = 89, ths[1] = 91, nt = 2, w = 0.5;
+		if (269 < th && th < 271) ths[0] = 269, ths[1] = 271, nt = 2, w = 0.5;
+		for (int t = 0; t < nt; t++) {
+			double tt = ths[t], sinth;
+			if (tt < 90 || 270 < tt) {
+				if (cimagf(P->ex[P->kMax])) continue;
+				sinth = sin(tt*M_PI/180) * sqrt(crealf(P->ex[P->kMax]));
 			}
 			else {
-				Sum -= S * Tp[k];
-				if (sym) Sum += S * Tp[-k] * W->coskz;
+				if (cimagf(P->ex[P->kMIN])) continue;
+				sinth = sin(tt*M_PI/180) * sqrt(crealf(P->ex[P->kMIN]));
 			}
+			D[m] = (dir) {tt, ph, w, a};
+			C[m].kx = -2 * M_PI * P->f * sinth * cos(ph*M_PI/180);
+			C[m].ky = -2 * M_PI * P->f * sinth * sin(ph*M_PI/180);
+			m++;
 		}
 	}
-	if (T->Hx || T->Ey) {
-		for (int i = i0; i <= i1; i++) X[i] = W->CE->dx[i] * cexp(I*kx*W->itox(W, i));
-		if (W->xMinSur==SYM) for (int i = i0; i <= i1; i++) X[i] -= conj(X[i]) * W->coskx;
-		for (int j = j0+1; j <= j1; j++) Y[j] = W->CH->dy[j] * cexp(I*ky*W->jtoy(W, j-0.5));
-		if (W->yMinSur==SYM) for (int j = j0+1; j <= j1; j++) Y[j] += conj(Y[j]) * W->cosky;
-		for (int pass = 0; pass < 2; pass++) {
-			dcx *Tp = pass ? T->Ey : T->Hx;
-			if (!Tp) continue;
-			int comp = pass ? 2 : 1;   /* E2 = Ey with T.Hx, H1 = Hx with T.Ey */
-			dcx s = 0; c = 0;
-			for (int j = j0+1; j <= j1; j++) {
-				dcx r = 0.5 * (F4(a, kind, comp, 0, j-j0) * X[i0] + F4(a, kind, comp, i1-i0, j-j0) * X[i1]);
-				for (int i = i0+1; i < i1; i++) r += F4(a, kind, comp, i-i0, j-j0) * X[i];
-				addC(&s, &c, Y[j] * r);
+
+	/* profiles: a P and an S slot per distinct polar angle, or one slot per column */
+	ffSlots L = {0, 0, P->kMax + 2 - P->kMIN, 0, 0, 0};
+	if (standard) {
+		int *slotOf = malloc((m ? m : 1) * sizeof(int));
+		/* open-addressing table from polar angle to slot pair */
+		int hcap = 16;
+		while (hcap < 4 * m) hcap *= 2;
+		int *hash = malloc(hcap * sizeof(int));
+		for (int h = 0; h < hcap; h++) hash[h] = -1;
+		for (int c = 0; c < m; c++) {
+			unsigned long long bits;
+			memcpy(&bits, &D[c].theta, sizeof bits);
+			int h = (int)((bits * 0x9e3779b97f4a7c15ULL) >> 40) & (hcap - 1);
+			while (hash[h] >= 0 && L.key[hash[h]] != D[c].theta) h = (h + 1) & (hcap - 1);
+			if (hash[h] < 0) {
+				ffNewSlot(&L), ffNewSlot(&L);
+				hash[h] = L.n - 2;
+				L.key[L.n - 2] = D[c].theta;
 			}
-			dcx S = s + c;
-			if (!pass) {
-				Sum -= S * 0.5 * (Tp[k]+Tp[k+1]);
-				if (sym) Sum -= S * 0.5 * (Tp[1-k]+Tp[-k]) * W->coskz;
+			slotOf[c] = hash[h];
+		}
+		free(hash);
+		#pragma omp parallel for schedule(dynamic)
+		for (int s = 0; s < L.n; s += 2) {
+			profile Tp = pPolD(P, L.key[s], 0), Ts = sPolD(P, L.key[s], 0);
+			ffStore(P, &L, s, TEX, Tp.Ex), ffStore(P, &L, s, THY, Tp.Hy), ffStore(P, &L, s, TEZ, Tp.Ez);
+			ffStore(P, &L, s+1, TEY, Ts.Ey), ffStore(P, &L, s+1, THX, Ts.Hx), ffStore(P, &L, s+1, THZ, Ts.Hz);
+			L.A[s] = Tp.A, L.A[s+1] = Ts.A;
+			freeProfiles(P, &Tp), freeProfiles(P, &Ts);
+		}
+		for (int c = 0; c < m; c++) {
+			alis_dev_ff_col *q = &C[c];
+			const int s = slotOf[c];
+			const double sp = sin(D[c].phi*M_PI/180), cp = cos(D[c].phi*M_PI/180);
+			const int rot = D[c].phi != 0, hasZ = D[c].theta != 0;
+			q->slot[0] = s, q->slot[1] = s + 1;
+			for (int t = 0; t < 6; t++) q->src[0][t] = q->src[1][t] = -1;
+			/* P: Ex Ey from Ex0, Hx Hy from Hy0, Ez0 (src/farfld.c:169-184) */
+			q->src[0][TEX] = TEX, q->fac[0][TEX] = rot ? cp : 1;
+			q->src[0][THY] = THY, q->fac[0][THY] = rot ? cp : 1;
+			if (rot) q->src[0][TEY] = TEX, q->fac[0][TEY] = sp, q->src[0][THX] = THY, q->fac[0][THX] = -sp;
+			if (hasZ) q->src[0][TEZ] = TEZ, q->fac[0][TEZ] = 1;
+			/* S: Ex Ey from Ey0, Hx Hy from Hx0, Hz0 (src/farfld.c:229-244) */
+			q->src[1][TEY] = TEY, q->fac[1][TEY] = rot ? cp : 1;
+			q->src[1][THX] = THX, q->fac[1][THX] = rot ? cp : 1;
+			if (rot) q->src[1][TEX] = TEY, q->fac[1][TEX] = -sp, q->src[1][THY] = THX, q->fac[1][THY] = sp;
+			if (hasZ) q->src[1][THZ] = THZ, q->fac[1][THZ] = 1;
+		}
+		free(slotOf);
+	}
+	else {
+		for (int c = 0; c < m; c++) {
+			const int s = ffNewSlot(&L);
+			profile T = profileFor(P, pol, D[c].theta, D[c].phi);
+			dcx *comp[6] = {T.Ex, T.Ey, T.Ez, T.Hx, T.Hy, T.Hz};
+			for (int t = 0; t < 6; t++) {
+				ffStore(P, &L, s, t, comp[t]);
+				C[c].src[0][t] = comp[t] ? t : -1, C[c].fac[0][t] = 1;
+				C[c].src[1][t] = -1;
 			}
-			else {
-				Sum += S * Tp[k];
-				if (sym) Sum -= S * Tp[-k] * W->coskz;
-			}
+			L.A[s] = T.A;
+			C[c].slot[0] = C[c].slot[1] = s;
+			freeProfiles(P, &T);
 		}
 	}
-	free(X + i0), free(Y + j0);
-	return Sum;
+
+	/* the engine's surface sums */
+	phaserGeom g;
+	phaserFaces(B, &g);
+	const int ni = P->iMax-P->iMin+1, nj = P->jMax-P->jMin+1, nk = P->kMax-P->kMin+1;
+	double *tab = malloc((4 * (ni + nj) + 2 * nk) * sizeof(double));
+	double *dxE = tab, *dxH = dxE + ni, *xN = dxH + ni, *xH = xN + ni;
+	double *dyE = xH + ni, *dyH = dyE + nj, *yN = dyH + nj, *yH = yN + nj;
+	double *dzE = yH + nj, *dzH = dzE + nk;
+	for (int i = P->iMin; i <= P->iMax; i++) {
+		const int a = i - P->iMin;
+		dxE[a] = W->CE->dx[i], dxH[a] = W->CH->dx[i], xN[a] = W->itox(W, i), xH[a] = W->itox(W, i-0.5);
+	}
+	for (int j = P->jMin; j <= P->jMax; j++) {
+		const int a = j - P->jMin;
+		dyE[a] = W->CE->dy[j], dyH[a] = W->CH->dy[j], yN[a] = W->jtoy(W, j), yH[a] = W->jtoy(W, j-0.5);
+	}
+	for (int k = P->kMin; k <= P->kMax; k++) dzE[k-P->kMin] = W->CE->dz[k], dzH[k-P->kMin] = W->CH->dz[k];
+	const int sideFaces = g.on[0] || g.on[1] || g.on[2] || g.on[3];
+	alis_dev_ff_geom G = {dxE, dxH, xN, xH, dyE, dyH, yN, yH, dzE, dzH,
+		W->xMinSur==SYM, W->yMinSur==SYM, W->zMinSur==SYM, W->coskx, W->cosky, W->coskz,
+		sideFaces ? L.n : 0, P->kMIN, L.nz, sideFaces ? (const double *)L.T : 0};
+	double *S = malloc(((size_t)m * 96 + 1) * sizeof(double));
+	if (m) alisFatal("far field", alis_dev_farfield(W->dev, B->devId, (int)(P - B), &G, m, C, S));
+
+	/* assembly */
+	const int sym = W->zMinSur==SYM;
+	for (int c = 0; c < m; c++) {
+		const double *s = S + (size_t)c * 96;
+		const double kx = C[c].kx, ky = C[c].ky;
+		for (int p = 0; p < (standard ? 2 : 1); p++) {
+			/* sums of face `kind`; those of z faces do not involve T and are stored once */
+#define SV(kind, comp) (s[2*(((kind)*4+(comp))*2+((kind) < 4 ? p : 0))] + I * s[2*(((kind)*4+(comp))*2+((kind) < 4 ? p : 0))+1])
+			const alis_dev_ff_col *q = &C[c];
+			/* component t of this column's profile at absolute k */
+#define TV(t, kk) (q->src[p][t] < 0 ? (dcx)0 : q->fac[p][t] * L.T[((size_t)q->slot[p] * 6 + q->src[p][t]) * L.nz + (kk) - P->kMIN])
+			dcx Sum = 0;
+			for (int kind = 0; kind < 2; kind++) {
+				if ((kind ? W->xMaxSur : W->xMinSur) != PML) continue;
+				const double x = W->itox(W, kind ? P->iMax : P->iMin);
+				dcx v = (SV(kind,0) - SV(kind,3) - SV(kind,1) + SV(kind,2)) * cexp(I*kx*x);
+				if (W->xMinSur==SYM) v += (SV(kind,0) + SV(kind,3) - SV(kind,1) - SV(kind,2)) * cexp(-I*kx*x) * W->coskx;
+				Sum += kind ? v : -v;
+			}
+			for (int kind = 2; kind < 4; kind++) {
+				if ((kind == 2 ? W->yMinSur : W->yMaxSur) != PML) continue;
+				const double y = W->jtoy(W, kind == 3 ? P->jMax : P->jMin);
+				dcx v = (SV(kind,0) - SV(kind,3) - SV(kind,1) + SV(kind,2)) * cexp(I*ky*y);
+				if (W->yMinSur==SYM) v += (SV(kind,0) + SV(kind,3) - SV(kind,1) - SV(kind,2)) * cexp(-I*ky*y) * W->cosky;
+				Sum += kind == 3 ? v : -v;
+			}
+			for (int kind = 4; kind < 6; kind++) {
+				if ((kind == 4 ? W->zMinSur : W->zMaxSur) != PML || !g.on[kind]) continue;
+				const int k = kind == 5 ? P->kMax : P->kMin;
+				dcx v = SV(kind,0) * 0.5 * (TV(THY,k) + TV(THY,k+1));
+				if (sym) v += SV(kind,0) * 0.5 * (TV(THY,1-k) + TV(THY,-k)) * W->coskz;
+				v -= SV(kind,3) * TV(TEX,k);
+				if (sym) v += SV(kind,3) * TV(TEX,-k) * W->coskz;
+				v -= SV(kind,1) * 0.5 * (TV(THX,k) + TV(THX,k+1));
+				if (sym) v -= SV(kind,1) * 0.5 * (TV(THX,1-k) + TV(THX,-k)) * W->coskz;
+				v += SV(kind,2) * TV(TEY,k);
+				if (sym) v -= SV(kind,2) * TV(TEY,-k) * W->coskz;
+				Sum += kind == 5 ? v : -v;
+			}
+#undef TV
+#undef SV
+			if (P->xNUM) {
+				dcx a = 0, r = 1, e = P->expikx * cexp(I*kx*(W->xMax-W->xMin));
+				for (int x = 0; x < P->xNUM; x++) a += (r *= e);
+				Sum *= a / P->xNUM;
+			}
+			if (P->yNUM) {
+				dcx a = 0, r = 1, e = P->expiky * cexp(I*ky*(W->yMax-W->yMin));
+				for (int y = 0; y < P->yNUM; y++) a += (r *= e);
+				Sum *= a / P->yNUM;
+			}
+			dcx v = Sum * L.A[q->slot[p]] * P->A * P->f;
+			if (p) out1[D[c].out] += D[c].w * v;
+			else out0[D[c].out] += D[c].w * v;
+		}
+	}
+	free(S), free(tab), free(D), free(C);
+	free(L.key), free(L.A), free(L.T);
 }
 
 /* Complex far-field amplitude of phaser P for polarisation pol at (theta, phi) in degrees,
  * including the periodic array factor (src/farfld.c:462-514). */
 dcx farFieldD(phaser P, tfield pol(phaser,float,float), double theta, double phi)
 {
-	if (!P->dE) farfieldCoeffs(P, P->W);
-	if (theta == INF || phi == INF) return 0;
-	while (theta < 0) theta += 360;
-	while (theta >= 360) theta -= 360;
-	while (phi < 0) phi += 360;
-	while (phi >= 360) phi -= 360;
-	if ( 89 < theta && theta <  91) return 0.5 * (farFieldD(P, pol,  89, phi) + farFieldD(P, pol,  91, phi));
-	if (269 < theta && theta < 271) return 0.5 * (farFieldD(P, pol, 269, phi) + farFieldD(P, pol, 271, phi));
-	double sinth = 0;
-	if (theta < 90 || 270 < theta) {
-		if (cimagf(P->ex[P->kMax])) return 0;
-		sinth = sin(theta*M_PI/180) * sqrt(crealf(P->ex[P->kMax]));
-	}
-	if (90 < theta && theta < 270) {
-		if (cimagf(P->ex[P->kMIN])) return 0;
-		sinth = sin(theta*M_PI/180) * sqrt(crealf(P->ex[P->kMIN]));
-	}
-	phaser B = phaserBase(P);
-	if (!B) return 0;
-	phaserGeom g;
-	acc a = { phaserDoubles(B, &g), &g, (int)(P - B) };
-	profile T = profileFor(P, pol, theta, phi);
-	double kx = -2 * M_PI * P->f * sinth * cos(phi*M_PI/180);
-	double ky = -2 * M_PI * P->f * sinth * sin(phi*M_PI/180);
-	world W = P->W;
-	dcx Sum = 0;
-	if (W->xMinSur==PML) Sum -= faceX(P, &a, 0, P->iMin, &T, kx, ky);
-	if (W->xMaxSur==PML) Sum += faceX(P, &a, 1, P->iMax, &T, kx, ky);
-	if (W->yMinSur==PML) Sum -= faceY(P, &a, 2, P->jMin, &T, kx, ky);
-	if (W->yMaxSur==PML) Sum += faceY(P, &a, 3, P->jMax, &T, kx, ky);
-	if (W->zMinSur==PML) Sum -= faceZ(P, &a, 4, P->kMin, &T, kx, ky);
-	if (W->zMaxSur==PML) Sum += faceZ(P, &a, 5, P->kMax, &T, kx, ky);
-	freeProfiles(P, &T);
-	if (P->xNUM) {
-		dcx s = 0, q = 1, p = P->expikx * cexp(I*kx*(W->xMax-W->xMin));
-		for (int x = 0; x < P->xNUM; x++) s += (q *= p);
-		Sum *= s / P->xNUM;
-	}
-	if (P->yNUM) {
-		dcx s = 0, q = 1, p = P->expiky * cexp(I*ky*(W->yMax-W->yMin));
-		for (int y = 0; y < P->yNUM; y++) s += (q *= p);
-		Sum *= s / P->yNUM;
-	}
-	return Sum * T.A * P->A * P->f;
+	dcx p, s;
+	farFieldMany(P, pol, 1, &theta, &phi, &p, &s);
+	return pol == Spol ? s : p;
 }
 
 float complex farField(phaser P, tfield pol(phaser,float,float), float theta, float phi)
@@ -492,7 +506,11 @@ float complex farField(phaser P, tfield pol(phaser,float,float), float theta, fl
 
 double farFieldID(phaser P, tfield pol(phaser,float,float), double theta, double phi)
 {
-	if (pol == Unpol) return farFieldID(P, Ppol, theta, phi) + farFieldID(P, Spol, theta, phi);
+	if (pol == Unpol) {
+		dcx p, s;
+		farFieldMany(P, Ppol, 1, &theta, &phi, &p, &s);
+		return creal(conj(p) * p) + creal(conj(s) * s);
+	}
 	dcx f = farFieldD(P, pol, theta, phi);
 	return creal(conj(f) * f);
 }
@@ -500,15 +518,6 @@ double farFieldID(phaser P, tfield pol(phaser,float,float), double theta, double
 float farFieldI(phaser P, tfield pol(phaser,float,float), float theta, float phi)
 {
 	return farFieldID(P, pol, theta, phi);
-}
-
-/* Prepare P for concurrent evaluation (coefficients and the accumulator view). */
-static void farFieldReady(phaser P)
-{
-	if (!P->dE) farfieldCoeffs(P, P->W);
-	phaser B = phaserBase(P);
-	phaserGeom g;
-	if (B) phaserDoubles(B, &g);
 }
 
 /* Far-field images over a projection map: X, Y, circular and linear polarisations
@@ -543,16 +552,22 @@ void farFieldProfile(world W, phaser P, projmap projMap, int X, int Y, ...)
 		if (angle[0] != INF) N++;
 	}
 	printf("Performing Near-to-Far-Field Transformation.\n");
-	farFieldReady(P);
+	const long XY = (long)X * Y;
+	double *th = malloc(2 * XY * sizeof(double)), *ph = th + XY;
+	dcx *pv = malloc(2 * XY * sizeof(dcx)), *sv = pv + XY;
+	for (long xy = 0; xy < XY; xy++) {
+		float ang[2];
+		projMap(xy / Y, xy % Y, X, Y, ang);
+		th[xy] = ang[0], ph[xy] = ang[1];
+	}
+	farFieldMany(P, Ppol, XY, th, ph, pv, sv);
 	int logScale = fieldname[0] != 0;
 	float max = 0;
-	#pragma omp parallel for schedule(dynamic) reduction(max:max)
-	for (long xy = 0; xy < (long)X*Y; xy++) {
+	#pragma omp parallel for schedule(static) reduction(max:max)
+	for (long xy = 0; xy < XY; xy++) {
 		int x = xy / Y, y = xy % Y;
-		float ang[2];
-		projMap(x, y, X, Y, ang);
-		dcx p = farFieldD(P, Ppol, ang[0], ang[1]);
-		dcx s = farFieldD(P, Spol, ang[0], ang[1]);
+		float ang[2] = {th[xy], ph[xy]};
+		dcx p = pv[xy], s = sv[xy];
 		double c = cos(ang[1]*M_PI/180), sn = sin(ang[1]*M_PI/180);
 		double vx = creal(conj(p*c-s*sn)*(p*c-s*sn)), vy = creal(conj(s*c+p*sn)*(s*c+p*sn));
 		double vl = sq(creal(p)+cimag(s))+sq(cimag(p)-creal(s)), vr = sq(creal(p)-cimag(s))+sq(cimag(p)+creal(s));
@@ -567,6 +582,7 @@ void farFieldProfile(world W, phaser P, projmap projMap, int X, int Y, ...)
 		}
 		max = MAX(max, Up[0][x][y]);
 	}
+	free(th), free(pv);
 	if (Max <= 0) Max = max;
 	const char *suffix[7] = { "Unpol", "P-pol", "S-pol", "CL-pol", "CR-pol", "X-pol", "Y-pol" };
 	const int order[7] = { 5, 6, 3, 4, 1, 2, 0 };
@@ -587,11 +603,13 @@ void farFieldTheta(world W, phaser P, float phi, char *format, ...)
 	va_start(ap, format);
 	vsnprintf(filename, sizeof filename, format, ap);
 	va_end(ap);
-	farFieldReady(P);
-	for (float theta = 0; theta < 360; theta += 1) {
-		dcx p = farFieldD(P, Ppol, theta, phi), s = farFieldD(P, Spol, theta, phi);
-		(writeRow)(W, filename, (double)theta, sq(cabs(p))+sq(cabs(s)), sq(cabs(p)), sq(cabs(s)), INT2NAN(1));
-	}
+	double th[360], ph[360];
+	dcx p[360], s[360];
+	int n = 0;
+	for (float theta = 0; theta < 360; theta += 1) th[n] = theta, ph[n] = phi, n++;
+	farFieldMany(P, Ppol, n, th, ph, p, s);
+	for (int a = 0; a < n; a++)
+		(writeRow)(W, filename, th[a], sq(cabs(p[a]))+sq(cabs(s[a])), sq(cabs(p[a])), sq(cabs(s[a])), INT2NAN(1));
 }
 
 void farFieldPhi(world W, phaser P, float theta, char *format, ...)
@@ -601,11 +619,13 @@ void farFieldPhi(world W, phaser P, float theta, char *format, ...)
 	va_start(ap, format);
 	vsnprintf(filename, sizeof filename, format, ap);
 	va_end(ap);
-	farFieldReady(P);
-	for (float phi = 0; phi < 360; phi += 1) {
-		dcx p = farFieldD(P, Ppol, theta, phi), s = farFieldD(P, Spol, theta, phi);
-		(writeRow)(W, filename, (double)phi, sq(cabs(p))+sq(cabs(s)), sq(cabs(p)), sq(cabs(s)), INT2NAN(1));
-	}
+	double th[360], ph[360];
+	dcx p[360], s[360];
+	int n = 0;
+	for (float phi = 0; phi < 360; phi += 1) th[n] = theta, ph[n] = phi, n++;
+	farFieldMany(P, Ppol, n, th, ph, p, s);
+	for (int a = 0; a < n; a++)
+		(writeRow)(W, filename, ph[a], sq(cabs(p[a]))+sq(cabs(s[a])), sq(cabs(p[a])), sq(cabs(s[a])), INT2NAN(1));
 }
 
 /* Power radiated into the cone of half angle aMax (or the ring aMin..aMax) around
@@ -615,7 +635,6 @@ float (farFieldFlux)(phaser P, tfield pol(phaser,float,float), float theta, floa
 {
 	if (!delta) delta = 1;
 	if (!aMax) aMax = aMin, aMin = 0;
-	farFieldReady(P);
 	double Sum = 0;
 	int nMin = floor(aMin/delta+0.5);
 	int nMax = ceil(aMax/delta-0.5);
@@ -625,17 +644,31 @@ float (farFieldFlux)(phaser P, tfield pol(phaser,float,float), float theta, floa
 		nMin++;
 	}
 	double ringSum = 0;
-	if (nMax) {
-		#pragma omp parallel for schedule(dynamic) reduction(+:ringSum)
+	if (nMax && nMax >= nMin) {
+		/* every sample of every ring in one evaluation */
+		long total = 0;
+		for (int n = nMin; n <= nMax; n++) total += n < floor(60/delta+0.5) ? n*6 : floor(60/delta+0.5)*6;
+		double *ab = malloc(3 * total * sizeof(double) + 1);
+		dcx *pv = malloc(2 * total * sizeof(dcx) + 1), *sv = pv + total;
+		long at = 0;
+		for (int n = nMin; n <= nMax; n++) {
+			int N = n < floor(60/delta+0.5) ? n*6 : floor(60/delta+0.5)*6;
+			for (int m = 0; m < N; m++, at++) {
+				float a = rotateA(theta, delta*n, 360.0*m/N);
+				float b = rotateB(theta, delta*n, 360.0*m/N)+phi;
+				ab[at] = a, ab[total+at] = b;
+				ab[2*total+at] = rotateB(-a, theta, phi-b)+180-360.0*m/N;
+			}
+		}
+		farFieldMany(P, Ppol, total, ab, ab + total, pv, sv);
+		at = 0;
 		for (int n = nMin; n <= nMax; n++) {
 			int N = n < floor(60/delta+0.5) ? n*6 : floor(60/delta+0.5)*6;
 			double dth = cos(MAX(aMin,delta*(n-0.5))*M_PI/180) - cos(MIN(aMax,delta*(n+0.5))*M_PI/180);
 			double ring = 0;
-			for (int m = 0; m < N; m++) {
-				float a = rotateA(theta, delta*n, 360.0*m/N);
-				float b = rotateB(theta, delta*n, 360.0*m/N)+phi;
-				float c = rotateB(-a, theta, phi-b)+180-360.0*m/N;
-				dcx p = farFieldD(P, Ppol, a, b), s = farFieldD(P, Spol, a, b);
+			for (int m = 0; m < N; m++, at++) {
+				float c = ab[2*total+at];
+				dcx p = pv[at], s = sv[at];
 				double pp = sq(cabs(p * cos(c*M_PI/180) - s * sin(c*M_PI/180)));
 				double ss = sq(cabs(s * cos(c*M_PI/180) + p * sin(c*M_PI/180)));
 				if (pol == Ppol) ring += dth * (pp * (M_PI/N + sin(M_PI/N)) + ss * (M_PI/N - sin(M_PI/N)));
@@ -644,6 +677,7 @@ float (farFieldFlux)(phaser P, tfield pol(phaser,float,float), float theta, floa
 			}
 			ringSum += ring;
 		}
+		free(ab), free(pv);
 	}
 	return Sum + ringSum;
 }

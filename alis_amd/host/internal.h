@@ -24,6 +24,7 @@ void alisInvalidate(world W);      /* setup changed: bring state home, drop the 
 void alisDropDevice(world W);      /* host arrays are newer: drop the device copy      */
 void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1);
 void alisSyncPhaser(phaser P);
+void alisPhaserDevice(phaser B);   /* register the array B on the device if it is not */
 alis_dev_batch *alisBatch(world *L, int n);
 void alisSyncAll(void);            /* every live world and phaser (timer() hook)        */
 void alisRegisterWorld(world W);

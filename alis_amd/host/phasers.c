@@ -90,19 +90,26 @@ phaser choosePhaser(phaser P, float wavelength)
  * phaser of its array (src/phaser.c:83-110).  The phase factors are computed here: in the
  * reference's float arithmetic for FP32 accumulators, in double (with the double pi) for
  * FP64 accumulators. */
+/* Make the phaser array B device-resident (its world committed, accumulators uploaded from
+ * the host cfields when it is not registered yet). */
+void alisPhaserDevice(phaser B)
+{
+	alisCommit(B->W);
+	if (B->devId < 0) {
+		alis_dev_phaser_desc d;
+		describePhaser(B, &d);
+		int rc = alis_dev_phaser_add(B->W->dev, &d, &B->devId);
+		free(d.cf);
+		alisFatal("phaser registration", rc);
+	}
+}
+
 void updatePhaser(world W, phaser P)
 {
 	phaser B = phaserBase(P);
 	if (!B) printf("updatePhaser: unknown phaser\n"), exit(1);
 	if (B->W != W) printf("updatePhaser: the phaser belongs to another world\n"), exit(1);
-	alisCommit(W);
-	if (B->devId < 0) {
-		alis_dev_phaser_desc d;
-		describePhaser(B, &d);
-		int rc = alis_dev_phaser_add(W->dev, &d, &B->devId);
-		free(d.cf);
-		alisFatal("phaser registration", rc);
-	}
+	alisPhaserDevice(B);
 	int q0 = (int)(P - B), nq = B->count - q0;
 	static double *eE, *eH;
 	static int cap;

@@ -53,7 +53,7 @@ static world requestWorld(const alisRequest *q, phaser *P)
 	const float r = q->radius * dx;
 	matter M = {{1.5}, {{q->plasma, 0.07}, {2.6, 0.3, 0.5}, {3.5, 0.6, 0.8}}};
 	object Sphere = {Ball, {{-r, r}, {-r, r}, {-r, r}}};
-	putObjects(W, M, Sphere, (matter){1});
+	putObjects(W, M, Sphere, (matter){{1}});
 	const float tf = r + 3 * dx, ph = r + 6 * dx;
 	dom DTF = {{-tf, tf}, {-tf, tf}, {-tf, tf}};
 	planewave(W, DTF, Ppol, 0, 0, Band, 400, 900);

@@ -138,6 +138,42 @@ long alis_dev_phaser_size(alis_dev_world *D, int id);
 int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
 int alis_dev_phaser_remove(alis_dev_world *D, int id);
 
+/* Near-to-far projection, the surface sums (src/farfld.c:248-459).  For each of n
+ * directions ("columns"), every component of the DFT field at frequency q on every
+ * PML-bounded face of the phaser box, summed against two weight vectors along the face's
+ * axes: in-plane phase factors d e^{i k p} (with the mirrored term of a symmetry plane) and,
+ * on x and y faces, the discrete plane-wave profile T(z) of the direction.  The host
+ * library computes the profiles (a 1-D recursion per polar angle, src/farfld.c:126-245)
+ * and assembles the far field from the sums (src/farfld.c:462-514).
+ *
+ * Axis tables run over the box (index 0 = its first cell): E and H cell widths, node and
+ * half-node (i - 1/2) positions.  Profiles: nslot slots of 6 components (Ex Ey Ez Hx Hy Hz)
+ * of nz complex values for absolute k = kLo .. kLo+nz-1.  A column gives (kx, ky) and, per
+ * polarisation, the slot and for each T component the slot component it comes from and a
+ * real factor (the azimuthal rotation; factor 0 for an absent component).
+ * Output S[col][face kind 0..5][component E1 E2 H1 H2][polarisation 0..1] as re/im pairs:
+ * on z faces the sums do not involve T and only polarisation 0 is written; absent faces and
+ * unwritten entries are zero. */
+typedef struct {
+	const double *dxE, *dxH, *xN, *xH;
+	const double *dyE, *dyH, *yN, *yH;
+	const double *dzE, *dzH;
+	int symX, symY, symZ;       /* a symmetry plane at the min face */
+	double coskx, cosky, coskz;
+	int nslot, kLo, nz;
+	const double *T;            /* [slot][component][nz][re,im] */
+} alis_dev_ff_geom;
+
+typedef struct {
+	double kx, ky;
+	int slot[2];
+	int src[2][6];
+	double fac[2][6];
+} alis_dev_ff_col;
+
+int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *g, int n,
+                      const alis_dev_ff_col *cols, double *S);
+
 /* Batches: several worlds -- with the TF/SF auxiliary worlds of each, which the caller
  * lists too -- stepped in shared launches.  All worlds join the first one's stream.  A half
  * step of a batch is alis_dev_batch_core (curl, CPML and ADE of every world in one launch),

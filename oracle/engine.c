@@ -20,6 +20,8 @@
  *
  * Every access goes through a bounds-checked index map that aborts on a miss.
  */
+#include <complex.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -452,6 +454,122 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 	if (!Q) return fail("no such phaser");
 	free(Q->accD), free(Q->accF), free(Q->cf);
 	Q->accD = 0, Q->accF = 0, Q->cf = 0, Q->live = 0;
+	return 0;
+}
+
+/* ---- near-to-far surface sums (src/farfld.c:248-459) ----
+ * For each column the loops of farFieldX/Y/Z: an inner sum along the face's last axis and an
+ * outer sum (compensated) along its first, of F times the two weight vectors. */
+
+typedef double complex dcx;
+
+/* In-plane phase weight of cell p (0..np-1) of an axis: E widths at nodes with the two end
+ * cells halved and the mirror image subtracted (half = 0), or H widths at half nodes, the
+ * first cell left out and the mirror image added (half = 1) (src/farfld.c:262-268,282-288). */
+static dcx ffPhase(int half, const double *d, const double *pos, int p, int np, double k, int sym, double cosk)
+{
+	double w = half ? (p ? 1 : 0) : (p == 0 ? 0.5 : 0) + (p == np-1 ? 0.5 : 0) + (p > 0 && p < np-1 ? 1 : 0);
+	dcx v = d[p] * cexp(I * k * pos[p]);
+	if (sym) v += (half ? 1 : -1) * conj(v) * cosk;
+	return w * v;
+}
+
+/* Profile weight of cell v of the box along z for T component t of polarisation pol. */
+static dcx ffProfile(int half, const alis_dev_ff_geom *g, const alis_dev_ff_col *c, int pol, int t, int k0, int v, int nk)
+{
+	const int src = c->src[pol][t];
+	const double fac = c->fac[pol][t];
+	if (src < 0 || !fac || (half && !v)) return 0;
+	const double *T = g->T + 2 * ((long)c->slot[pol] * 6 + src) * g->nz;
+	const int k = k0 + v;
+#define TV(kk) (T[2*((kk)-g->kLo)] + I * T[2*((kk)-g->kLo)+1])
+	dcx z;
+	if (half) z = g->dzH[v] * (TV(k) + (g->symZ ? TV(1-k) * g->coskz : 0));
+	else z = ((v == 0 ? 0.5 : 0) + (v == nk-1 ? 0.5 : 0) + (v > 0 && v < nk-1 ? 1 : 0)) * g->dzE[v] * (TV(k) - (g->symZ ? TV(-k) * g->coskz : 0));
+#undef TV
+	return fac * z;
+}
+
+static void ffAdd(dcx *s, dcx *c, dcx x)
+{
+	double sr = creal(*s), si = cimag(*s), cr = creal(*c), ci = cimag(*c);
+	double t = sr + creal(x);
+	cr += fabs(sr) >= fabs(creal(x)) ? (sr - t) + creal(x) : (creal(x) - t) + sr;
+	sr = t;
+	t = si + cimag(x);
+	ci += fabs(si) >= fabs(cimag(x)) ? (si - t) + cimag(x) : (cimag(x) - t) + si;
+	si = t;
+	*s = sr + I * si, *c = cr + I * ci;
+}
+
+int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *g, int n,
+                      const alis_dev_ff_col *cols, double *S)
+{
+	phs *Q = getPhaser(D, id);
+	if (!Q) return fail("no such phaser");
+	if (q < 0 || q >= Q->nf) return fail("phaser frequency out of range");
+	const int ni = Q->iMax-Q->iMin+1, nj = Q->jMax-Q->jMin+1, nk = Q->kMax-Q->kMin+1;
+	int side = 0;
+	for (int f = 0; f < Q->nface; f++) side |= Q->kind[f] < 4;
+	if (side) {
+		const int lo = g->symZ ? MIN(Q->kMin, -Q->kMax) : Q->kMin, hi = g->symZ ? MAX(Q->kMax, 1 - Q->kMin) : Q->kMax;
+		if (!g->T || g->kLo > lo || g->kLo + g->nz - 1 < hi) return fail("far field: profiles do not cover the phaser box");
+		for (int c = 0; c < n; c++)
+			for (int p = 0; p < 2; p++) {
+				if (cols[c].slot[p] < 0 || cols[c].slot[p] >= g->nslot) return fail("far field: profile slot out of range");
+				for (int t = 0; t < 6; t++) if (cols[c].src[p][t] >= 6) return fail("far field: no such profile component");
+			}
+	}
+	memset(S, 0, (size_t)n * 96 * sizeof(double));
+	/* T components paired with the face components (E1 E2 H1 H2) of x and y faces */
+	static const int tX[4] = {5, 4, 2, 1}, tY[4] = {3, 5, 0, 2};
+	for (int col = 0; col < n; col++) {
+		const alis_dev_ff_col *c = &cols[col];
+		for (int f = 0; f < Q->nface; f++) {
+			const int kind = Q->kind[f];
+			for (int comp = 0; comp < 4; comp++) {
+				const long base = 2 * (((long)q*4 + comp) * Q->ncell + Q->off[f]);
+#define FV(u, w, nw) (Q->dft64 ? Q->accD[base + 2*((long)(u)*(nw)+(w))] + I * Q->accD[base + 2*((long)(u)*(nw)+(w))+1] \
+                               : Q->accF[base + 2*((long)(u)*(nw)+(w))] + I * Q->accF[base + 2*((long)(u)*(nw)+(w))+1])
+				const int blockA = comp == 0 || comp == 3;
+				for (int pol = 0; pol < (kind < 4 ? 2 : 1); pol++) {
+					dcx s = 0, e = 0;
+					if (kind >= 4) {          /* F[i][j]: Y (node) with X (half) or X (node) with Y (half) */
+						for (int i = 0; i < ni; i++) {
+							dcx r = 0;
+							for (int j = 0; j < nj; j++)
+								r += FV(i, j, nj) * (blockA ? ffPhase(0, g->dyE, g->yN, j, nj, c->ky, g->symY, g->cosky)
+								                            : ffPhase(1, g->dyH, g->yH, j, nj, c->ky, g->symY, g->cosky));
+							ffAdd(&s, &e, r * (blockA ? ffPhase(1, g->dxH, g->xH, i, ni, c->kx, g->symX, g->coskx)
+							                          : ffPhase(0, g->dxE, g->xN, i, ni, c->kx, g->symX, g->coskx)));
+						}
+					}
+					else if (kind < 2) {      /* F[j][k]: Y (half) with Z_N, or Y (node) with Z_H */
+						for (int j = 0; j < nj; j++) {
+							dcx r = 0;
+							for (int k = 0; k < nk; k++)
+								r += FV(j, k, nk) * ffProfile(!blockA, g, c, pol, tX[comp], Q->kMin, k, nk);
+							ffAdd(&s, &e, r * (blockA ? ffPhase(1, g->dyH, g->yH, j, nj, c->ky, g->symY, g->cosky)
+							                          : ffPhase(0, g->dyE, g->yN, j, nj, c->ky, g->symY, g->cosky)));
+						}
+					}
+					else {                    /* F[i][k]: X (node) with Z_H, or X (half) with Z_N */
+						for (int i = 0; i < ni; i++) {
+							dcx r = 0;
+							for (int k = 0; k < nk; k++)
+								r += FV(i, k, nk) * ffProfile(blockA, g, c, pol, tY[comp], Q->kMin, k, nk);
+							ffAdd(&s, &e, r * (blockA ? ffPhase(0, g->dxE, g->xN, i, ni, c->kx, g->symX, g->coskx)
+							                          : ffPhase(1, g->dxH, g->xH, i, ni, c->kx, g->symX, g->coskx)));
+						}
+					}
+#undef FV
+					s += e;
+					double *o = S + 2 * ((((long)col * 6 + kind) * 4 + comp) * 2 + pol);
+					o[0] = creal(s), o[1] = cimag(s);
+				}
+			}
+		}
+	}
 	return 0;
 }
 

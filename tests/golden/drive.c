@@ -387,12 +387,14 @@ static void stepScenario(scen *c)
 	sample(&c->T, c->W, c->F, c->pts, c->np);
 }
 
-static void dumpScenario(scen *c)
+/* The far field is evaluated from stepped worlds only (on the device, which a zero-step
+ * setup check does not have). */
+static void dumpScenario(scen *c, int stepped)
 {
 	if (c->P) sync(c->W);
 	putTrace(c->name, &c->T);
 	dumpState(c->name, c->W);
-	if (c->P) dumpPhasers(c->name, c->P, 1);
+	if (c->P) dumpPhasers(c->name, c->P, stepped);
 }
 
 /* Run the scenarios of a comma-separated list.  One scenario steps with updateE/updateH;
@@ -429,7 +431,7 @@ static int run(const char *names, int steps)
 		}
 		for (int a = 0; a < n; a++) stepScenario(&c[a]);
 	}
-	for (int a = 0; a < n; a++) dumpScenario(&c[a]);
+	for (int a = 0; a < n; a++) dumpScenario(&c[a], steps > 0);
 	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P;
 	NRUN = n;
 	LAST = c[0].W, LASTP = c[0].P;
@@ -536,6 +538,31 @@ int alis_physics_dispersion(int cells, int m, int p, int steps, int fp64, double
 	free(e), free(buf);
 	deleteWorld(W);
 	return c;
+}
+
+/* The host library's double-precision far field (farFieldD, alis_amd/host/farfield.c). */
+double complex farFieldD(phaser P, tfield pol(phaser,float,float), double theta, double phi);
+
+/* Run a scenario with phasers and evaluate the far field of every frequency of its phaser
+ * array in double precision at the n directions (theta[a], phi[a]): out[(q*n + a)*4 + ...]
+ * = P re, P im, S re, S im.  Returns the number of frequencies written (at most maxq), or a
+ * negative code. */
+int alis_drive_farfield(const char *name, int steps, int n, const double *theta, const double *phi, int maxq, double *out)
+{
+	OUT = fopen("/dev/null", "w");
+	int rc = run(name, steps);
+	fclose(OUT);
+	OUT = 0;
+	if (rc || !LASTP) return rc ? -rc : -3;
+	int nq = 0;
+	for (phaser Q = LASTP; Q->f && nq < maxq; Q++, nq++)
+		for (int a = 0; a < n; a++) {
+			double complex p = farFieldD(Q, Ppol, theta[a], phi[a]), s = farFieldD(Q, Spol, theta[a], phi[a]);
+			double *o = out + ((long)nq * n + a) * 4;
+			o[0] = creal(p), o[1] = cimag(p), o[2] = creal(s), o[3] = cimag(s);
+		}
+	releaseRun();
+	return nq;
 }
 #endif
 

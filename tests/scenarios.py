@@ -30,6 +30,8 @@ def drive_lib(kind):
         lib.alis_drive.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]
         lib.alis_drive_raw.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]
         lib.alis_physics_dispersion.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_double)] * 2
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.alis_drive_farfield.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, dp]
         _loaded[kind] = lib
     return _loaded[kind]
 
@@ -55,6 +57,25 @@ def run_raw(kind, name, steps, precision):
     head = np.frombuffer(b[:64], dtype=np.int64)
     data = np.frombuffer(b[64:], dtype=np.float64)
     return data[:head[4]].copy(), data[head[4]:head[4] + head[5]].copy()
+
+
+# directions of the far-field checks (degrees): normal, oblique, both sides of the 90-degree
+# average, the lower half space, and azimuths that mix the polarisation components
+FF_THETA = (0.0, 20.0, 45.0, 89.5, 90.0, 120.0, 135.0, 180.0, 271.0, 330.0)
+FF_PHI = (0.0, 30.0, 0.0, 10.0, 45.0, 200.0, 75.0, 0.0, 300.0, 135.0)
+
+
+def farfield(kind, name, steps, precision, theta=FF_THETA, phi=FF_PHI, maxq=64):
+    """Double-precision far field of a scenario's phasers: array [frequency][direction][P re,
+    P im, S re, S im]."""
+    os.environ["ALIS_PRECISION"] = precision
+    n = len(theta)
+    th = (ctypes.c_double * n)(*theta)
+    ph = (ctypes.c_double * n)(*phi)
+    out = (ctypes.c_double * (maxq * n * 4))()
+    nq = drive_lib(kind).alis_drive_farfield(name.encode(), steps, n, th, ph, maxq, out)
+    assert nq > 0, f"far field of {name} returned {nq}"
+    return np.array(out[:nq * n * 4]).reshape(nq, n, 4)
 
 
 def dispersion(kind, cells, m, p, steps, fp64):
