@@ -14,6 +14,7 @@ void resetField3(float ***f, int iMin, int iMax, int jMin, int jMax, int kMin, i
 void makeAuxFieldArrays(world W, vfield *F, coeffs *C);
 
 /* media.c */
+int shapeBounds(const object *O, float b[6]);   /* basic shapes: widened bounds, else 0 */
 int dispersiveMatter(matter M);   /* 2: nonlinear tensor, 1: poles, 0: plain dielectric */
 float complex polePermittivity(matter M, float w, int p);
 

@@ -82,11 +82,85 @@ float complex polePermittivity(matter M, float w, int p)
 
 /* 1 when grid point (i,j,k) (fractional indices allowed) lies in object o and in none of
  * the objects before it (earlier objects take precedence; src/object.c:70-75). */
+/* Candidate objects of a point, for the inside queries of putObjectArray: the x-y plane
+ * of the grid in NB x NB bins; an object with basic-shape bounds is listed in the bins
+ * those bounds meet, any other object in every bin.  Lists are in object order.  A query
+ * still evaluates the shapes themselves, so the answers are those of a scan over all
+ * objects; only objects that cannot contain the point are skipped. */
+#define NB 64
+typedef struct {
+	float x0, y0, sx, sy;       /* bin origin and inverse widths */
+	int start[NB*NB+1], *list;
+	float (*b)[6];              /* per object: bounds, or infinite */
+} objIndex;
+
+static objIndex *buildIndex(world W, object *O, int N)
+{
+	objIndex *X = calloc(1, sizeof *X);
+	const float x0 = W->itox(W, W->iMIN - 1), x1 = W->itox(W, W->iMAX + 1);
+	const float y0 = W->jtoy(W, W->jMIN - 1), y1 = W->jtoy(W, W->jMAX + 1);
+	X->x0 = x0, X->y0 = y0;
+	X->sx = x1 > x0 ? NB / (x1 - x0) : 0, X->sy = y1 > y0 ? NB / (y1 - y0) : 0;
+	MALLOC(X->b, 0, N ? N : 1);
+	int (*range)[4];
+	MALLOC(range, 0, N ? N : 1);
+	long total = 0;
+	for (int o = 0; o < N; o++) {
+		float *b = X->b[o];
+		if (!shapeBounds(&O[o], b)) for (int a = 0; a < 6; a++) b[a] = a & 1 ? INF : -INF;
+		float f[4] = { (b[0] - x0) * X->sx, (b[1] - x0) * X->sx, (b[2] - y0) * X->sy, (b[3] - y0) * X->sy };
+		for (int a = 0; a < 4; a++) range[o][a] = f[a] < 0 ? 0 : f[a] > NB - 1 ? NB - 1 : (int)f[a];
+		total += (long)(range[o][1] - range[o][0] + 1) * (range[o][3] - range[o][2] + 1);
+	}
+	int *count = calloc(NB * NB, sizeof(int));
+	for (int o = 0; o < N; o++)
+		for (int u = range[o][0]; u <= range[o][1]; u++) for (int v = range[o][2]; v <= range[o][3]; v++) count[u*NB+v]++;
+	for (int c = 0; c < NB * NB; c++) X->start[c+1] = X->start[c] + count[c];
+	MALLOC(X->list, 0, total ? total : 1);
+	memset(count, 0, NB * NB * sizeof(int));
+	for (int o = 0; o < N; o++)
+		for (int u = range[o][0]; u <= range[o][1]; u++) for (int v = range[o][2]; v <= range[o][3]; v++)
+			X->list[X->start[u*NB+v] + count[u*NB+v]++] = o;
+	free(count), free(range);
+	return X;
+}
+
+static void freeIndex(objIndex *X)
+{
+	if (!X) return;
+	free(X->list), free(X->b), free(X);
+}
+
+static inline int binOf(const objIndex *X, float x, float y)
+{
+	float u = (x - X->x0) * X->sx, v = (y - X->y0) * X->sy;
+	int a = u < 0 ? 0 : u > NB - 1 ? NB - 1 : (int)u, b = v < 0 ? 0 : v > NB - 1 ? NB - 1 : (int)v;
+	return a * NB + b;
+}
+
+static inline int maybeInside(const float *b, float x, float y, float z)
+{
+	return b[0] <= x && x <= b[1] && b[2] <= y && y <= b[3] && b[4] <= z && z <= b[5];
+}
+
+/* The index of the putObjectArray call in progress (read by its parallel loops). */
+static objIndex *IDX;
+
+/* 1 if the point (grid coordinates) lies in object o and in no earlier object. */
 static float ownedBy(world W, object *O, int o, float i, float j, float k)
 {
 	float x = W->itox(W, i), y = W->jtoy(W, j), z = W->ktoz(W, k);
 	if (!O[o].S(O[o], x, y, z)) return 0;
-	while (o--) if (O[o].S(O[o], x, y, z)) return 0;
+	if (!IDX) {
+		while (o--) if (O[o].S(O[o], x, y, z)) return 0;
+		return 1;
+	}
+	const int c = binOf(IDX, x, y);
+	for (int q = IDX->start[c]; q < IDX->start[c+1]; q++) {
+		const int p = IDX->list[q];
+		if (p >= o) break;
+		if (maybeInside(IDX->b[p], x, y, z) && O[p].S(O[p], x, y, z)) return 0;
+	}
 	return 1;
 }
 
@@ -152,8 +226,19 @@ static box3 *objectBounds(world W, object *O, int N)
 		#pragma omp for schedule(dynamic)
 		for (int i = W->iMIN; i <= W->iMAX; i++) for (int j = W->jMIN; j <= W->jMAX; j++) for (int k = W->kMIN; k <= W->kMAX; k++) {
 			float x = W->itox(W, i), y = W->jtoy(W, j), z = W->ktoz(W, k);
-			int o = 0;
-			while (O[o].S && !O[o].S(O[o], x, y, z)) o++;
+			int o = N;
+			if (IDX) {
+				/* the first object containing the node, among the candidates of its bin */
+				const int c = binOf(IDX, x, y);
+				for (int q = IDX->start[c]; q < IDX->start[c+1]; q++) {
+					const int p = IDX->list[q];
+					if (maybeInside(IDX->b[p], x, y, z) && O[p].S(O[p], x, y, z)) { o = p; break; }
+				}
+			}
+			else {
+				o = 0;
+				while (O[o].S && !O[o].S(O[o], x, y, z)) o++;
+			}
 			if (o < N) {
 				b[o].i0 = MIN(i, b[o].i0), b[o].i1 = MAX(i, b[o].i1);
 				b[o].j0 = MIN(j, b[o].j0), b[o].j1 = MAX(j, b[o].j1);
@@ -303,6 +388,7 @@ void putObjectArray(world W, matter *M, object *O)
 	for (N = 0; O[N].S; N++) if (dispersiveMatter(M[N])) W->CE->N++;
 	printf("\rPreparing to put %d objects.\n", N);
 	W->subN = W->Res.spec.subN = W->Res.spec.subN ? W->Res.spec.subN : W->CE->N ? 1 : 12;
+	IDX = buildIndex(W, O, N);
 	box3 *B = objectBounds(W, O, N);
 	coeffs *C = W->CE;
 	makeMediumArrays(W, W->E, C, permittivity(W, M[N], 0), permittivity(W, M[N], 1), permittivity(W, M[N], 2));
@@ -360,6 +446,8 @@ void putObjectArray(world W, matter *M, object *O)
 		}
 	}
 	free(B);
+	freeIndex(IDX);
+	IDX = 0;
 	printf("\rPutting objects: %d done.\n", N);
 	fflush(stdout);
 }

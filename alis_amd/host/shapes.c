@@ -141,6 +141,27 @@ int HRodZY(object O, float x, float y, float z)
 	return e.z0 <= z && z <= e.z1 && e.x0 < x && x < e.x1 && fabsf(U(x,e.x0,e.x1))+fabsf(U(y,e.y0,e.y1))*2 < 2;
 }
 
+/* Bounds of a basic shape, widened by a margin that covers the rounding of its inside test:
+ * every point the shape contains lies inside b = {x0, x1, y0, y1, z0, z1}.  0 for compound
+ * and imported shapes, whose points can lie anywhere. */
+int shapeBounds(const object *O, float b[6])
+{
+	static shape *const basic[] = { Box, Ball, Diamond, BiconeX, BiconeY, BiconeZ, RodX, RodY, RodZ,
+		DRodX, DRodY, DRodZ, HRodXY, HRodYZ, HRodZX, HRodXZ, HRodYX, HRodZY };
+	int known = 0;
+	for (unsigned a = 0; a < sizeof basic / sizeof basic[0]; a++) known |= O->S == basic[a];
+	if (!known) return 0;
+	extent e = extentOf(O);
+	const float lo[3] = { fminf(e.x0, e.x1), fminf(e.y0, e.y1), fminf(e.z0, e.z1) };
+	const float hi[3] = { fmaxf(e.x0, e.x1), fmaxf(e.y0, e.y1), fmaxf(e.z0, e.z1) };
+	for (int a = 0; a < 3; a++) {
+		const float m = 1e-3f * (hi[a] - lo[a]) + 1e-3f * fmaxf(fabsf(lo[a]), fabsf(hi[a])) + 1e-6f;
+		b[2*a] = lo[a] - m, b[2*a+1] = hi[a] + m;
+		if (!(b[2*a] <= b[2*a+1])) b[2*a] = -INF, b[2*a+1] = INF;   /* NaN-safe */
+	}
+	return 1;
+}
+
 /* Compound shapes; O.p points at the operand object(s). */
 int Translation(object O, float x, float y, float z)
 {

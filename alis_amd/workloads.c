@@ -17,6 +17,13 @@
  *              of period a; complex fields), 8 point dipoles per world at positions and
  *              phases drawn from a splitmix64 stream (seed 2024).  All worlds step together
  *              with updateWorldsE/updateWorldsH.
+ *   metasurface an 8 x 8 array of TiO2 pillars (eps 6.25, 600 nm tall, radii drawn uniformly
+ *              from 60-120 nm by a splitmix64 stream, seed 2024) on glass (eps 2.25) in a
+ *              periodic cell of 8 x 400 nm: dx = 5 nm, Courant 0.5, periodic x and y, 16-cell
+ *              CPML in z, normal-incidence x-polarised plane wave with a 400-900 nm band
+ *              waveform on a TF/SF box, one 550 nm phaser on the z faces.  `cells` is the
+ *              x/y period in cells (640: 641 x 641 x 320 nodes including CPML, z edge half
+ *              of it); the pitch stays 400 nm, so the array has cells/80 pillars a side.
  *
  * One step is updateE + updateH (+ updatePhaser for nanoblock), src/update.c:275-341.
  */
@@ -33,6 +40,7 @@ typedef struct {
 	double probeSum;
 	int nw;                     /* worlds stepped together (W = Ws[0]) */
 	world *Ws;
+	object *parts;              /* objects the world refers to after putObjects */
 } wl;
 
 static uint64_t splitmix(uint64_t *s)
@@ -107,6 +115,34 @@ void *wl_create(const char *name, int cells, int fieldBits, int dftBits)
 		float pts[8][3] = {{0,0,50}, {50,0,0}, {0,-50,0}, {80,80,0}, {-100,30,60}, {40,-120,-40}, {150,150,150}, {-170,0,-90}};
 		memcpy(c->pts, pts, sizeof pts);
 		c->probes = 8;
+	}
+	else if (!strcmp(name, "metasurface")) {
+		const float dx = 5, half = dx * cells / 2;
+		const int nz = cells / 2, d = nz - 2 * 17;            /* z nodes without the CPML */
+		dom D = {{-half, half}, {-half, half}, {-dx * ((d - 2) / 2), dx * (d / 2)}};
+		res R = {2.5, {dx}, {1240}};
+		sur S = {{PBC}, {PBC}, {PML}, {16}};
+		c->W = createWorld(D, R, S, "");
+		/* one object per pillar (small bounding boxes keep the sub-cell averaging local),
+		 * then the glass (a layer reaching outside the TF/SF box comes after the objects
+		 * inside it, which the TF/SF background leaves out), then the air background */
+		const int side = cells / 80, np = side * side;
+		object *O = calloc(np + 2, sizeof(object));
+		matter *M = calloc(np + 2, sizeof(matter));
+		uint64_t seed = 2024;
+		for (int a = 0; a < np; a++) {
+			const float x = -half + 400 * (a % side) + 200, y = -half + 400 * (a / side) + 200;
+			const float r = (float)(60 + 60 * uniform(&seed));
+			M[a] = n(2.5), O[a] = (object) {RodZ, {{x - r, x + r}, {y - r, y + r}, {0, 600}}};
+		}
+		M[np] = n(1.5), O[np] = (object) {Box, {{-INF, INF}, {-INF, INF}, {-INF, 0}}};
+		M[np+1] = Air;
+		putObjectArray(c->W, M, O);
+		free(M);
+		c->parts = O;               /* the TF/SF background is cut from the same objects */
+		dom DTF = {{-half, half}, {-half, half}, {-40, 640}};
+		planewave(c->W, DTF, Ppol, 0, 0, Band, 400, 900);
+		c->P = createPhaser(c->W, 550, -INF, INF, -INF, INF, -80, 670);
 	}
 	else if (!strcmp(name, "kpoints")) {
 		uint64_t seed = 2024;
@@ -186,7 +222,8 @@ int wl_phaser_info(void *ctx, long *surfaceCells)
 	return n;
 }
 
-/* CPML layer thickness (Num+1 cells) per face, 0 where the face is not PML. */
+/* Grid nodes and CPML layer thickness (Num+1 cells) per face, 0 where the face is not PML;
+ * returns 1 when the E update reads 1/eps (objects were put, src/update.c:109 vs :114). */
 int wl_geometry(void *ctx, int *num, int *pml)
 {
 	wl *c = ctx;
@@ -195,7 +232,7 @@ int wl_geometry(void *ctx, int *num, int *pml)
 	pml[0] = W->xMinSur==PML ? W->iMin-W->iMIN : 0, pml[1] = W->xMaxSur==PML ? W->iMAX-W->iMax : 0;
 	pml[2] = W->yMinSur==PML ? W->jMin-W->jMIN : 0, pml[3] = W->yMaxSur==PML ? W->jMAX-W->jMax : 0;
 	pml[4] = W->zMinSur==PML ? W->kMin-W->kMIN : 0, pml[5] = W->zMaxSur==PML ? W->kMAX-W->kMax : 0;
-	return W->CE->N;
+	return W->CE->x != 0;
 }
 
 /* Sum of squares of every E and H component over the grid (sequential, in double), plus
@@ -224,12 +261,37 @@ double wl_checksum(void *ctx)
 	return s;
 }
 
+/* Far-field images of the workload's phaser over an X x Y cylindrical (theta, phi) map,
+ * written as text files named <dir>/ff-*.txt (src/farfld.c:525-594). */
+void wl_farfield(void *ctx, int X, int Y, const char *dir)
+{
+	wl *c = ctx;
+	if (!c->P) return;
+#ifndef ALIS_REF
+	farFieldProfile(c->W, c->P, CylindericalMap, X, Y, txt, "%s/ff-", dir);
+#else
+	/* the reference build has no image writers (slices.c needs libpng): the same two
+	 * evaluations per direction that its farFieldProfile makes (src/farfld.c:553-556) */
+	(void)dir;
+	double sum = 0;
+	#pragma omp parallel for schedule(dynamic) reduction(+:sum)
+	for (long xy = 0; xy < (long)X * Y; xy++) {
+		float ang[2];
+		CylindericalMap(xy / Y, xy % Y, X, Y, ang);
+		float complex p = farField(c->P, Ppol, ang[0], ang[1]), s = farField(c->P, Spol, ang[0], ang[1]);
+		sum += crealf(p * conjf(p)) + crealf(s * conjf(s));
+	}
+	c->probeSum += sum;
+#endif
+}
+
 void wl_free(void *ctx)
 {
 	wl *c = ctx;
 	if (c->P) deletePhaser(c->P);
 	for (int q = 0; q < c->nw; q++) deleteWorld(c->Ws[q]);
 	free(c->Ws);
+	free(c->parts);
 	free(c);
 }
 
@@ -254,7 +316,7 @@ void *wl_device_of(void *ctx, int q)
 #include <unistd.h>
 int main(int argc, char **argv)
 {
-	if (argc < 5) return fprintf(stderr, "usage: %s NAME CELLS WARMUP STEPS\n", argv[0]), 2;
+	if (argc < 5) return fprintf(stderr, "usage: %s NAME CELLS WARMUP STEPS [X Y DIR]\n", argv[0]), 2;
 	/* the library's progress lines go to /dev/null, the result line to the original stdout */
 	fflush(stdout);
 	FILE *keep = fdopen(dup(1), "w");
@@ -268,7 +330,16 @@ int main(int argc, char **argv)
 	wl_step(c, steps);
 	clock_gettime(CLOCK_MONOTONIC, &b);
 	double sec = (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
-	fprintf(keep, "%.9g %ld %d %.17g\n", sec / steps, wl_cells(c), omp_get_max_threads(), wl_checksum(c));
+	double ff = 0;
+	if (argc >= 8) {
+		/* far-field map of X x Y directions after the steps: seconds per direction */
+		const int X = atoi(argv[5]), Y = atoi(argv[6]);
+		clock_gettime(CLOCK_MONOTONIC, &a);
+		wl_farfield(c, X, Y, argv[7]);
+		clock_gettime(CLOCK_MONOTONIC, &b);
+		ff = ((b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec)) / ((double)X * Y);
+	}
+	fprintf(keep, "%.9g %ld %d %.17g %.9g\n", sec / (steps ? steps : 1), wl_cells(c), omp_get_max_threads(), wl_checksum(c), ff);
 	fclose(keep);
 	return 0;
 }

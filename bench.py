@@ -65,6 +65,7 @@ def load():
     w.wl_checksum.restype = ctypes.c_double
     w.wl_checksum.argtypes = [ctypes.c_void_p]
     w.wl_worlds.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+    w.wl_farfield.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
     alis.alis_dev_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     alis.alis_dev_timing_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
     alis.alis_dev_synchronize.argtypes = [ctypes.c_void_p]
@@ -91,7 +92,7 @@ def curl_bytes(num, pml, s, has_eps):
     return e + slab * 4 * s, h + slab * 4 * s
 
 
-def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps):
+def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None):
     ctx = w.wl_create(name.encode(), cells, fbits, dbits)
     if not ctx:
         sys.exit(f"bench: unknown workload {name}")
@@ -111,20 +112,30 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps):
         kern.append((ms.value, n.value))
     alis.alis_dev_timing(dev, 0)
     num, pml = (ctypes.c_int * 3)(), (ctypes.c_int * 6)()
-    nobj = w.wl_geometry(ctx, num, pml)
+    eps = w.wl_geometry(ctx, num, pml)
     surf = ctypes.c_long()
     nf = w.wl_phaser_info(ctx, ctypes.byref(surf))
     cplx = ctypes.c_int()
     nw = w.wl_worlds(ctx, ctypes.byref(cplx))
     out = {"cells": w.wl_cells(ctx), "sec": t1 - t0, "kern": kern, "num": list(num), "pml": list(pml),
-           "nobj": nobj, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev),
+           "eps": eps, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev),
            "worlds": nw, "parts": 1 + cplx.value}
+    if farfield:
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            X, Y = farfield
+            w.wl_farfield(ctx, 1, 1, d.encode())         # first call: GEMM library set-up
+            hip.hipDeviceSynchronize()
+            t0 = time.perf_counter()
+            w.wl_farfield(ctx, X, Y, d.encode())
+            hip.hipDeviceSynchronize()
+            out["farfield_sec"] = time.perf_counter() - t0
     w.wl_free(ctx)
     return out
 
 
 def roofline(r, s, dft_s, steps):
-    be, bh = curl_bytes(r["num"], r["pml"], s, r["nobj"] > 0)
+    be, bh = curl_bytes(r["num"], r["pml"], s, r["eps"] > 0)
     be, bh = be * r["worlds"] * r["parts"], bh * r["worlds"] * r["parts"]    # one launch covers all
     (mse, ne), (msh, nh) = r["kern"]
     te, th = mse / max(ne, 1) * 1e-3, msh / max(nh, 1) * 1e-3
@@ -136,7 +147,7 @@ def roofline(r, s, dft_s, steps):
     return {
         "bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
         "frac": round(ach / PEAK_HBM, 4), "traffic": None,
-        "kernel": f"k_curl ({dom[0]} half step, {'with' if dom[0] == 'E' and r['nobj'] else 'no'} 1/eps)",
+        "kernel": f"k_curl ({dom[0]} half step, {'with' if dom[0] == 'E' and r['eps'] else 'no'} 1/eps)",
         "bytes_per_launch": dom[1], "launch_ms": round(dom[2] * 1e3, 5), "launches": dom[3],
         "other_curl": {"bytes_per_launch": bh if dom[0] == "E" else be,
                        "launch_ms": round((th if dom[0] == "E" else te) * 1e3, 5)},
@@ -194,6 +205,28 @@ def cpu_baseline(cells, steps):
                       f"{steps} steps after 2 warm-up steps, OMP_NUM_THREADS={threads}"}
 
 
+def farfield_cpu_baseline(X, Y, cells=160):
+    """The reference's own far-field code (oracle/_ref/wl_ref) per direction on a reduced
+    metasurface (cells x cells z faces), scaled to the full faces by their cell count: the
+    cost of a direction is one pass over the face cells per component."""
+    exe = ROOT / "oracle" / "_ref" / "wl_ref"
+    if not exe.exists():
+        return None
+    import tempfile
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([str(exe), "metasurface", str(cells), "0", "0", str(X), str(Y), d],
+                           capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode or len(r.stdout.split()) < 5:
+        print(f"bench: far-field CPU baseline failed (rc {r.returncode}): {r.stdout[-200:]} {r.stderr[-400:]}",
+              file=sys.stderr)
+        return None
+    per = float(r.stdout.split()[4])
+    return {"sec_per_direction": per, "cores": cores, "kind": "reference", "face_cells": (cells + 1) ** 2,
+            "sample": f"reference farField (P and S) at {X}x{Y} directions on the {cells}-cell metasurface"}
+
+
 def serving(alis, args):
     """The serving load: seeded Poisson arrivals of scattering requests (grid edges 64-192
     with 8-cell CPML, a Drude-Lorentz sphere of 10-40 cells, 1000-6000 steps, 16-wavelength
@@ -228,6 +261,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kpoints", type=int, default=64, help="worlds of the batched k-point sweep (0: skip)")
     ap.add_argument("--serve", type=int, default=24, help="requests of the serving run (0: skip)")
+    ap.add_argument("--metasurface", type=int, default=640, help="x/y period of the metasurface run in cells (0: skip)")
+    ap.add_argument("--ff-angles", type=int, default=181, help="far-field map of N x N directions")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-seed", type=int, default=2024)
     ap.add_argument("--serve-slots", type=int, default=8, help="most requests on the device at once")
@@ -286,6 +321,31 @@ def main():
         }
     if args.serve > 0:
         line["serving"] = serving(alis, args)
+    if args.metasurface > 0:
+        msteps = max(args.steps // 5, 1)
+        X, Y = args.ff_angles, args.ff_angles
+        with quiet():
+            rm = run(alis, w, hip, "metasurface", args.metasurface, 32, 64, args.warmup, msteps, farfield=(X, Y))
+        vm = rm["cells"] * msteps / rm["sec"] / 1e6
+        rlm = roofline(rm, 4, 8, msteps)
+        ms = {
+            "workload": f"metasurface: {args.metasurface // 80}x{args.metasurface // 80} TiO2 pillars (eps 6.25, r 60-120 nm "
+                        "seeded, 600 nm) on glass, dx 5 nm, periodic x/y, 16-cell CPML z, TF/SF plane wave 400-900 nm, "
+                        f"{rm['num'][0]}x{rm['num'][1]}x{rm['num'][2]} nodes; one 550 nm phaser on the z faces",
+            "value": round(vm, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rm["sec"] / msteps * 1e3, 4),
+            "steps": msteps, "roofline": {k: rlm[k] for k in ("bound", "achieved", "peak", "unit", "frac", "kernel",
+                                                            "bytes_per_launch", "launch_ms")},
+            "farfield": {"directions": X * Y, "map": f"{X}x{Y} cylindrical (theta, phi)", "dtype": "f64",
+                         "sec": round(rm["farfield_sec"], 4),
+                         "directions_per_s": round(X * Y / rm["farfield_sec"], 1)},
+        }
+        fc = farfield_cpu_baseline(X, 8) if args.cpu_steps > 0 else None
+        if fc:
+            scale = rm["num"][0] * rm["num"][1] / fc["face_cells"]
+            fc["estimated_sec_full"] = round(fc["sec_per_direction"] * scale * X * Y, 1)
+            fc["scaled_to"] = f"{rm['num'][0]}x{rm['num'][1]} face cells"
+            ms["farfield"]["cpu_baseline"] = fc
+        line["metasurface"] = ms
     line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(line), flush=True)
 

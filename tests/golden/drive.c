@@ -310,6 +310,35 @@ static void obliqueSetup(scen *c)
 	memcpy(c->F, F, sizeof F);
 	memcpy(c->pts, pts, sizeof pts);
 }
+/* A periodic cell of 2 x 2 dielectric pillars of different radii on glass under a
+ * normal-incidence plane wave (the metasurface geometry at small size): one object per
+ * pillar through putObjectArray, the glass after them. */
+static object pillarObjects[6];
+static matter pillarMatter[6];
+static void pillarsSetup(scen *c)
+{
+	dom D = {{-200, 200}, {-200, 200}, {-160, 400}};
+	res R = {5, {10}, {1240}};
+	sur S = {{PBC}, {PBC}, {PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	const float r[4] = {32, 47, 55, 38};
+	for (int a = 0; a < 4; a++) {
+		float x = -100 + 200 * (a % 2), y = -100 + 200 * (a / 2);
+		pillarMatter[a] = n(2.5);
+		pillarObjects[a] = (object) {RodZ, {{x - r[a], x + r[a]}, {y - r[a], y + r[a]}, {0, 240}}};
+	}
+	pillarMatter[4] = n(1.5), pillarObjects[4] = (object) {Box, {{-INF, INF}, {-INF, INF}, {-INF, 0}}};
+	pillarMatter[5] = Air, pillarObjects[5] = (object) {0};
+	putObjectArray(W, pillarMatter, pillarObjects);
+	dom DTF = {{-200, 200}, {-200, 200}, {-40, 300}};
+	planewave(W, DTF, Ppol, 0, 0, Band, 400, 900);
+	phaser P = createPhasers(W, 500, 700, 100, -INF, INF, -INF, INF, -80, 340);
+	field *F[4] = { Ex, Ey, Hy, Ez };
+	float pts[4][3] = {{0,0,100}, {-100,-100,120}, {60,-30,-60}, {100,100,320}};
+	c->W = W, c->P = P, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
+}
 static void periodicSetup(scen *c)
 {
 	dom D = {{40}, {40}, {-40, 80}};
@@ -376,6 +405,7 @@ static int setupScenario(scen *c, const char *name)
 	if (!strcmp(name, "nanoblock")) return nanoblockSetup(c), 0;
 	if (!strcmp(name, "oblique")) return obliqueSetup(c), 0;
 	if (!strcmp(name, "periodic")) return periodicSetup(c), 0;
+	if (!strcmp(name, "pillars")) return pillarsSetup(c), 0;
 	if (!strcmp(name, "hexagonal")) return hexagonalSetup(c), 0;
 	if (!strcmp(name, "multipole")) return multipoleSetup(c), 0;
 	return fprintf(stderr, "unknown scenario %s\n", name), 2;

@@ -27,6 +27,7 @@ SCENARIOS = {
     "periodic": 200,
     "hexagonal": 200,
     "multipole": 200,
+    "pillars": 200,
 }
 
 

@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 GOLDEN = ROOT / "tests" / "golden"
 MANIFEST = json.loads((GOLDEN / "manifest.json").read_text())
 SCENARIOS = MANIFEST["scenarios"]
-WITH_PHASERS = ("symband", "nanoblock", "oblique")
+WITH_PHASERS = ("symband", "nanoblock", "oblique", "pillars")
 
 LIBS = {"hip": GOLDEN / "libdrive.so", "oracle": GOLDEN / "libdrive_oracle.so"}
 _loaded = {}

@@ -8,7 +8,8 @@ half space and azimuths that rotate the polarisation components:
   frequency);
 * default precision (FP32 fields, FP64 DFT and far field): within 1e-5 of the FP64 oracle.
 nanoblock has all six PML faces; symband a z symmetry plane with Bloch x/y; oblique an
-oblique plane wave in a periodic cell (z faces only, array factor).
+oblique plane wave in a periodic cell (z faces only, array factor); pillars a periodic
+pillar array under normal incidence (z faces only, no array factor).
 """
 import numpy as np
 import pytest
@@ -17,7 +18,7 @@ from scenarios import SCENARIOS, farfield
 
 pytestmark = pytest.mark.gpu
 
-CASES = ("nanoblock", "symband", "oblique")
+CASES = ("nanoblock", "symband", "oblique", "pillars")
 
 
 def _rel(a, b):
