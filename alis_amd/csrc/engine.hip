@@ -1880,30 +1880,38 @@ __device__ inline double2 cmad(double2 acc, double2 a, double2 b)
 	return acc;
 }
 
-// S[col][kind][comp][pol] = sum_v alpha_v(col, pol) C_m[v][col].
-__global__ void k_ff_reduce(double2 *S, FFReduce r, const alis_dev_ff_col *cols)
+// Partial sums over the split-th range of v: part[((split * 4 + m) * 2 + pol) * nc + col]
+// = sum_v alpha_v(col, pol) C_m[v][col].  Each thread owns one column of one range; the
+// ranges run in parallel (blockIdx.y) and k_ff_finish adds them in order.
+constexpr int FF_SPLIT = 16;
+
+__global__ void k_ff_reduce(double2 *part, FFReduce r, const alis_dev_ff_col *cols)
 {
-	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	const int col = blockIdx.x * blockDim.x + threadIdx.x, split = blockIdx.y;
 	if (col >= r.nc) return;
+	const int per = (r.nv + FF_SPLIT - 1) / FF_SPLIT, v0 = split * per, v1 = min(r.nv, v0 + per);
 	const alis_dev_ff_col &c = cols[col];
+	double2 acc[4][2] = {};
 	if (!r.profile) {
-		double2 acc[4] = {};
 		const double k = r.ph.useKy ? c.ky : c.kx;
-		for (int v = 0; v < r.nv; v++) {
+		for (int v = v0; v < v1; v++) {
 			const double2 a = ffPhaseAt(r.ph, v, k);
-			for (int m = 0; m < r.nm; m++) acc[m] = cmad(acc[m], a, r.C[((size_t)m * r.nv + v) * r.nc + col]);
+#pragma unroll
+			for (int m = 0; m < 4; m++)
+				if (m < r.nm) acc[m][0] = cmad(acc[m][0], a, r.C[((size_t)m * r.nv + v) * r.nc + col]);
 		}
-		for (int m = 0; m < r.nm; m++) S[((size_t)col * 24 + r.out[m]) * 2] = acc[m];
-		return;
 	}
-	for (int pol = 0; pol < 2; pol++)
-		for (int m = 0; m < r.nm; m++) {
-			const int src = c.src[pol][r.tcomp[m]];
-			const double fac = c.fac[pol][r.tcomp[m]];
-			double2 acc = make_double2(0.0, 0.0);
-			if (src >= 0 && fac != 0.0) {
+	else {
+#pragma unroll
+		for (int pol = 0; pol < 2; pol++)
+#pragma unroll
+			for (int m = 0; m < 4; m++) {
+				if (m >= r.nm) continue;
+				const int src = c.src[pol][r.tcomp[m]];
+				const double fac = c.fac[pol][r.tcomp[m]];
+				if (src < 0 || fac == 0.0) continue;
 				const double2 *Tb = r.T + ((size_t)c.slot[pol] * 6 + src) * r.nz - r.kLo;   // index by absolute k
-				for (int v = 0; v < r.nv; v++) {
+				for (int v = v0; v < v1; v++) {
 					if (r.half && !v) continue;
 					const int k = r.k0 + v;
 					double2 t = Tb[k];
@@ -1914,10 +1922,30 @@ __global__ void k_ff_reduce(double2 *S, FFReduce r, const alis_dev_ff_col *cols)
 					}
 					const double w = r.half ? 1.0 : (v == 0 ? 0.5 : 0.0) + (v == r.nv - 1 ? 0.5 : 0.0) + (v > 0 && v < r.nv - 1 ? 1.0 : 0.0);
 					const double s = fac * w * r.dz[v];
-					acc = cmad(acc, make_double2(s * t.x, s * t.y), r.C[((size_t)m * r.nv + v) * r.nc + col]);
+					acc[m][pol] = cmad(acc[m][pol], make_double2(s * t.x, s * t.y), r.C[((size_t)m * r.nv + v) * r.nc + col]);
 				}
 			}
-			S[((size_t)col * 24 + r.out[m]) * 2 + pol] = acc;
+	}
+#pragma unroll
+	for (int m = 0; m < 4; m++)
+#pragma unroll
+		for (int pol = 0; pol < 2; pol++)
+			part[((size_t)(split * 4 + m) * 2 + pol) * r.nc + col] = acc[m][pol];
+}
+
+// S[col][kind][comp][pol] = the partial sums of k_ff_reduce, added in range order.
+__global__ void k_ff_finish(double2 *S, const double2 *part, FFReduce r)
+{
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	if (col >= r.nc) return;
+	for (int m = 0; m < r.nm; m++)
+		for (int pol = 0; pol < (r.profile ? 2 : 1); pol++) {
+			double2 s = make_double2(0.0, 0.0);
+			for (int split = 0; split < FF_SPLIT; split++) {
+				const double2 p = part[((size_t)(split * 4 + m) * 2 + pol) * r.nc + col];
+				s.x += p.x, s.y += p.y;
+			}
+			S[((size_t)col * 24 + r.out[m]) * 2 + pol] = s;
 		}
 }
 
@@ -1984,7 +2012,8 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 	const size_t fB = D->dft64 ? 0 : al((size_t)4 * Q.ncell * 16);
 	const size_t sB = al((size_t)n * 96 * 8);
 	const size_t betaB = al((size_t)npMax * NC * 16), cB = al((size_t)4 * nvMax * NC * 16);
-	if (ffWorkspace(D, tabB + tB + colB + fB + sB + betaB + cB)) return -1;
+	const size_t partB = al((size_t)FF_SPLIT * 4 * 2 * NC * 16);
+	if (ffWorkspace(D, tabB + tB + colB + fB + sB + betaB + cB + partB)) return -1;
 	char *w = (char *)D->ffWork;
 	double *tab = (double *)w;                          w += tabB;
 	double2 *Tdev = (double2 *)w;                       w += tB;
@@ -1992,7 +2021,8 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 	double *Fw = (double *)w;                           w += fB;
 	double2 *Sdev = (double2 *)w;                       w += sB;
 	double2 *beta = (double2 *)w;                       w += betaB;
-	double2 *Cm = (double2 *)w;
+	double2 *Cm = (double2 *)w;                         w += cB;
+	double2 *part = (double2 *)w;
 
 	// tables: dxE dxH xN xH (ni) dyE dyH yN yH (nj) dzE dzH (nk)
 	std::vector<double> t;
@@ -2065,7 +2095,9 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 			r.profile = b.profile, r.half = b.half, r.ph = b.alpha;
 			r.dz = b.half ? dzH : dzE;
 			r.T = Tdev, r.nz = g->nz, r.kLo = g->kLo, r.k0 = Q.kMin, r.symZ = g->symZ, r.coskz = g->coskz;
-			k_ff_reduce<<<blocksFor(nc), 256, 0, D->st>>>(Sdev + (size_t)c0 * 48, r, cdev + c0);
+			k_ff_reduce<<<dim3(blocksFor(nc), FF_SPLIT), 256, 0, D->st>>>(part, r, cdev + c0);
+			HIPCHECK(hipGetLastError());
+			k_ff_finish<<<blocksFor(nc), 256, 0, D->st>>>(Sdev + (size_t)c0 * 48, part, r);
 			HIPCHECK(hipGetLastError());
 		}
 	}

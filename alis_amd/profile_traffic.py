@@ -18,6 +18,16 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 
 
+def kernels_sha256():
+    """SHA-256 of the device kernels of the engine (the "Kernels" section of engine.hip,
+    which holds the curl kernel and everything it inlines): a traffic measurement stays valid
+    while this section is unchanged."""
+    src = (ROOT / "alis_amd/csrc/engine.hip").read_text()
+    a = src.index("// Kernels\n")
+    b = src.index("// Device world\n", a)
+    return hashlib.sha256(src[a:b].encode()).hexdigest()
+
+
 def rows(d):
     f = next(Path(d).rglob("*counter_collection.csv"))
     return list(csv.DictReader(f.open()))
@@ -31,7 +41,7 @@ def main():
     cW = max(float(r["Counter_Value"]) for r in W if "copyBuffer" in r["Kernel_Name"])
     fs, ws = nbytes / 1024 / cF, nbytes / 1024 / cW
     cells = max(int(r["Grid_Size"]) for r in F if "k_curl" in r["Kernel_Name"])
-    out = {"engine_sha256": hashlib.sha256((ROOT / "alis_amd/csrc/engine.hip").read_bytes()).hexdigest(),
+    out = {"kernels_sha256": kernels_sha256(),
            "cells": cells, "fields": "fp32",
            "calibration": {"copy_bytes": nbytes, "FETCH_SIZE_kib": cF, "WRITE_SIZE_kib": cW,
                            "fetch_scale": round(fs, 5), "write_scale": round(ws, 5)}}

@@ -274,6 +274,7 @@ void wl_farfield(void *ctx, int X, int Y, const char *dir)
 	 * evaluations per direction that its farFieldProfile makes (src/farfld.c:553-556) */
 	(void)dir;
 	double sum = 0;
+	farField(c->P, Ppol, 0, 0);     /* sets up the coefficients once, before the threads read them */
 	#pragma omp parallel for schedule(dynamic) reduction(+:sum)
 	for (long xy = 0; xy < (long)X * Y; xy++) {
 		float ang[2];

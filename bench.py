@@ -169,16 +169,16 @@ def calibration_copy(hip, nbytes):
 
 
 def measured_traffic(cells, fields):
-    """HBM bytes per E-curl launch measured with rocprofv3 counters on this engine source
+    """HBM bytes per E-curl launch measured with rocprofv3 counters on these device kernels
     (profiles/k_curl_nanoblock256_traffic.json, made by alis_amd/profile_traffic.py), or
-    None when that measurement was taken on another version of the engine or workload."""
+    None when that measurement was taken on other kernels or another workload."""
     f = ROOT / "profiles" / "k_curl_nanoblock256_traffic.json"
     if not f.exists():
         return None
-    import hashlib
+    sys.path.insert(0, str(ROOT))
+    from alis_amd.profile_traffic import kernels_sha256
     m = json.loads(f.read_text())
-    src = (ROOT / "alis_amd" / "csrc" / "engine.hip").read_bytes()
-    if m.get("engine_sha256") != hashlib.sha256(src).hexdigest() or m.get("cells") != cells or m.get("fields") != fields:
+    if m.get("kernels_sha256") != kernels_sha256() or m.get("cells") != cells or m.get("fields") != fields:
         return None
     return m["E"]["traffic_bytes"]
 
@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--serve", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--metasurface", type=int, default=640, help="x/y period of the metasurface run in cells (0: skip)")
     ap.add_argument("--ff-angles", type=int, default=181, help="far-field map of N x N directions")
+    ap.add_argument("--only", choices=["kpoints", "serving", "metasurface"],
+                    help="run this section alone and print its object (for profiling)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-seed", type=int, default=2024)
     ap.add_argument("--serve-slots", type=int, default=8, help="most requests on the device at once")
@@ -279,6 +281,12 @@ def main():
     alis, w, hip = load()
     if args.copy_mib:
         calibration_copy(hip, args.copy_mib << 20)
+    if args.only:
+        # one section alone (for profiling its kernels): its object as the JSON line
+        out = {"kpoints": lambda: batched(alis, w, hip, args), "serving": lambda: serving(alis, args),
+               "metasurface": lambda: metasurface(alis, w, hip, args)}[args.only]()
+        print(json.dumps({"section": args.only, **out}), flush=True)
+        return
     with quiet():
         r32 = run(alis, w, hip, "nanoblock", args.cells, 32, 64, args.warmup, args.steps)
     value = r32["cells"] * args.steps / r32["sec"] / 1e6
@@ -302,52 +310,62 @@ def main():
         line["fp64"] = {"value": round(v64, 1), "ms_per_step": round(r64["sec"] / max(args.steps // 2, 1) * 1e3, 4),
                         "roofline_frac": roofline(r64, 8, 8, max(args.steps // 2, 1))["frac"], "fp32_over_fp64": round(value / v64, 2)}
     if args.kpoints > 0:
-        ksteps = max(args.steps // 2, 1)
-        with quiet():
-            rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
-            r1 = run(alis, w, hip, "kpoints", 1, 64, 64, args.warmup, ksteps)
-        vb = rb["cells"] * ksteps / rb["sec"] / 1e6
-        v1 = r1["cells"] * ksteps / r1["sec"] / 1e6
-        rlb = roofline(rb, 4, 8, ksteps)
-        line["batched"] = {
-            "workload": f"{args.kpoints} k-point worlds along Gamma-X-M, each 65x65x50 (64x64x31 cells + 8-cell CPML in z), "
-                        "Bloch-periodic x/y, complex fields, eps=12 PhC slab with air holes, 8 random-phase dipoles",
-            "value": round(vb, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rb["sec"] / ksteps * 1e3, 4),
-            "roofline": {"bound": "hbm", "achieved": rlb["achieved"], "peak": rlb["peak"], "unit": "GB/s", "frac": rlb["frac"],
-                         "kernel": "k_curl_batch (E half step of all worlds, both parts)", "bytes_per_launch": rlb["bytes_per_launch"],
-                         "launch_ms": rlb["launch_ms"]},
-            "fp64_single_world": {"value": round(v1, 1), "ms_per_step": round(r1["sec"] / ksteps * 1e3, 4)},
-            "batched_fp32_over_single_fp64": round(vb / v1, 2),
-        }
+        line["batched"] = batched(alis, w, hip, args)
     if args.serve > 0:
         line["serving"] = serving(alis, args)
     if args.metasurface > 0:
-        msteps = max(args.steps // 5, 1)
-        X, Y = args.ff_angles, args.ff_angles
-        with quiet():
-            rm = run(alis, w, hip, "metasurface", args.metasurface, 32, 64, args.warmup, msteps, farfield=(X, Y))
-        vm = rm["cells"] * msteps / rm["sec"] / 1e6
-        rlm = roofline(rm, 4, 8, msteps)
-        ms = {
-            "workload": f"metasurface: {args.metasurface // 80}x{args.metasurface // 80} TiO2 pillars (eps 6.25, r 60-120 nm "
-                        "seeded, 600 nm) on glass, dx 5 nm, periodic x/y, 16-cell CPML z, TF/SF plane wave 400-900 nm, "
-                        f"{rm['num'][0]}x{rm['num'][1]}x{rm['num'][2]} nodes; one 550 nm phaser on the z faces",
-            "value": round(vm, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rm["sec"] / msteps * 1e3, 4),
-            "steps": msteps, "roofline": {k: rlm[k] for k in ("bound", "achieved", "peak", "unit", "frac", "kernel",
-                                                            "bytes_per_launch", "launch_ms")},
-            "farfield": {"directions": X * Y, "map": f"{X}x{Y} cylindrical (theta, phi)", "dtype": "f64",
-                         "sec": round(rm["farfield_sec"], 4),
-                         "directions_per_s": round(X * Y / rm["farfield_sec"], 1)},
-        }
-        fc = farfield_cpu_baseline(X, 8) if args.cpu_steps > 0 else None
-        if fc:
-            scale = rm["num"][0] * rm["num"][1] / fc["face_cells"]
-            fc["estimated_sec_full"] = round(fc["sec_per_direction"] * scale * X * Y, 1)
-            fc["scaled_to"] = f"{rm['num'][0]}x{rm['num'][1]} face cells"
-            ms["farfield"]["cpu_baseline"] = fc
-        line["metasurface"] = ms
+        line["metasurface"] = metasurface(alis, w, hip, args)
     line["cpu_baseline"] = cpu_baseline(args.cells, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(line), flush=True)
+
+
+def batched(alis, w, hip, args):
+    """The k-point band sweep: all worlds in shared launches, against one world alone in FP64."""
+    ksteps = max(args.steps // 2, 1)
+    with quiet():
+        rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
+        r1 = run(alis, w, hip, "kpoints", 1, 64, 64, args.warmup, ksteps)
+    vb = rb["cells"] * ksteps / rb["sec"] / 1e6
+    v1 = r1["cells"] * ksteps / r1["sec"] / 1e6
+    rlb = roofline(rb, 4, 8, ksteps)
+    return {
+        "workload": f"{args.kpoints} k-point worlds along Gamma-X-M, each 65x65x50 (64x64x31 cells + 8-cell CPML in z), "
+                    "Bloch-periodic x/y, complex fields, eps=12 PhC slab with air holes, 8 random-phase dipoles",
+        "value": round(vb, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rb["sec"] / ksteps * 1e3, 4),
+        "roofline": {"bound": "hbm", "achieved": rlb["achieved"], "peak": rlb["peak"], "unit": "GB/s", "frac": rlb["frac"],
+                     "kernel": "k_curl_batch (E half step of all worlds, both parts)", "bytes_per_launch": rlb["bytes_per_launch"],
+                     "launch_ms": rlb["launch_ms"]},
+        "fp64_single_world": {"value": round(v1, 1), "ms_per_step": round(r1["sec"] / ksteps * 1e3, 4)},
+        "batched_fp32_over_single_fp64": round(vb / v1, 2),
+    }
+
+
+def metasurface(alis, w, hip, args):
+    """The large periodic metasurface: step throughput and the far-field map of its phaser."""
+    msteps = max(args.steps // 5, 1)
+    X, Y = args.ff_angles, args.ff_angles
+    with quiet():
+        rm = run(alis, w, hip, "metasurface", args.metasurface, 32, 64, args.warmup, msteps, farfield=(X, Y))
+    vm = rm["cells"] * msteps / rm["sec"] / 1e6
+    rlm = roofline(rm, 4, 8, msteps)
+    ms = {
+        "workload": f"metasurface: {args.metasurface // 80}x{args.metasurface // 80} TiO2 pillars (eps 6.25, r 60-120 nm "
+                    "seeded, 600 nm) on glass, dx 5 nm, periodic x/y, 16-cell CPML z, TF/SF plane wave 400-900 nm, "
+                    f"{rm['num'][0]}x{rm['num'][1]}x{rm['num'][2]} nodes; one 550 nm phaser on the z faces",
+        "value": round(vm, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rm["sec"] / msteps * 1e3, 4),
+        "steps": msteps, "roofline": {k: rlm[k] for k in ("bound", "achieved", "peak", "unit", "frac", "kernel",
+                                                        "bytes_per_launch", "launch_ms")},
+        "farfield": {"directions": X * Y, "map": f"{X}x{Y} cylindrical (theta, phi)", "dtype": "f64",
+                     "sec": round(rm["farfield_sec"], 4),
+                     "directions_per_s": round(X * Y / rm["farfield_sec"], 1)},
+    }
+    fc = farfield_cpu_baseline(X, 8) if args.cpu_steps > 0 else None
+    if fc:
+        scale = rm["num"][0] * rm["num"][1] / fc["face_cells"]
+        fc["estimated_sec_full"] = round(fc["sec_per_direction"] * scale * X * Y, 1)
+        fc["scaled_to"] = f"{rm['num'][0]}x{rm['num'][1]} face cells"
+        ms["farfield"]["cpu_baseline"] = fc
+    return ms
 
 
 if __name__ == "__main__":
