@@ -137,106 +137,7 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 template<class T, bool HASC, bool NT, bool NTG>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
-
-	const long id = a.L.at(i, j, k);
-	const long oI = a.alt * a.L.si, oJ = a.alt * a.L.sj, oK = a.alt;
-	const T gx = ldS<NTG>(a.G0 + id), gy = ldS<NTG>(a.G1 + id), gz = ldS<NTG>(a.G2 + id);
-	const T dx = a.dtx[i - a.iMIN], dy = a.dty[j - a.jMIN], dz = a.dtz[k - a.kMIN];
-	T fx = ldS<NT>(a.F0 + id), fy = ldS<NT>(a.F1 + id), fz = ldS<NT>(a.F2 + id);
-	T cx = 1, cy = 1, cz = 1;
-
-	// E: conductivity scaling of every object, then -sum_p J_p of every object
-	// (src/update.c:89-108), before the curl
-	if (a.alt > 0 && a.nobj) {
-		for (int n = 0; n < a.nobj; n++) {
-			const ObjDev<T> &O = a.obj[n];
-			if (!O.scale || i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
-			const long o = O.LO.at(i, j, k);
-			T f0 = O.fill[0][o], f1 = O.fill[1][o], f2 = O.fill[2][o];
-			fx *= (T(1) - O.a[0] * f0) / (T(1) + O.a[0] * f0);
-			fy *= (T(1) - O.a[1] * f1) / (T(1) + O.a[1] * f1);
-			fz *= (T(1) - O.a[2] * f2) / (T(1) + O.a[2] * f2);
-		}
-		for (int n = 0; n < a.nobj; n++) {
-			const ObjDev<T> &O = a.obj[n];
-			if (i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
-			const long o = O.LO.at(i, j, k);
-			for (int p = 0; p < O.NK; p++) {
-				fx -= O.J[a.m][0][p][o];
-				fy -= O.J[a.m][1][p][o];
-				fz -= O.J[a.m][2][p][o];
-			}
-		}
-	}
-
-	T ux = dz * (gy - a.G1[id + oK]) + dy * (a.G2[id + oJ] - gz);
-	T uy = dx * (gz - a.G2[id + oI]) + dz * (a.G0[id + oK] - gx);
-	T uz = dy * (gx - a.G0[id + oJ]) + dx * (a.G1[id + oI] - gy);
-	if (HASC) {
-		cx = ldS<NT>(a.C0 + id), cy = ldS<NT>(a.C1 + id), cz = ldS<NT>(a.C2 + id);
-		fx = fx + ux * cx, fy = fy + uy * cy, fz = fz + uz * cz;
-	}
-	else {
-		fx = fx + ux, fy = fy + uy, fz = fz + uz;
-	}
-
-#pragma unroll
-	for (int f = 0; f < 6; f++) {
-		const CurlFace<T> &P = a.face[f];
-		if (!P.on) continue;
-		const int ax = f % 3;
-		const int c = ax == 0 ? i : ax == 1 ? j : k;
-		if (c < P.lo || c > P.hi) continue;
-		const long p = P.L.at(i, j, k);
-		const int q = c - (ax == 0 ? a.iMIN : ax == 1 ? a.jMIN : a.kMIN);
-		const T b = P.b[q], ca = P.ca[q], cb = P.cb[q];
-		if (ax == 0) {
-			T pa = b * ldS<NT>(P.pa + p) + ca * (gz - a.G2[id + oI]);
-			T pb = b * ldS<NT>(P.pb + p) + cb * (gy - a.G1[id + oI]);
-			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
-			fy = fy + (HASC ? pa * cy : pa);
-			fz = fz + (HASC ? pb * cz : pb);
-		}
-		else if (ax == 1) {
-			T pa = b * ldS<NT>(P.pa + p) + ca * (gx - a.G0[id + oJ]);
-			T pb = b * ldS<NT>(P.pb + p) + cb * (gz - a.G2[id + oJ]);
-			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
-			fz = fz + (HASC ? pa * cz : pa);
-			fx = fx + (HASC ? pb * cx : pb);
-		}
-		else {
-			T pa = b * ldS<NT>(P.pa + p) + ca * (gy - a.G1[id + oK]);
-			T pb = b * ldS<NT>(P.pb + p) + cb * (gx - a.G0[id + oK]);
-			stS<NT>(P.pa + p, pa), stS<NT>(P.pb + p, pb);
-			fx = fx + (HASC ? pa * cx : pa);
-			fy = fy + (HASC ? pb * cy : pb);
-		}
-	}
-	stS<NT>(a.F0 + id, fx), stS<NT>(a.F1 + id, fy), stS<NT>(a.F2 + id, fz);
-
-	// H: polarisation currents on the new E (src/update.c:128-158); E is this kernel's G
-	if (a.alt < 0 && a.nobj) {
-		const T e[3] = { gx, gy, gz };
-		for (int n = 0; n < a.nobj; n++) {
-			const ObjDev<T> &O = a.obj[n];
-			if (i < O.i0 || i > O.i1 || j < O.j0 || j > O.j1 || k < O.k0 || k > O.k1) continue;
-			const long o = O.LO.at(i, j, k);
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				const T fl = O.fill[c][o];
-				for (int p = 0; p < O.NJ; p++) {
-					T *J = O.J[a.m][c][p] + o;
-					*J = O.JJ[c][p] * *J + O.JE[c][p] * fl * e[c];
-				}
-				for (int p = O.NJ; p < O.NK; p++) {
-					T *J = O.J[a.m][c][p] + o, *K = O.K[a.m][c][p] + o;
-					T Jn = O.JJ[c][p] * *J + O.JK[c][p] * *K + O.JE[c][p] * fl * e[c];
-					T Kn = O.KJ[c][p] * Jn + O.KK[c][p] * *K + O.KE[c][p] * fl * e[c];
-					*J = Jn, *K = Kn;
-				}
-			}
-		}
-	}
+#include "curl_cell.inc"
 }
 
 template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
@@ -248,7 +149,11 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 		const int tile = xcdTile((int)blockIdx.x, (int)gridDim.x);
 		bx = tile % nkb, by = (tile / nkb) % njb, bz = tile / (nkb * njb);
 	}
-	curlCell<T, HASC, NT, NTG, BY>(a, bx, by, bz);
+	const int k = a.kMIN + bx * 64 + (int)threadIdx.x;
+	const int j = a.jMIN + by * BY + (int)threadIdx.y;
+	const int i = a.iMIN + bz;
+	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
+#include "curl_cell.inc"
 }
 
 // The entry owning workgroup b of a batch launch: start[] is increasing; every wave finds it

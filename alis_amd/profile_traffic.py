@@ -19,13 +19,14 @@ ROOT = Path(__file__).resolve().parent.parent
 
 
 def kernels_sha256():
-    """SHA-256 of the device kernels of the engine (the "Kernels" section of engine.hip,
-    which holds the curl kernel and everything it inlines): a traffic measurement stays valid
-    while this section is unchanged."""
+    """SHA-256 of the device kernels of the engine (the "Kernels" section of engine.hip and
+    the cell update it includes, alis_amd/csrc/curl_cell.inc): a traffic measurement stays
+    valid while these are unchanged."""
     src = (ROOT / "alis_amd/csrc/engine.hip").read_text()
     a = src.index("// Kernels\n")
     b = src.index("// Device world\n", a)
-    return hashlib.sha256(src[a:b].encode()).hexdigest()
+    cell = (ROOT / "alis_amd/csrc/curl_cell.inc").read_text()
+    return hashlib.sha256((src[a:b] + cell).encode()).hexdigest()
 
 
 def rows(d):
