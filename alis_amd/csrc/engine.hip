@@ -405,11 +405,10 @@ template<class T> struct PhsArgs {
 	double eE[MAXF][2], eH[MAXF][2];
 };
 
+// One face cell t of a phaser, frequencies qa .. qa+PHS_QPT-1 of the launch's range.
 template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
+__device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
 {
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= a.ncell) return;
 	int f = 0;
 	while (f + 1 < a.nface && t >= a.face[f + 1].off) f++;
 	const PhsFaceGeom &g = a.face[f];
@@ -437,7 +436,7 @@ __global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
 	}
 	TA *acc = (TA *)a.acc;
 	const TA half = TA(0.5);
-	const int qa = (int)blockIdx.y * PHS_QPT, qb = min(a.nq, qa + PHS_QPT);
+	const int qb = min(a.nq, qa + PHS_QPT);
 	for (int q = qa; q < qb; q++) {
 		const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
 		const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
@@ -452,6 +451,28 @@ __global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
 		stS<true>(p2, ldS<true>(p2) + hr * TA(h1) * half), stS<true>(p2 + 1, ldS<true>(p2 + 1) + hi * TA(h1) * half);
 		stS<true>(p3, ldS<true>(p3) + hr * TA(h2) * half), stS<true>(p3 + 1, ldS<true>(p3 + 1) + hi * TA(h2) * half);
 	}
+}
+
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= a.ncell) return;
+	phaserCell<T, TA>(a, t, (int)blockIdx.y * PHS_QPT);
+}
+
+// Several phasers (of several worlds) in one launch: entry e owns workgroups
+// [start[e], start[e+1]) along x; y runs over frequency pairs up to the largest entry.
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
+{
+	const int b = (int)blockIdx.x;
+	const int e = batchEntry(start, n, b);
+	const PhsArgs<T> &a = args[e];
+	const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+	const int qa = (int)blockIdx.y * PHS_QPT;
+	if (t >= a.ncell || qa >= a.nq) return;
+	phaserCell<T, TA>(a, t, qa);
 }
 
 // Pack a box of a device array into a compact buffer (float or double), and back.
@@ -583,6 +604,10 @@ struct alis_dev_world {
 	std::vector<DevPhaser> phs;
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
+	void *phsPinned = nullptr;       // staging of batched phaser updates led by this world
+	void *phsDev = nullptr;
+	size_t phsBytes = 0;
+	hipEvent_t phsDone = nullptr;    // the last staging copy has been made
 	rocblas_handle blas = nullptr;   // far-field products, created on first use
 	void *ffWork = nullptr;          // far-field workspace
 	size_t ffWorkBytes = 0;
@@ -1344,6 +1369,63 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 	return 0;
 }
 
+// One sample into several phasers in one launch (entries of one field and accumulator type).
+// Arguments and workgroup starts go through L's pinned staging buffer; before it is
+// rewritten, the copy made from it by the previous call must have been made.
+template<class T, class TA>
+int phaserBatchUpdate(alis_dev_world *L, const std::vector<PhsArgs<T>> &args)
+{
+	if (args.empty()) return 0;
+	std::vector<int> start;
+	int blocks = 0, ny = 1;
+	for (const PhsArgs<T> &a : args) {
+		start.push_back(blocks);
+		blocks += (int)blocksFor(a.ncell);
+		ny = std::max(ny, (a.nq + PHS_QPT - 1) / PHS_QPT);
+	}
+	const size_t ab = (args.size() * sizeof(PhsArgs<T>) + 255) / 256 * 256, bytes = ab + start.size() * sizeof(int);
+	if (bytes > L->phsBytes) {
+		if (L->phsPinned) {
+			HIPCHECK(hipStreamSynchronize(L->st));
+			HIPCHECK(hipHostFree(L->phsPinned));
+			HIPCHECK(hipFree(L->phsDev));
+		}
+		L->phsPinned = nullptr, L->phsDev = nullptr, L->phsBytes = 0;
+		HIPCHECK(hipHostMalloc(&L->phsPinned, bytes, hipHostMallocDefault));
+		HIPCHECK(hipMalloc(&L->phsDev, bytes));
+		L->phsBytes = bytes;
+		if (!L->phsDone) HIPCHECK(hipEventCreateWithFlags(&L->phsDone, hipEventDisableTiming));
+	}
+	else HIPCHECK(hipEventSynchronize(L->phsDone));
+	memcpy(L->phsPinned, args.data(), args.size() * sizeof(PhsArgs<T>));
+	memcpy((char *)L->phsPinned + ab, start.data(), start.size() * sizeof(int));
+	HIPCHECK(hipMemcpyAsync(L->phsDev, L->phsPinned, bytes, hipMemcpyHostToDevice, L->st));
+	HIPCHECK(hipEventRecord(L->phsDone, L->st));
+	dim3 grid(blocks, ny);
+	k_phaser_batch<T, TA><<<grid, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsDev, (const int *)((char *)L->phsDev + ab), (int)args.size());
+	HIPCHECK(hipGetLastError());
+	return 0;
+}
+
+// The arguments of one update launch of phaser P (frequencies qa .. qa+qn-1, qn <= MAXF).
+template<class T>
+PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH)
+{
+	PhsArgs<T> a{};
+	for (int c = 0; c < 3; c++) a.E[c] = (const T *)D->F[0][0][c], a.H[c] = (const T *)D->F[1][0][c];
+	a.L = D->L;
+	a.iMin = P.iMin, a.jMin = P.jMin, a.kMin = P.kMin, a.nface = P.nface;
+	for (int f = 0; f < P.nface; f++) a.face[f] = P.face[f];
+	a.ncell = P.ncell;
+	a.acc = P.acc;
+	a.q0 = qa, a.nq = qn;
+	for (int q = 0; q < qn; q++) {
+		a.eE[q][0] = expE[2 * q], a.eE[q][1] = expE[2 * q + 1];
+		a.eH[q][0] = expH[2 * q], a.eH[q][1] = expH[2 * q + 1];
+	}
+	return a;
+}
+
 // Make D use the stream of `to` (reference counted; the last user destroys a stream).
 int joinStream(alis_dev_world *D, alis_dev_world *to)
 {
@@ -1584,6 +1666,9 @@ void alis_dev_release(alis_dev_world *D)
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
+	if (D->phsPinned) hipHostFree(D->phsPinned);
+	if (D->phsDev) hipFree(D->phsDev);
+	if (D->phsDone) hipEventDestroy(D->phsDone);
 	if (D->blas) rocblas_destroy_handle(D->blas);
 	if (D->st && --*D->streamRefs == 0) {
 		hipStreamDestroy(D->st);
@@ -1695,6 +1780,32 @@ int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const doub
 	if (!P.ncell || !nq) return 0;
 	if (D->fp64) return D->dft64 ? phaserUpdate<double, double>(D, P, q0, nq, expE, expH) : phaserUpdate<double, float>(D, P, q0, nq, expE, expH);
 	return D->dft64 ? phaserUpdate<float, double>(D, P, q0, nq, expE, expH) : phaserUpdate<float, float>(D, P, q0, nq, expE, expH);
+}
+
+int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
+                                const double *const *expE, const double *const *expH, int n)
+{
+	if (n < 1) return 0;
+	for (int e = 0; e < n; e++) {
+		if (id[e] < 0 || id[e] >= (int)D[e]->phs.size() || !D[e]->phs[id[e]].live) return fail("no such phaser");
+		if (q0[e] < 0 || nq[e] < 0 || q0[e] + nq[e] > D[e]->phs[id[e]].nf) return fail("phaser frequency range out of bounds");
+		if (D[e]->fp64 != D[0]->fp64 || D[e]->dft64 != D[0]->dft64)
+			return fail("phasers updated together must share their field and accumulator precision");
+	}
+	for (int e = 1; e < n; e++) if (joinStream(D[e], D[0])) return -1;
+	std::vector<PhsArgs<float>> af;
+	std::vector<PhsArgs<double>> ad;
+	for (int e = 0; e < n; e++) {
+		DevPhaser &P = D[e]->phs[id[e]];
+		if (!P.ncell) continue;
+		for (int c = 0; c < nq[e]; c += MAXF) {
+			const int qn = std::min(MAXF, nq[e] - c);
+			if (D[0]->fp64) ad.push_back(phaserArgs<double>(D[e], P, q0[e] + c, qn, expE[e] + 2 * c, expH[e] + 2 * c));
+			else af.push_back(phaserArgs<float>(D[e], P, q0[e] + c, qn, expE[e] + 2 * c, expH[e] + 2 * c));
+		}
+	}
+	if (D[0]->fp64) return D[0]->dft64 ? phaserBatchUpdate<double, double>(D[0], ad) : phaserBatchUpdate<double, float>(D[0], ad);
+	return D[0]->dft64 ? phaserBatchUpdate<float, double>(D[0], af) : phaserBatchUpdate<float, float>(D[0], af);
 }
 
 long alis_dev_phaser_size(alis_dev_world *D, int id)

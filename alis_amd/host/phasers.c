@@ -104,16 +104,10 @@ void alisPhaserDevice(phaser B)
 	}
 }
 
-void updatePhaser(world W, phaser P)
+/* Phase factors e^{2 pi i f t} of the frequencies of P (to the end of its array) at the E
+ * and H times of W (src/phaser.c:97-98), as re/im pairs in eE/eH. */
+static void phaserFactors(world W, phaser P, int nq, double *eE, double *eH)
 {
-	phaser B = phaserBase(P);
-	if (!B) printf("updatePhaser: unknown phaser\n"), exit(1);
-	if (B->W != W) printf("updatePhaser: the phaser belongs to another world\n"), exit(1);
-	alisPhaserDevice(B);
-	int q0 = (int)(P - B), nq = B->count - q0;
-	static double *eE, *eH;
-	static int cap;
-	if (nq > cap) cap = nq, eE = realloc(eE, 2*cap*sizeof(double)), eH = realloc(eH, 2*cap*sizeof(double));
 	for (int q = 0; q < nq; q++) {
 		if (W->devDftPrec) {
 			double aE = 2 * M_PI * (double)P[q].f * (double)W->E->t, aH = 2 * M_PI * (double)P[q].f * (double)W->H->t;
@@ -127,9 +121,49 @@ void updatePhaser(world W, phaser P)
 			eH[2*q] = crealf(h), eH[2*q+1] = cimagf(h);
 		}
 	}
+}
+
+static phaser checkedBase(world W, phaser P, const char *who)
+{
+	phaser B = phaserBase(P);
+	if (!B) printf("%s: unknown phaser\n", who), exit(1);
+	if (B->W != W) printf("%s: the phaser belongs to another world\n", who), exit(1);
+	alisPhaserDevice(B);
+	return B;
+}
+
+void updatePhaser(world W, phaser P)
+{
+	phaser B = checkedBase(W, P, "updatePhaser");
+	int q0 = (int)(P - B), nq = B->count - q0;
+	static double *eE, *eH;
+	static int cap;
+	if (nq > cap) cap = nq, eE = realloc(eE, 2*cap*sizeof(double)), eH = realloc(eH, 2*cap*sizeof(double));
+	phaserFactors(W, P, nq, eE, eH);
 	alisFatal("phaser update", alis_dev_phaser_update(W->dev, B->devId, q0, nq, eE, eH));
 	B->hostStale = 1;
 	B->viewValid = 0;
+}
+
+/* updatePhaser(W[a], P[a]) for a = 0..n-1 in one launch (the phasers' worlds share their
+ * precision; they join one stream). */
+void updatePhaserArrays(world *W, phaser *P, int n)
+{
+	if (n < 1) return;
+	alis_dev_world **D = malloc(n * sizeof *D);
+	int *id = malloc(3 * n * sizeof(int)), *q0 = id + n, *nq = q0 + n;
+	double **eE = malloc(2 * n * sizeof(double *)), **eH = eE + n;
+	for (int a = 0; a < n; a++) {
+		phaser B = checkedBase(W[a], P[a], "updatePhaserArrays");
+		D[a] = W[a]->dev, id[a] = B->devId, q0[a] = (int)(P[a] - B), nq[a] = B->count - q0[a];
+		eE[a] = malloc(4 * (nq[a] ? nq[a] : 1) * sizeof(double)), eH[a] = eE[a] + 2 * nq[a];
+		phaserFactors(W[a], P[a], nq[a], eE[a], eH[a]);
+		B->hostStale = 1;
+		B->viewValid = 0;
+	}
+	alisFatal("phaser update", alis_dev_phaser_update_many(D, id, q0, nq, (const double *const *)eE, (const double *const *)eH, n));
+	for (int a = 0; a < n; a++) free(eE[a]);
+	free(eE), free(id), free(D);
 }
 
 void deletePhaser(phaser P)

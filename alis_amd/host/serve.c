@@ -134,6 +134,7 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 	double used = 0;
 	active *A = calloc(n ? n : 1, sizeof *A);
 	world *Ws = calloc(n ? n : 1, sizeof *Ws);
+	phaser *Ps = calloc(n ? n : 1, sizeof *Ps);
 	memset(out, 0, n * sizeof *out);
 	const double t0 = nowSec();
 	while (finished < n) {
@@ -159,11 +160,11 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 		}
 		int k = C->stepsPerTick;
 		for (int a = 0; a < na; a++) if (A[a].steps - A[a].done < k) k = A[a].steps - A[a].done;
-		for (int a = 0; a < na; a++) Ws[a] = A[a].W;
+		for (int a = 0; a < na; a++) Ws[a] = A[a].W, Ps[a] = A[a].P;
 		for (int s = 0; s < k; s++) {
 			updateWorldsE(Ws, na);
 			updateWorldsH(Ws, na);
-			for (int a = 0; a < na; a++) updatePhaser(A[a].W, A[a].P);
+			updatePhaserArrays(Ws, Ps, na);
 		}
 		if (joined) {
 			/* the new worlds are committed now: replace estimates by the real footprint */
@@ -195,5 +196,6 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 	}
 	free(A);
 	free(Ws);
+	free(Ps);
 	return 0;
 }

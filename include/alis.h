@@ -358,6 +358,9 @@ void syncWorld(world W);
  * same results as updateE (updateH) on each world in turn. */
 void updateWorldsE(world *W, int n);
 void updateWorldsH(world *W, int n);
+/* One DFT sample of the phaser arrays P[0..n-1] (P[a] recording world W[a]) in one launch;
+ * the same as updatePhaser(W[a], P[a]) on each. */
+void updatePhaserArrays(world *W, phaser *P, int n);
 
 #ifdef __cplusplus
 }

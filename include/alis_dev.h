@@ -133,6 +133,12 @@ int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double
  * yMin, yMax, zMin, zMax (x faces (j,k), y faces (i,k), z faces (i,j), last index fastest). */
 int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *id);
 int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH);
+/* One sample into n phasers at once (entry e: world D[e], phaser id[e], frequencies
+ * q0[e]..q0[e]+nq[e]-1 with factors expE[e]/expH[e]) in one launch, with the results of
+ * alis_dev_phaser_update on each; the worlds join D[0]'s stream and share their precision.
+ * No counterpart in the reference (one updatePhaser per phaser, src/phaser.c:83-110). */
+int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
+                                const double *const *expE, const double *const *expH, int n);
 int alis_dev_phaser_sync(alis_dev_world *D, int id);
 long alis_dev_phaser_size(alis_dev_world *D, int id);
 int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);

@@ -417,6 +417,13 @@ int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const doub
 	return 0;
 }
 
+int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
+                                const double *const *expE, const double *const *expH, int n)
+{
+	for (int e = 0; e < n; e++) if (alis_dev_phaser_update(D[e], id[e], q0[e], nq[e], expE[e], expH[e])) return -1;
+	return 0;
+}
+
 int alis_dev_phaser_sync(alis_dev_world *D, int id)
 {
 	phs *Q = getPhaser(D, id);

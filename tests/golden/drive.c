@@ -411,9 +411,9 @@ static int setupScenario(scen *c, const char *name)
 	return fprintf(stderr, "unknown scenario %s\n", name), 2;
 }
 
-static void stepScenario(scen *c)
+static void stepScenario(scen *c, int phaserDone)
 {
-	if (c->P) updatePhaser(c->W, c->P);
+	if (c->P && !phaserDone) updatePhaser(c->W, c->P);
 	sample(&c->T, c->W, c->F, c->pts, c->np);
 }
 
@@ -451,15 +451,23 @@ static int run(const char *names, int steps)
 			updateE(Ws[0]);
 			updateH(Ws[0]);
 		}
-		else {
+		int phasersDone = 0;
+		if (n > 1) {
 #ifndef ALIS_REF
+			/* several scenarios: shared launches for the half steps and the phasers */
 			updateWorldsE(Ws, n);
 			updateWorldsH(Ws, n);
+			world PW[8];
+			phaser PP[8];
+			int np = 0;
+			for (int a = 0; a < n; a++) if (c[a].P) PW[np] = c[a].W, PP[np++] = c[a].P;
+			updatePhaserArrays(PW, PP, np);
+			phasersDone = 1;
 #else
 			for (int a = 0; a < n; a++) updateE(Ws[a]), updateH(Ws[a]);
 #endif
 		}
-		for (int a = 0; a < n; a++) stepScenario(&c[a]);
+		for (int a = 0; a < n; a++) stepScenario(&c[a], phasersDone);
 	}
 	for (int a = 0; a < n; a++) dumpScenario(&c[a], steps > 0);
 	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P;
