@@ -475,6 +475,33 @@ __global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restri
 	phaserCell<T, TA>(a, t, qa);
 }
 
+// Several boxes of several arrays into one float buffer: segment s covers dst[off[s] ..
+// off[s+1]) with the cells of its box, k fastest.
+struct GatherSeg {
+	const void *src;
+	Lay L;
+	Box b;
+	long off;
+};
+
+template<class T>
+__global__ __launch_bounds__(256) void k_gather(const GatherSeg *__restrict__ seg, int nseg, long total, float *dst)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= total) return;
+	int lo = 0, hi = nseg - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) / 2;
+		if (seg[mid].off <= t) lo = mid;
+		else hi = mid - 1;
+	}
+	const GatherSeg &g = seg[lo];
+	const long l = t - g.off;
+	const int nk = g.b.k1 - g.b.k0 + 1, nj = g.b.j1 - g.b.j0 + 1;
+	const int dk = (int)(l % nk), dj = (int)(l / nk % nj), di = (int)(l / nk / nj);
+	dst[t] = (float)((const T *)g.src)[g.L.at(g.b.i0 + di, g.b.j0 + dj, g.b.k0 + dk)];
+}
+
 // Pack a box of a device array into a compact buffer (float or double), and back.
 template<class T, class U>
 __global__ __launch_bounds__(256) void k_pack(const T *src, Lay L, Box b, U *dst)
@@ -604,6 +631,9 @@ struct alis_dev_world {
 	std::vector<DevPhaser> phs;
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
+	void *gatherDev = nullptr;       // probe copies: segment table + gathered values
+	float *gatherPinned = nullptr;
+	size_t gatherBytes = 0, gatherPinnedBytes = 0;
 	void *phsPinned = nullptr;       // staging of batched phaser updates led by this world
 	void *phsDev = nullptr;
 	size_t phsBytes = 0;
@@ -1667,6 +1697,8 @@ void alis_dev_release(alis_dev_world *D)
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
 	if (D->phsPinned) hipHostFree(D->phsPinned);
+	if (D->gatherDev) hipFree(D->gatherDev);
+	if (D->gatherPinned) hipHostFree(D->gatherPinned);
 	if (D->phsDev) hipFree(D->phsDev);
 	if (D->phsDone) hipEventDestroy(D->phsDone);
 	if (D->blas) rocblas_destroy_handle(D->blas);
@@ -1705,6 +1737,97 @@ int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int
 {
 	Box b{i0, i1, j0, j1, k0, k1};
 	return D->fp64 ? syncHost<double>(D, b) : syncHost<float>(D, b);
+}
+
+// One world's share of alis_dev_sync_boxes: segments of E, H (every part) and the currents
+// of dispersive objects inside each box, gathered on the device and copied into pinned
+// memory (the caller waits for the stream and scatters).
+extern "C++" {
+struct GatherHost { float ***host; Box b; long off; };
+
+template<class T>
+int gatherBoxes(alis_dev_world *D, const std::vector<Box> &boxes, std::vector<GatherHost> &out)
+{
+	const alis_dev_world_desc *W = &D->W;
+	const Box fb = fieldBox(D);
+	std::vector<GatherSeg> seg;
+	long total = 0;
+	auto add = [&](const void *src, const Lay &L, const Box &b, float ***host) {
+		if (!b.count()) return;
+		seg.push_back(GatherSeg{src, L, b, total});
+		out.push_back(GatherHost{host, b, total});
+		total += b.count();
+	};
+	for (const Box &box : boxes) {
+		const Box b{std::max(box.i0, fb.i0), std::min(box.i1, fb.i1), std::max(box.j0, fb.j0), std::min(box.j1, fb.j1),
+		            std::max(box.k0, fb.k0), std::min(box.k1, fb.k1)};
+		for (int h = 0; h < 2; h++)
+			for (int m = 0; m <= D->cplx; m++)
+				for (int a = 0; a < 3; a++) add(D->F[h][m][a], D->L, b, W->field[h][m][a]);
+		for (size_t n = 0; n < D->obj.size(); n++) {
+			const DevObj &O = D->obj[n];
+			const Box c{std::max(box.i0, O.box.i0), std::min(box.i1, O.box.i1), std::max(box.j0, O.box.j0), std::min(box.j1, O.box.j1),
+			            std::max(box.k0, O.box.k0), std::min(box.k1, O.box.k1)};
+			if (c.i0 > c.i1 || c.j0 > c.j1 || c.k0 > c.k1) continue;
+			for (int m = 0; m <= D->cplx; m++)
+				for (int q = 0; q < 3; q++)
+					for (int p = 0; p < O.NK; p++) add(O.J[m][q][p], O.LO, c, W->obj[n].J[m][q][p]);
+		}
+	}
+	if (!total) return 0;
+	const size_t segB = (seg.size() * sizeof(GatherSeg) + 255) / 256 * 256, need = segB + total * sizeof(float);
+	if (need > D->gatherBytes) {
+		HIPCHECK(hipStreamSynchronize(D->st));
+		if (D->gatherDev) HIPCHECK(hipFree(D->gatherDev));
+		D->gatherDev = nullptr, D->gatherBytes = 0;
+		HIPCHECK(hipMalloc(&D->gatherDev, need));
+		D->gatherBytes = need;
+	}
+	if ((size_t)total * sizeof(float) > D->gatherPinnedBytes) {
+		if (D->gatherPinned) HIPCHECK(hipHostFree(D->gatherPinned));
+		D->gatherPinned = nullptr, D->gatherPinnedBytes = 0;
+		HIPCHECK(hipHostMalloc((void **)&D->gatherPinned, total * sizeof(float), hipHostMallocDefault));
+		D->gatherPinnedBytes = total * sizeof(float);
+	}
+	float *vals = (float *)((char *)D->gatherDev + segB);
+	HIPCHECK(hipMemcpyAsync(D->gatherDev, seg.data(), seg.size() * sizeof(GatherSeg), hipMemcpyHostToDevice, D->st));
+	k_gather<T><<<blocksFor(total), 256, 0, D->st>>>((const GatherSeg *)D->gatherDev, (int)seg.size(), total, vals);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(D->gatherPinned, vals, total * sizeof(float), hipMemcpyDeviceToHost, D->st));
+	return 0;
+}
+}  // extern "C++"
+
+int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
+{
+	// the worlds in order of first appearance, each with its boxes
+	std::vector<alis_dev_world *> worlds;
+	std::vector<std::vector<Box>> boxes;
+	for (int e = 0; e < n; e++) {
+		const int *b = box + 6 * e;
+		size_t w = 0;
+		while (w < worlds.size() && worlds[w] != D[e]) w++;
+		if (w == worlds.size()) worlds.push_back(D[e]), boxes.emplace_back();
+		boxes[w].push_back(Box{b[0], b[1], b[2], b[3], b[4], b[5]});
+	}
+	std::vector<std::vector<GatherHost>> out(worlds.size());
+	for (size_t w = 0; w < worlds.size(); w++) {
+		const int rc = worlds[w]->fp64 ? gatherBoxes<double>(worlds[w], boxes[w], out[w]) : gatherBoxes<float>(worlds[w], boxes[w], out[w]);
+		if (rc) return rc;
+	}
+	for (size_t w = 0; w < worlds.size(); w++) {
+		bool seen = false;
+		for (size_t v = 0; v < w; v++) seen |= worlds[v]->st == worlds[w]->st;
+		if (!seen) HIPCHECK(hipStreamSynchronize(worlds[w]->st));
+	}
+	for (size_t w = 0; w < worlds.size(); w++)
+		for (const GatherHost &g : out[w]) {
+			const float *v = worlds[w]->gatherPinned + g.off;
+			const int nk = g.b.k1 - g.b.k0 + 1;
+			for (int i = g.b.i0; i <= g.b.i1; i++)
+				for (int j = g.b.j0; j <= g.b.j1; j++, v += nk) memcpy(&g.host[i][j][g.b.k0], v, nk * sizeof(float));
+		}
+	return 0;
 }
 
 int alis_dev_read_field(alis_dev_world *D, int h, int m, int axis, double *out)

@@ -52,7 +52,7 @@ float get(world W, field F, float x, float y, float z)
 	if (W->dev && W->hostStale) {
 		int readsAux = F==SoEx || F==SoEy || F==SoEz || F==SoEE || F==ScEx || F==ScEy || F==ScEz || F==ScEE;
 		if (readsAux || W->isAux) syncWorld(W);
-		else alisSyncBox(W, MIN(i,-i)-2, MAX(i,-i)+2, MIN(j,-j)-2, MAX(j,-j)+2, MIN(k,-k)-2, MAX(k,-k)+2);
+		else alisProbe(W, MIN(i,-i)-2, MAX(i,-i)+2, MIN(j,-j)-2, MAX(j,-j)+2, MIN(k,-k)-2, MAX(k,-k)+2);
 	}
 	return F(W, i, j, k);
 }

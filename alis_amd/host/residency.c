@@ -89,8 +89,11 @@ void alisRegisterWorld(world W)
 	liveWorlds[nWorlds++] = W;
 }
 
+static void forgetProbes(world W);
+
 void alisForgetWorld(world W)
 {
+	forgetProbes(W);
 	if (W->dev) {
 		dropBatch();
 		alis_dev_release(W->dev);
@@ -217,6 +220,81 @@ void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 {
 	if (!W->dev || !W->hostStale) return;
 	alisFatal("box sync", alis_dev_sync(W->dev, i0, i1, j0, j1, k0, k1));
+}
+
+/*
+ * Probe boxes.  get() reads a few cells around one node per call, every step in a typical
+ * program, often at several nodes of several worlds.  Each box it reads is remembered per
+ * world with the half step it was last copied at; the first get() after a half step copies
+ * the boxes of every world probed so far in one engine call (one gather and one copy per
+ * world, one wait per stream), and later calls in the same half step read the host arrays.
+ */
+#define MAXPROBES 4096
+typedef struct {
+	world W;
+	int n;
+	int (*box)[6];
+	unsigned long *epoch;
+} probeSet;
+static probeSet *probeSets;
+static int nProbeSets, capProbeSets;
+static unsigned long epochNow = 1;
+
+void alisStepped(void) { epochNow++; }
+
+static void forgetProbes(world W)
+{
+	for (int a = 0; a < nProbeSets; a++)
+		if (probeSets[a].W == W) {
+			free(probeSets[a].box), free(probeSets[a].epoch);
+			probeSets[a] = probeSets[--nProbeSets];
+			return;
+		}
+}
+
+void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1)
+{
+	if (!W->dev || !W->hostStale) return;
+	probeSet *P = 0;
+	for (int a = 0; a < nProbeSets && !P; a++) if (probeSets[a].W == W) P = &probeSets[a];
+	if (!P) {
+		if (nProbeSets == capProbeSets) capProbeSets = capProbeSets ? 2 * capProbeSets : 8, probeSets = realloc(probeSets, capProbeSets * sizeof *probeSets);
+		P = &probeSets[nProbeSets++];
+		*P = (probeSet) {W, 0, malloc(MAXPROBES * sizeof *P->box), malloc(MAXPROBES * sizeof *P->epoch)};
+	}
+	const int b[6] = {i0, i1, j0, j1, k0, k1};
+	int at = -1;
+	for (int q = 0; q < P->n && at < 0; q++) if (!memcmp(P->box[q], b, sizeof b)) at = q;
+	if (at < 0) {
+		if (P->n == MAXPROBES) {              /* too many distinct probes: copy this one alone */
+			alisSyncBox(W, i0, i1, j0, j1, k0, k1);
+			return;
+		}
+		at = P->n++;
+		memcpy(P->box[at], b, sizeof b);
+		P->epoch[at] = 0;
+	}
+	if (P->epoch[at] == epochNow) return;
+	/* copy every stale box of every probed world that the device holds newer */
+	int m = 0;
+	for (int a = 0; a < nProbeSets; a++)
+		if (probeSets[a].W->dev && probeSets[a].W->hostStale) m += probeSets[a].n;
+	alis_dev_world **D = malloc(m * sizeof *D);
+	int *boxes = malloc(6 * m * sizeof(int));
+	m = 0;
+	for (int a = 0; a < nProbeSets; a++) {
+		probeSet *Q = &probeSets[a];
+		if (!Q->W->dev || !Q->W->hostStale) continue;
+		for (int q = 0; q < Q->n; q++) {
+			if (Q->epoch[q] == epochNow) continue;
+			D[m] = Q->W->dev;
+			memcpy(boxes + 6 * m, Q->box[q], sizeof b);
+			Q->epoch[q] = epochNow;
+			m++;
+		}
+	}
+	alisFatal("probe sync", alis_dev_sync_boxes(D, boxes, m));
+	free(D), free(boxes);
 }
 
 void alisSyncAll(void)

@@ -59,6 +59,7 @@ void updateE(world W)
 	alisFatal("E ghosts", alis_dev_ghosts(W->dev, 0));
 	W->t = W->E->t = ++W->n * W->dt;
 	W->hostStale = 1;
+	alisStepped();
 }
 
 /* H half step: curl of E with CPML and the ADE current update on the new E, TF/SF,
@@ -76,6 +77,7 @@ void updateH(world W)
 	alisFatal("H ghosts", alis_dev_ghosts(W->dev, 1));
 	W->t = W->H->t = W->E->t + 0.5 * W->dt;
 	W->hostStale = 1;
+	alisStepped();
 }
 
 
@@ -115,6 +117,7 @@ static void halfStepMany(world *Ws, int n, int h)
 		else W->t = W->H->t = W->E->t + 0.5 * W->dt;
 		W->hostStale = 1;
 	}
+	alisStepped();
 }
 
 void updateWorldsE(world *W, int n) { halfStepMany(W, n, 0); }

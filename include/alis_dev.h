@@ -120,6 +120,12 @@ int alis_dev_ghosts(alis_dev_world *D, int h);
  * ADE currents) when i0 > i1, else E/H inside the box and the currents of objects meeting it. */
 int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1);
 
+/* Probe copies: E and H (every part) inside box[6e..6e+5] = {i0,i1,j0,j1,k0,k1} of world
+ * D[e], and the currents of its dispersive objects inside that box, to the host arrays of
+ * the descriptor, for e = 0..n-1: one gather launch and one copy per world, one wait per
+ * stream.  Serves get() (src/output.c:50-53), which reads a few cells per call. */
+int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n);
+
 /* Read / overwrite one whole field array (h: 0 E, 1 H; m: real/imag; axis 0..2) as doubles
  * in the layout [iMIN-1..iMAX+1][jMIN-1..jMAX+1][kMIN-1..kMAX+1], k fastest. */
 int alis_dev_read_field(alis_dev_world *D, int h, int m, int axis, double *out);

@@ -417,6 +417,15 @@ int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const doub
 	return 0;
 }
 
+int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
+{
+	for (int e = 0; e < n; e++) {
+		const int *b = box + 6 * e;
+		if (alis_dev_sync(D[e], b[0], b[1], b[2], b[3], b[4], b[5])) return -1;
+	}
+	return 0;
+}
+
 int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
                                 const double *const *expE, const double *const *expH, int n)
 {
