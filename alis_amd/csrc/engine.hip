@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cmath>
 #include <algorithm>
+#include <map>
 #include <rocblas/rocblas.h>
 
 #include "alis_dev.h"
@@ -225,14 +226,18 @@ __device__ inline void boxCell(long t, int ni, int nj, int nk, int &di, int &dj,
 // The end of a half step in one launch: TF/SF strips and sources (additive operations,
 // src/update.c:161-179,194-272) followed by the ghost copies (src/update.c:182-191 with the
 // schedules at :284-297 and :311-339).  The operations form a program in the reference's
-// call order; one thread per (operation, cell) reproduces its sequential result:
-//  * a cell touched by additive operations on one array is handled by the first of them,
-//    which applies all of them in program order;
-//  * a cell written by ghost copies of one component is handled by the last of them, which
-//    walks its copy chain back to a cell no earlier copy writes and replays the chain
-//    forward from that cell's value.
-// The host splits a program into several launches where a chain's base cell is written by
-// the same launch (see buildPost), so every launch only reads cells it does not write.
+// call order.  The host compiles each launch's program into pieces, boxes of cells whose
+// sequential result is computed the same way (buildPieces), and the launch runs one thread
+// per piece cell:
+//  * an additive piece holds cells whose first additive operation on the array is the
+//    piece's; its list is every additive operation covering them, in program order, and
+//    the thread applies those switched on this half step;
+//  * a copy piece holds cells whose last ghost copy is the piece's; its list is the copy
+//    chain from that copy back to a cell that no earlier copy writes (the base cell, at a
+//    fixed offset), replayed forward from the base cell's value.
+// Cells that a later copy overwrites belong to no earlier piece.  The host splits a
+// program into several launches where a chain's base cell is written by the same launch
+// (see buildPost), so every launch only reads cells it does not write.
 enum { OP_COPY = 0, OP_STRIP = 1, OP_SOURCE = 2 };
 
 template<class T> struct PostOp {
@@ -246,27 +251,31 @@ template<class T> struct PostOp {
 	int di, dj, dk;
 	T c, s;                     // COPY: cos, sin; STRIP: dt/ds; SOURCE: scalar amplitude
 	int exX, exY, auxIMAX, auxJMAX, altX, altY;
-	long off;                   // first thread of this operation in the launch
 };
 
 constexpr int MAXSRC = 128;     // sources per chain (E or H) of one world
 constexpr int MAXCHAIN = 24;    // copies per launch
-constexpr int MAXPOST = 48;     // operations per launch (staged in LDS)
+constexpr int MAXPOST = 48;     // operations per launch
 constexpr int CURL_TPB = 2;     // tiles per workgroup of a batched curl launch (measured best of 1..16)
+
+struct PostPiece {
+	int kind;                   // OP_COPY, or OP_STRIP for an additive piece
+	int n;                      // operations in list
+	int si, sj, sk;             // copy piece: base cell - cell
+	Box b;                      // the cells
+	long off;                   // first thread in the launch (a multiple of 256)
+	unsigned char list[MAXPOST];// operation indices (copy piece: its chain, last copy first)
+};
 
 template<class T> struct PostArgs {
 	const PostOp<T> *ops;
-	int nops, nadd;             // additive operations first: [0, nadd), copies [nadd, nops)
+	const PostPiece *pieces;
+	int npieces;
 	long total;
 	unsigned strips;            // TF/SF boxes whose strips apply this half step (bit s)
 	unsigned src[MAXSRC / 32];  // sources active this half step
 	T wave[MAXSRC];             // their scalar wave values
 };
-
-__device__ inline bool inBox(const Box &b, int i, int j, int k)
-{
-	return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
-}
 
 template<class T> __device__ inline bool opOn(const PostArgs<T> &a, const PostOp<T> &o)
 {
@@ -291,94 +300,67 @@ template<class T> __device__ inline T addOne(const PostArgs<T> &a, const PostOp<
 	return f + amp * w;
 }
 
-// One workgroup (number blk within its program) of k_post; ops: LDS for the program.
+// One workgroup (number blk within its launch) of k_post.  Pieces start on workgroup
+// boundaries, so the piece and its operations are the same for the whole workgroup.
 template<class T>
-__device__ __forceinline__ void postBlock(const PostArgs<T> &a, PostOp<T> *ops, long blk)
+__device__ __forceinline__ void postBlock(const PostArgs<T> &a, long blk)
 {
-	// the launch's operations in LDS; every operation starts on a workgroup boundary, so a
-	// workgroup belongs to exactly one operation
-	{
-		const int words = a.nops * (int)sizeof(PostOp<T>) / 4;
-		const int *src = (const int *)a.ops;
-		int *dst = (int *)ops;
-		for (int w = (int)threadIdx.x; w < words; w += (int)blockDim.x) dst[w] = src[w];
+	const long first = blk * 256;
+	int lo = 0, hi = a.npieces - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) / 2;
+		if (a.pieces[mid].off <= first) lo = mid;
+		else hi = mid - 1;
 	}
-	__syncthreads();
-	const long t = blk * blockDim.x + threadIdx.x;
-	int p = 0;
-	while (p + 1 < a.nops && blk * blockDim.x >= ops[p + 1].off) p++;
-	const PostOp<T> &o = ops[p];
-	if (!opOn(a, o)) return;
-	const int ni = o.b.i1 - o.b.i0 + 1, nj = o.b.j1 - o.b.j0 + 1, nk = o.b.k1 - o.b.k0 + 1;
-	if (t - o.off >= (long)ni * nj * nk) return;
+	const PostPiece &P = a.pieces[lo];
+	const int ni = P.b.i1 - P.b.i0 + 1, nj = P.b.j1 - P.b.j0 + 1, nk = P.b.k1 - P.b.k0 + 1;
+	const long t = first + threadIdx.x - P.off;
+	if (t >= (long)ni * nj * nk) return;
 	int di, dj, dk;
-	boxCell(t - o.off, ni, nj, nk, di, dj, dk);
-	const int i = o.b.i0 + di, j = o.b.j0 + dj, k = o.b.k0 + dk;
-	if (o.kind != OP_COPY) {
-		for (int q = 0; q < p; q++) {
-			const PostOp<T> &r = ops[q];
-			if (r.F == o.F && opOn(a, r) && inBox(r.b, i, j, k)) return;
-		}
-		for (int q = a.nadd; q < a.nops; q++) {
-			const PostOp<T> &r = ops[q];
-			if ((r.F == o.F || r.G == o.F) && inBox(r.b, i, j, k)) return;
-		}
-		const long d = o.LF.at(i, j, k);
-		T f = o.F[d];
-		for (int q = p; q < a.nadd; q++) {
-			const PostOp<T> &r = ops[q];
-			if (r.F == o.F && opOn(a, r) && inBox(r.b, i, j, k)) f = addOne(a, r, f, i, j, k);
+	boxCell(t, ni, nj, nk, di, dj, dk);
+	const int i = P.b.i0 + di, j = P.b.j0 + dj, k = P.b.k0 + dk;
+	const PostOp<T> &o = a.ops[P.list[0]];
+	const long d = o.LF.at(i, j, k);
+	if (P.kind == OP_COPY) {
+		const long s = o.LF.at(i + P.si, j + P.sj, k + P.sk);
+		T f = o.F[s], g = o.G ? o.G[s] : T(0);
+		for (int u = P.n - 1; u >= 0; u--) {
+			const PostOp<T> &r = a.ops[P.list[u]];
+			if (o.G) {
+				const T fs = f, gs = g;
+				f = r.c * fs + r.s * gs;
+				g = r.c * gs - r.s * fs;
+			}
+			else f = r.c * f;
 		}
 		o.F[d] = f;
+		if (o.G) o.G[d] = g;
 		return;
 	}
-	for (int q = p + 1; q < a.nops; q++) {
-		const PostOp<T> &r = ops[q];
-		if (r.F == o.F && inBox(r.b, i, j, k)) return;
+	bool on = false;
+	T f = T(0);
+	for (int u = 0; u < P.n; u++) {
+		const PostOp<T> &r = a.ops[P.list[u]];
+		if (!opOn(a, r)) continue;
+		if (!on) f = o.F[d], on = true;
+		f = addOne(a, r, f, i, j, k);
 	}
-	int chain[MAXCHAIN], n = 0, q = p;
-	int ci = i, cj = j, ck = k;
-	for (;;) {
-		chain[n++] = q;
-		const PostOp<T> &r = ops[q];
-		ci += r.di, cj += r.dj, ck += r.dk;
-		int prev = -1;
-		for (int u = q - 1; u >= a.nadd; u--)
-			if (ops[u].F == o.F && inBox(ops[u].b, ci, cj, ck)) { prev = u; break; }
-		if (prev < 0 || n == MAXCHAIN) break;
-		q = prev;
-	}
-	const long sidx = o.LF.at(ci, cj, ck);
-	T f = o.F[sidx], g = o.G ? o.G[sidx] : T(0);
-	for (int u = n - 1; u >= 0; u--) {
-		const PostOp<T> &r = ops[chain[u]];
-		if (o.G) {
-			const T fs = f, gs = g;
-			f = r.c * fs + r.s * gs;
-			g = r.c * gs - r.s * fs;
-		}
-		else f = r.c * f;
-	}
-	const long d = o.LF.at(i, j, k);
-	o.F[d] = f;
-	if (o.G) o.G[d] = g;
+	if (on) o.F[d] = f;
 }
 
 template<class T>
 __global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
 {
-	__shared__ PostOp<T> ops[MAXPOST];
-	postBlock(a, ops, (long)blockIdx.x);
+	postBlock(a, (long)blockIdx.x);
 }
 
-// The programs of several worlds in one launch: entry e owns workgroups [start[e], start[e+1]).
+// Several worlds' programs in one launch: entry e owns workgroups [start[e], start[e+1]).
 template<class T>
 __global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
 {
-	__shared__ PostOp<T> ops[MAXPOST];
 	const int b = (int)blockIdx.x;
 	const int e = batchEntry(start, n, b);
-	postBlock(args[e], ops, (long)(b - start[e]));
+	postBlock(args[e], (long)(b - start[e]));
 }
 
 // Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
@@ -392,7 +374,12 @@ struct PhsFaceGeom {
 	long off;                   // first cell of this face in the concatenated cell list
 };
 
-constexpr int PHS_QPT = 2;      // frequencies per thread
+constexpr int PHS_QPT = 4;      // frequencies per thread (measured: 1 < 2 < 4)
+
+// A complex accumulator (re, im) as one vector for 8- or 16-byte accesses.
+template<class TA> struct Pair;
+template<> struct Pair<float> { typedef float V __attribute__((vector_size(8))); };
+template<> struct Pair<double> { typedef double V __attribute__((vector_size(16))); };
 
 template<class T> struct PhsArgs {
 	const T *E[3], *H[3];
@@ -434,23 +421,33 @@ __device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
 		e1 = a.E[0][id], e2 = a.E[1][id];
 		h1 = a.H[0][id] + a.H[0][id + 1], h2 = a.H[1][id] + a.H[1][id + 1];
 	}
-	TA *acc = (TA *)a.acc;
+	// one (re, im) pair per vector access; accumulators are streamed once per sample, so
+	// non-temporally.  All loads are issued before the first store: a store between them
+	// would make the next loads wait for it.
+	typedef typename Pair<TA>::V V;
+	V *acc = (V *)a.acc;
 	const TA half = TA(0.5);
-	const int qb = min(a.nq, qa + PHS_QPT);
-	for (int q = qa; q < qb; q++) {
-		const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
-		const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
-		const long base = (long)(a.q0 + q) * 4 * a.ncell;
-		TA *p0 = acc + 2 * (base + t);
-		TA *p1 = acc + 2 * (base + a.ncell + t);
-		TA *p2 = acc + 2 * (base + 2 * a.ncell + t);
-		TA *p3 = acc + 2 * (base + 3 * a.ncell + t);
-		// accumulators are streamed once per sample: non-temporal
-		stS<true>(p0, ldS<true>(p0) + er * TA(e1)), stS<true>(p0 + 1, ldS<true>(p0 + 1) + ei * TA(e1));
-		stS<true>(p1, ldS<true>(p1) + er * TA(e2)), stS<true>(p1 + 1, ldS<true>(p1 + 1) + ei * TA(e2));
-		stS<true>(p2, ldS<true>(p2) + hr * TA(h1) * half), stS<true>(p2 + 1, ldS<true>(p2 + 1) + hi * TA(h1) * half);
-		stS<true>(p3, ldS<true>(p3) + hr * TA(h2) * half), stS<true>(p3 + 1, ldS<true>(p3 + 1) + hi * TA(h2) * half);
-	}
+	const int nq = min(PHS_QPT, a.nq - qa);
+	V x[PHS_QPT][4];
+#pragma unroll
+	for (int u = 0; u < PHS_QPT; u++)
+		if (u < nq) {
+			const V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
+			for (int c = 0; c < 4; c++) x[u][c] = __builtin_nontemporal_load(p + c * a.ncell);
+		}
+#pragma unroll
+	for (int u = 0; u < PHS_QPT; u++)
+		if (u < nq) {
+			const int q = qa + u;
+			const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
+			const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
+			x[u][0][0] = x[u][0][0] + er * TA(e1), x[u][0][1] = x[u][0][1] + ei * TA(e1);
+			x[u][1][0] = x[u][1][0] + er * TA(e2), x[u][1][1] = x[u][1][1] + ei * TA(e2);
+			x[u][2][0] = x[u][2][0] + hr * TA(h1) * half, x[u][2][1] = x[u][2][1] + hi * TA(h1) * half;
+			x[u][3][0] = x[u][3][0] + hr * TA(h2) * half, x[u][3][1] = x[u][3][1] + hi * TA(h2) * half;
+			V *p = acc + (long)(a.q0 + q) * 4 * a.ncell + t;
+			for (int c = 0; c < 4; c++) __builtin_nontemporal_store(x[u][c], p + c * a.ncell);
+		}
 }
 
 template<class T, class TA>
@@ -580,14 +577,17 @@ struct DevSrc {
 };
 
 struct PostSeg {
-	void *ops = nullptr;        // PostOp<T>[nops] in device memory
-	int nops = 0, nadd = 0;
-	long total = 0;
+	void *ops = nullptr;        // PostOp<T>[] in device memory
+	void *pieces = nullptr;     // PostPiece[npieces] in device memory
+	int npieces = 0;
+	long total = 0;             // threads (a multiple of 256)
 };
 
 struct DevTfsf {
 	alis_dev_tfsf S{};
 	alis_dev_world *aux = nullptr;
+	hipEvent_t auxDone = nullptr;   // recorded on the auxiliary world's stream before strips
+	hipEvent_t mainDone = nullptr;  // recorded on the main world's stream after them
 };
 
 struct DevPhaser {
@@ -858,6 +858,100 @@ bool postConflict(const std::vector<PostOp<T>> &A, const std::vector<PostOp<T>> 
 				}
 	}
 	return false;
+}
+
+// The work of cell (i,j,k) of operation p in a launch's program v (additive operations
+// first, nadd of them), as the key (kind, base offset, operation list); false when another
+// piece computes the cell.  Follows the sequential semantics described at k_post.
+template<class T>
+bool cellWork(const std::vector<PostOp<T>> &v, int nadd, int p, int i, int j, int k, std::vector<int> &key)
+{
+	const PostOp<T> &o = v[p];
+	const int n = (int)v.size();
+	key.assign(4, 0);
+	if (p < nadd) {
+		for (int r = 0; r < p; r++) if (v[r].F == o.F && inBoxH(v[r].b, i, j, k)) return false;
+		for (int q = nadd; q < n; q++) if ((v[q].F == o.F || v[q].G == o.F) && inBoxH(v[q].b, i, j, k)) return false;
+		key[0] = OP_STRIP;
+		for (int q = p; q < nadd; q++) if (v[q].F == o.F && inBoxH(v[q].b, i, j, k)) key.push_back(q);
+		return true;
+	}
+	for (int q = p + 1; q < n; q++) if (v[q].F == o.F && inBoxH(v[q].b, i, j, k)) return false;
+	int ci = i, cj = j, ck = k, q = p;
+	for (;;) {
+		key.push_back(q);
+		ci += v[q].di, cj += v[q].dj, ck += v[q].dk;
+		int prev = -1;
+		for (int u = q - 1; u >= nadd; u--) if (v[u].F == o.F && inBoxH(v[u].b, ci, cj, ck)) { prev = u; break; }
+		if (prev < 0) break;
+		q = prev;
+	}
+	key[0] = OP_COPY, key[1] = ci - i, key[2] = cj - j, key[3] = ck - k;
+	return true;
+}
+
+// Compile a launch's program into pieces: per operation, the cells of its box labelled by
+// their work, then grown greedily into boxes of equal work (along k, then j, then i).
+template<class T>
+void buildPieces(const std::vector<PostOp<T>> &v, int nadd, std::vector<PostPiece> &out, long &total)
+{
+	std::map<std::vector<int>, int> ids;
+	std::vector<std::vector<int>> keys;
+	std::vector<int> key, last;
+	int lastId = -1;
+	total = 0;
+	for (int p = 0; p < (int)v.size(); p++) {
+		const Box &b = v[p].b;
+		const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+		if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+		std::vector<int> sig((size_t)ni * nj * nk);
+		size_t c = 0;
+		for (int di = 0; di < ni; di++)
+			for (int dj = 0; dj < nj; dj++)
+				for (int dk = 0; dk < nk; dk++, c++) {
+					if (!cellWork(v, nadd, p, b.i0 + di, b.j0 + dj, b.k0 + dk, key)) { sig[c] = -1; continue; }
+					if (lastId < 0 || key != last) {
+						auto it = ids.find(key);
+						if (it == ids.end()) it = ids.emplace(key, (int)keys.size()).first, keys.push_back(key);
+						lastId = it->second, last = key;
+					}
+					sig[c] = lastId;
+				}
+		auto at = [&](int di, int dj, int dk) -> int & { return sig[((size_t)di * nj + dj) * nk + dk]; };
+		for (int di = 0; di < ni; di++)
+			for (int dj = 0; dj < nj; dj++)
+				for (int dk = 0; dk < nk; dk++) {
+					const int id = at(di, dj, dk);
+					if (id < 0) continue;
+					int dk1 = dk, dj1 = dj, di1 = di;
+					while (dk1 + 1 < nk && at(di, dj, dk1 + 1) == id) dk1++;
+					auto rowSame = [&](int ii, int jj) {
+						for (int kk = dk; kk <= dk1; kk++) if (at(ii, jj, kk) != id) return false;
+						return true;
+					};
+					while (dj1 + 1 < nj && rowSame(di, dj1 + 1)) dj1++;
+					for (;;) {
+						if (di1 + 1 >= ni) break;
+						bool same = true;
+						for (int jj = dj; jj <= dj1 && same; jj++) same = rowSame(di1 + 1, jj);
+						if (!same) break;
+						di1++;
+					}
+					for (int ii = di; ii <= di1; ii++)
+						for (int jj = dj; jj <= dj1; jj++)
+							for (int kk = dk; kk <= dk1; kk++) at(ii, jj, kk) = -1;
+					const std::vector<int> &K = keys[id];
+					PostPiece P{};
+					P.kind = K[0];
+					P.si = K[1], P.sj = K[2], P.sk = K[3];
+					P.n = (int)K.size() - 4;
+					for (int u = 0; u < P.n; u++) P.list[u] = (unsigned char)K[4 + u];
+					P.b = Box{b.i0 + di, b.i0 + di1, b.j0 + dj, b.j0 + dj1, b.k0 + dk, b.k0 + dk1};
+					P.off = total;
+					total += (P.b.count() + 255) / 256 * 256;
+					out.push_back(P);
+				}
+	}
 }
 
 // Commit: allocate and upload everything of the descriptor.
@@ -1249,13 +1343,17 @@ int buildPost(alis_dev_world *D)
 	for (int h = 0; h < 2; h++) {
 		D->post[h].clear();
 		for (size_t g = 0; g < segOps[h].size(); g++) {
-			std::vector<PostOp<T>> &v = segOps[h][g];
-			long off = 0;
-			for (PostOp<T> &o : v) o.off = off, off += (o.b.count() + 255) / 256 * 256;
+			const std::vector<PostOp<T>> &v = segOps[h][g];
+			std::vector<PostPiece> pieces;
 			PostSeg seg;
-			seg.nops = (int)v.size(), seg.nadd = segAdd[h][g], seg.total = off;
+			buildPieces(v, segAdd[h][g], pieces, seg.total);
+			if (pieces.empty()) continue;
+			seg.npieces = (int)pieces.size();
 			if (devAlloc(D, &seg.ops, v.size() * sizeof(PostOp<T>))) return -1;
+			if (devAlloc(D, &seg.pieces, pieces.size() * sizeof(PostPiece))) return -1;
 			HIPCHECK(hipMemcpyAsync(seg.ops, v.data(), v.size() * sizeof(PostOp<T>), hipMemcpyHostToDevice, D->st));
+			HIPCHECK(hipMemcpyAsync(seg.pieces, pieces.data(), pieces.size() * sizeof(PostPiece), hipMemcpyHostToDevice, D->st));
+			HIPCHECK(hipStreamSynchronize(D->st));   // the host vectors end with this iteration
 			D->post[h].push_back(seg);
 		}
 	}
@@ -1273,12 +1371,30 @@ int postHalf(alis_dev_world *D, int h)
 	a.strips = D->pendStrips[h];
 	for (size_t w = 0; w < D->pendSrc[h].size(); w++) a.src[w] = D->pendSrc[h][w];
 	for (size_t q = 0; q < D->pendWave[h].size(); q++) a.wave[q] = (T)D->pendWave[h][q];
+	// Strips read the auxiliary fields of the other kind, which the auxiliary world's last
+	// half step wrote (its stream runs ahead of this one), and its next half step must not
+	// overwrite them before the strips ran.
+	for (size_t s = 0; s < D->tf.size(); s++) {
+		DevTfsf &X = D->tf[s];
+		if (!((a.strips >> s) & 1u) || X.aux->st == D->st) continue;
+		if (!X.auxDone) HIPCHECK(hipEventCreateWithFlags(&X.auxDone, hipEventDisableTiming));
+		HIPCHECK(hipEventRecord(X.auxDone, X.aux->st));
+		HIPCHECK(hipStreamWaitEvent(D->st, X.auxDone, 0));
+	}
 	for (const PostSeg &g : D->post[h]) {
 		if (!g.total) continue;
 		a.ops = (const PostOp<T> *)g.ops;
-		a.nops = g.nops, a.nadd = g.nadd, a.total = g.total;
+		a.pieces = (const PostPiece *)g.pieces;
+		a.npieces = g.npieces, a.total = g.total;
 		k_post<T><<<blocksFor(g.total), 256, 0, D->st>>>(a);
 		HIPCHECK(hipGetLastError());
+	}
+	for (size_t s = 0; s < D->tf.size(); s++) {
+		DevTfsf &X = D->tf[s];
+		if (!((a.strips >> s) & 1u) || X.aux->st == D->st) continue;
+		if (!X.mainDone) HIPCHECK(hipEventCreateWithFlags(&X.mainDone, hipEventDisableTiming));
+		HIPCHECK(hipEventRecord(X.mainDone, D->st));
+		HIPCHECK(hipStreamWaitEvent(X.aux->st, X.mainDone, 0));
 	}
 	D->pendStrips[h] = 0;
 	std::fill(D->pendSrc[h].begin(), D->pendSrc[h].end(), 0u);
@@ -1602,7 +1718,8 @@ int batchPost(alis_dev_batch *B, int h)
 			const PostSeg &g = D->post[h][R.seg[e]];
 			memset(&a[e], 0, sizeof(PostArgs<T>));
 			a[e].ops = (const PostOp<T> *)g.ops;
-			a[e].nops = g.nops, a[e].nadd = g.nadd, a[e].total = g.total;
+			a[e].pieces = (const PostPiece *)g.pieces;
+			a[e].npieces = g.npieces, a[e].total = g.total;
 			a[e].strips = D->pendStrips[h];
 			for (size_t w = 0; w < D->pendSrc[h].size(); w++) a[e].src[w] = D->pendSrc[h][w];
 			for (size_t q = 0; q < D->pendWave[h].size(); q++) a[e].wave[q] = (T)D->pendWave[h][q];
@@ -1677,9 +1794,9 @@ int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
 	if (aux->fp64 != D->fp64) return fail("TF/SF auxiliary world precision differs");
 	D->tf[s].aux = aux;
 	D->postBuilt = false;
-	// The strips read the auxiliary fields between its half steps, so both worlds run on
-	// one stream: the auxiliary world gives up its own and joins the main world's.
-	return joinStream(aux, D);
+	// The auxiliary world keeps its own stream, so its half steps run beside the main
+	// world's curl launches; postHalf orders its strips against them with events.
+	return 0;
 }
 
 void alis_dev_release(alis_dev_world *D)
@@ -1693,6 +1810,10 @@ void alis_dev_release(alis_dev_world *D)
 			if (s.waveDone) hipEventDestroy(s.waveDone);
 		}
 	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
+	for (DevTfsf &X : D->tf) {
+		if (X.auxDone) hipEventDestroy(X.auxDone);
+		if (X.mainDone) hipEventDestroy(X.mainDone);
+	}
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
