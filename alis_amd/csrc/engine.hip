@@ -14,6 +14,7 @@
 // bitwise those of the reference's float code.  FP64 worlds run the same programs on
 // doubles (coefficients widened from the float setup).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <vector>
 #include <string>
 #include <cstring>
@@ -66,9 +67,15 @@ struct Box {
 // Kernels
 // ---------------------------------------------------------------------------------------
 
+// Four values (CPML coefficients) as one vector for a single load.
+template<class T> struct Quad;
+template<> struct Quad<float> { typedef float V __attribute__((vector_size(16))); };
+template<> struct Quad<double> { typedef double V __attribute__((vector_size(32))); };
+
 template<class T> struct CurlFace {
 	T *pa, *pb;                 // the two auxiliary components of this face
-	const T *b, *ca, *cb;       // recursion coefficients along the face axis (index - MIN)
+	const T *bc;                // recursion coefficients (b, ca, cb, 0) along the face axis
+	                            // (index - MIN), four to an index for one vector load
 	Lay L;
 	int lo, hi, on;
 };
@@ -135,10 +142,14 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 	curlAt<T, HASC, NT, NTG>(a, i, j, k);
 }
 
-// The update of cell (i, j, k).
+// The update of cell (i, j, k) in batched launches: the CPML faces reuse the curl's
+// neighbour values, and the psi slabs go through the cache (the short rows of small worlds
+// cover a slab with partial lines that neighbouring tiles complete).  Measured on 64 k-point
+// worlds: 5 % (E) and 14 % (H) faster than reloading and streaming; slower in k_curl.
 template<class T, bool HASC, bool NT, bool NTG>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
+	constexpr bool SHARE = true, NTP = false;
 #include "curl_cell.inc"
 }
 
@@ -155,6 +166,7 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
+	constexpr bool SHARE = false, NTP = NT;
 #include "curl_cell.inc"
 }
 
@@ -586,8 +598,6 @@ struct PostSeg {
 struct DevTfsf {
 	alis_dev_tfsf S{};
 	alis_dev_world *aux = nullptr;
-	hipEvent_t auxDone = nullptr;   // recorded on the auxiliary world's stream before strips
-	hipEvent_t mainDone = nullptr;  // recorded on the main world's stream after them
 };
 
 struct DevPhaser {
@@ -616,6 +626,7 @@ struct alis_dev_world {
 	void *C[3] = {};            // inverse permittivity of the E grid, or null
 	void *dt[2][3] = {};        // [h][axis] dt/d arrays (T), index - MIN
 	void *cp[2][3][3] = {};     // [h][axis][b, ca, cb] CPML arrays or null
+	void *cpk[2][3] = {};       // [h][axis] the same packed (b, ca, cb, 0) per index
 	DevSlab slab[2][6];
 	std::vector<DevObj> obj;
 	void *objTab = nullptr;     // ObjDev<T>[obj.size()] in device memory
@@ -646,6 +657,7 @@ struct alis_dev_world {
 	std::vector<void *> allocs;
 	long bytes = 0;
 	bool timing = false;
+	hipEvent_t halfDone = nullptr;  // the end of the last postHalf (bound to its last launch)
 	std::vector<hipEvent_t> evPairs[2];
 	double evMs[2] = {0, 0};
 	long evCount[2] = {0, 0};
@@ -986,7 +998,13 @@ int commitWorld(alis_dev_world *D)
 		for (int a = 0; a < 3; a++) {
 			if (upload1<T>(D, &D->dt[h][a], W->dt[h][a], NUMs[a])) return -1;
 			if (W->cpml[h][a][0])
+			{
 				for (int c = 0; c < 3; c++) if (upload1<T>(D, &D->cp[h][a][c], W->cpml[h][a][c], NUMs[a])) return -1;
+				std::vector<float> packed(4 * (size_t)NUMs[a], 0.f);
+				for (int q = 0; q < NUMs[a]; q++)
+					for (int c = 0; c < 3; c++) packed[4 * q + c] = W->cpml[h][a][c][q];
+				if (upload1<T>(D, &D->cpk[h][a], packed.data(), 4 * NUMs[a])) return -1;
+			}
 		}
 
 	// CPML slabs: ranges per face for E (Alt=+1, Offset 0) and H (Alt=-1, Offset 1)
@@ -1140,7 +1158,7 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 		if (!S.on) continue;
 		const int ax = f % 3;
 		P.pa = (T *)S.p[m][0], P.pb = (T *)S.p[m][1];
-		P.b = (const T *)D->cp[h][ax][0], P.ca = (const T *)D->cp[h][ax][1], P.cb = (const T *)D->cp[h][ax][2];
+		P.bc = (const T *)D->cpk[h][ax];
 		P.L = S.L, P.lo = S.lo, P.hi = S.hi;
 	}
 	a.obj = (const ObjDev<T> *)D->objTab;
@@ -1158,14 +1176,15 @@ int coreHalf(alis_dev_world *D, int h)
 		HIPCHECK(hipEventCreate(&e1));
 	}
 	dim3 grid((D->kNUM + 63) / 64, (D->jNUM + 3) / 4, D->iNUM), block(64, 4);
-	if (e0) HIPCHECK(hipEventRecord(e0, D->st));
+	// timing events are bound to the launches themselves (their start and end), so timing
+	// adds no marker between launches
 	for (int m = 0; m <= D->cplx; m++) {
 		CurlArgs<T> a = curlArgs<T>(D, h, m);
-		if (h == 0 && D->C[0]) k_curl<T, true><<<grid, block, 0, D->st>>>(a);
-		else k_curl<T, false><<<grid, block, 0, D->st>>>(a);
+		hipEvent_t s0 = m == 0 ? e0 : nullptr, s1 = m == D->cplx ? e1 : nullptr;
+		if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, true>, grid, block, 0, D->st, s0, s1, 0, a);
+		else hipExtLaunchKernelGGL(k_curl<T, false>, grid, block, 0, D->st, s0, s1, 0, a);
 	}
 	if (e1) {
-		HIPCHECK(hipEventRecord(e1, D->st));
 		D->evPairs[h].push_back(e0);
 		D->evPairs[h].push_back(e1);
 	}
@@ -1371,31 +1390,36 @@ int postHalf(alis_dev_world *D, int h)
 	a.strips = D->pendStrips[h];
 	for (size_t w = 0; w < D->pendSrc[h].size(); w++) a.src[w] = D->pendSrc[h][w];
 	for (size_t q = 0; q < D->pendWave[h].size(); q++) a.wave[q] = (T)D->pendWave[h][q];
-	// Strips read the auxiliary fields of the other kind, which the auxiliary world's last
-	// half step wrote (its stream runs ahead of this one), and its next half step must not
-	// overwrite them before the strips ran.
+	// Strips read the auxiliary fields of the other kind, which the auxiliary world's half
+	// steps write on its own stream: they wait for its last half step, and its next one
+	// waits for them.  Both events are bound to the worlds' last post launches, so the
+	// ordering adds no marker to either stream.
+	auto sideStream = [&](size_t s) { return ((a.strips >> s) & 1u) && D->tf[s].aux->st != D->st; };
 	for (size_t s = 0; s < D->tf.size(); s++) {
-		DevTfsf &X = D->tf[s];
-		if (!((a.strips >> s) & 1u) || X.aux->st == D->st) continue;
-		if (!X.auxDone) HIPCHECK(hipEventCreateWithFlags(&X.auxDone, hipEventDisableTiming));
-		HIPCHECK(hipEventRecord(X.auxDone, X.aux->st));
-		HIPCHECK(hipStreamWaitEvent(D->st, X.auxDone, 0));
+		if (!sideStream(s)) continue;
+		alis_dev_world *X = D->tf[s].aux;
+		if (!X->halfDone) {
+			HIPCHECK(hipEventCreateWithFlags(&X->halfDone, hipEventDisableTiming));
+			HIPCHECK(hipEventRecord(X->halfDone, X->st));
+		}
+		HIPCHECK(hipStreamWaitEvent(D->st, X->halfDone, 0));
 	}
-	for (const PostSeg &g : D->post[h]) {
-		if (!g.total) continue;
-		a.ops = (const PostOp<T> *)g.ops;
-		a.pieces = (const PostPiece *)g.pieces;
-		a.npieces = g.npieces, a.total = g.total;
-		k_post<T><<<blocksFor(g.total), 256, 0, D->st>>>(a);
+	if (!D->halfDone) HIPCHECK(hipEventCreateWithFlags(&D->halfDone, hipEventDisableTiming));
+	int last = -1;
+	for (int g = 0; g < (int)D->post[h].size(); g++) if (D->post[h][g].total) last = g;
+	for (int g = 0; g <= last; g++) {
+		const PostSeg &G = D->post[h][g];
+		if (!G.total) continue;
+		a.ops = (const PostOp<T> *)G.ops;
+		a.pieces = (const PostPiece *)G.pieces;
+		a.npieces = G.npieces, a.total = G.total;
+		if (g == last) hipExtLaunchKernelGGL(k_post<T>, dim3(blocksFor(G.total)), dim3(256), 0, D->st, nullptr, D->halfDone, 0, a);
+		else k_post<T><<<blocksFor(G.total), 256, 0, D->st>>>(a);
 		HIPCHECK(hipGetLastError());
 	}
-	for (size_t s = 0; s < D->tf.size(); s++) {
-		DevTfsf &X = D->tf[s];
-		if (!((a.strips >> s) & 1u) || X.aux->st == D->st) continue;
-		if (!X.mainDone) HIPCHECK(hipEventCreateWithFlags(&X.mainDone, hipEventDisableTiming));
-		HIPCHECK(hipEventRecord(X.mainDone, D->st));
-		HIPCHECK(hipStreamWaitEvent(X.aux->st, X.mainDone, 0));
-	}
+	if (last < 0) HIPCHECK(hipEventRecord(D->halfDone, D->st));
+	for (size_t s = 0; s < D->tf.size(); s++)
+		if (sideStream(s)) HIPCHECK(hipStreamWaitEvent(D->tf[s].aux->st, D->halfDone, 0));
 	D->pendStrips[h] = 0;
 	std::fill(D->pendSrc[h].begin(), D->pendSrc[h].end(), 0u);
 	return 0;
@@ -1693,12 +1717,11 @@ int batchCore(alis_dev_batch *B, int h)
 	if (D0->timing) {
 		HIPCHECK(hipEventCreate(&e0));
 		HIPCHECK(hipEventCreate(&e1));
-		HIPCHECK(hipEventRecord(e0, B->st));
 	}
-	k_curl_batch<T, CURL_TPB><<<B->curlBlocks[h], dim3(64, 4), 0, B->st>>>((const CurlArgs<T> *)B->curl[h], (const int *)B->curlStart[h], B->curlN[h]);
+	hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB>, dim3(B->curlBlocks[h]), dim3(64, 4), 0, B->st, e0, e1, 0,
+	                      (const CurlArgs<T> *)B->curl[h], (const int *)B->curlStart[h], B->curlN[h]);
 	HIPCHECK(hipGetLastError());
 	if (e1) {
-		HIPCHECK(hipEventRecord(e1, B->st));
 		D0->evPairs[h].push_back(e0);
 		D0->evPairs[h].push_back(e1);
 	}
@@ -1810,10 +1833,7 @@ void alis_dev_release(alis_dev_world *D)
 			if (s.waveDone) hipEventDestroy(s.waveDone);
 		}
 	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
-	for (DevTfsf &X : D->tf) {
-		if (X.auxDone) hipEventDestroy(X.auxDone);
-		if (X.mainDone) hipEventDestroy(X.mainDone);
-	}
+	if (D->halfDone) hipEventDestroy(D->halfDone);
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
