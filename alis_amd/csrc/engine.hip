@@ -492,6 +492,14 @@ __global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restri
 	phaserCell<T, TA>(a, t, qa);
 }
 
+// Per-call launch arguments of batched launches from pinned host memory into device memory,
+// in stream order: a copy-engine transfer there would add a round trip between the
+// launches around it (about 25 us measured between the batched curl and post launches).
+__global__ __launch_bounds__(256) void k_fetch(unsigned *__restrict__ dst, const unsigned *__restrict__ src, int words)
+{
+	for (int w = (int)(blockIdx.x * blockDim.x + threadIdx.x); w < words; w += (int)(gridDim.x * blockDim.x)) dst[w] = src[w];
+}
+
 // Several boxes of several arrays into one float buffer: segment s covers dst[off[s] ..
 // off[s+1]) with the cells of its box, k fastest.
 struct GatherSeg {
@@ -1554,6 +1562,13 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 	return 0;
 }
 
+// Copy launch arguments staged in pinned host memory to the device in stream order (k_fetch).
+void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st)
+{
+	const int words = (int)(bytes / 4), blocks = std::max(1, std::min(64, (words + 255) / 256));
+	k_fetch<<<blocks, 256, 0, st>>>((unsigned *)dev, (const unsigned *)pinned, words);
+}
+
 // One sample into several phasers in one launch (entries of one field and accumulator type).
 // Arguments and workgroup starts go through L's pinned staging buffer; before it is
 // rewritten, the copy made from it by the previous call must have been made.
@@ -1584,7 +1599,7 @@ int phaserBatchUpdate(alis_dev_world *L, const std::vector<PhsArgs<T>> &args)
 	else HIPCHECK(hipEventSynchronize(L->phsDone));
 	memcpy(L->phsPinned, args.data(), args.size() * sizeof(PhsArgs<T>));
 	memcpy((char *)L->phsPinned + ab, start.data(), start.size() * sizeof(int));
-	HIPCHECK(hipMemcpyAsync(L->phsDev, L->phsPinned, bytes, hipMemcpyHostToDevice, L->st));
+	fetchArgs(L->phsDev, L->phsPinned, bytes, L->st);
 	HIPCHECK(hipEventRecord(L->phsDone, L->st));
 	dim3 grid(blocks, ny);
 	k_phaser_batch<T, TA><<<grid, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsDev, (const int *)((char *)L->phsDev + ab), (int)args.size());
@@ -1775,7 +1790,7 @@ int batchPost(alis_dev_batch *B, int h)
 			for (size_t w = 0; w < D->pendSrc[h].size(); w++) a[e].src[w] = D->pendSrc[h][w];
 			for (size_t q = 0; q < D->pendWave[h].size(); q++) a[e].wave[q] = (T)D->pendWave[h][q];
 		}
-		HIPCHECK(hipMemcpyAsync(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), hipMemcpyHostToDevice, B->st));
+		fetchArgs(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), B->st);
 		HIPCHECK(hipEventRecord(R.done, B->st));
 		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
 		HIPCHECK(hipGetLastError());
