@@ -97,6 +97,8 @@ template<class T> struct CurlArgs {
 	T *F0, *F1, *F2;
 	const T *G0, *G1, *G2;
 	const T *C0, *C1, *C2;
+	const typename Quad<T>::V *seg;  // k_curl E: per 64-cell row segment (tile), 1/eps
+	                            // (x, y, z, 1) when the segment holds one triple, else 0s
 	const T *dtx, *dty, *dtz;   // dt/d along each axis, index - MIN
 	Lay L;
 	int iMIN, jMIN, kMIN, ni, nj, nk;
@@ -129,31 +131,31 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // dealt out per XCD (see xcdTile), otherwise grid (k tiles, y tiles, x planes).
 __device__ inline int xcdTile(int wid, int nwg);
 
-template<class T, bool HASC, bool NT = true, bool NTG = false>
+template<class T, int CM, bool NT = true, bool NTG = false>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k);
 
-template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4>
+template<class T, int CM, bool NT = true, bool NTG = false, int BY = 4>
 __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, int bz)
 {
 	const int k = a.kMIN + bx * 64 + (int)threadIdx.x;
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
-	curlAt<T, HASC, NT, NTG>(a, i, j, k);
+	curlAt<T, CM, NT, NTG>(a, i, j, k);
 }
 
 // The update of cell (i, j, k) in batched launches: the CPML faces reuse the curl's
 // neighbour values, and the psi slabs go through the cache (the short rows of small worlds
 // cover a slab with partial lines that neighbouring tiles complete).  Measured on 64 k-point
 // worlds: 5 % (E) and 14 % (H) faster than reloading and streaming; slower in k_curl.
-template<class T, bool HASC, bool NT, bool NTG>
+template<class T, int CM, bool NT, bool NTG>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
 	constexpr bool SHARE = true, NTP = false;
 #include "curl_cell.inc"
 }
 
-template<class T, bool HASC, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
+template<class T, int CM, bool NT = true, bool NTG = false, int BY = 4, bool XCD = false>
 __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 {
 	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
@@ -188,12 +190,12 @@ __device__ __forceinline__ int batchEntry(const int *start, int n, int b)
 // Several worlds (and real/imaginary parts) in one launch: entry e owns workgroups
 // [start[e], start[e+1]) of a one-dimensional grid; each workgroup updates TPB consecutive
 // tiles of its entry (tiles as in k_curl).
-// FLAT, HASC: 1 or 0 when every entry of the launch has packed rows / a 1/eps array, -1 to
-// decide per entry.  Specialised launches compile to leaner code (64 k-point worlds: E curl
+// FLAT: 1 or 0 when every entry of the launch has packed rows or none has; CM: 1 or 0 when
+// every entry reads 1/eps arrays or none does; -1 to decide per entry.  Specialised launches compile to leaner code (64 k-point worlds: E curl
 // 380 -> 332 us).  H launches (alt < 0), which read the next plane of E, run each entry's
 // tiles in descending order, so that plane was read as a centre just before (64 k-point
 // worlds: H curl 331 -> 306 us; the 256^3 k_curl shows no difference).
-template<class T, int TPB, int FLAT = -1, int HASC = -1>
+template<class T, int TPB, int FLAT = -1, int CM = -1>
 __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
 {
 	const int b = (int)blockIdx.x;
@@ -211,8 +213,8 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 			const int off = (t % per) * 256 + lid, jj = off / pitch - 1, kk = off - (jj + 1) * pitch;
 			if (jj < 0 || jj >= a.nj || kk < 1 || kk > a.nk) continue;
 			const int i = a.iMIN + t / per, j = a.jMIN + jj, k = a.kMIN - 1 + kk;
-			if (HASC == 1 || (HASC < 0 && a.C0)) curlAt<T, true>(a, i, j, k);
-			else curlAt<T, false>(a, i, j, k);
+			if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1>(a, i, j, k);
+			else curlAt<T, 0>(a, i, j, k);
 		}
 		return;
 	}
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 	for (int t0 = (b - start[e]) * TPB, t1 = min(t0 + TPB, tiles); t0 < t1; t0++) {
 		const int t = a.alt < 0 ? tiles - 1 - t0 : t0;
 		const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
-		if (HASC == 1 || (HASC < 0 && a.C0)) curlCell<T, true>(a, bx, by, bz);
-		else curlCell<T, false>(a, bx, by, bz);
+		if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1>(a, bx, by, bz);
+		else curlCell<T, 0>(a, bx, by, bz);
 	}
 }
 
@@ -643,6 +645,8 @@ struct alis_dev_world {
 	void *dt[2][3] = {};        // [h][axis] dt/d arrays (T), index - MIN
 	void *cp[2][3][3] = {};     // [h][axis][b, ca, cb] CPML arrays or null
 	void *cpk[2][3] = {};       // [h][axis] the same packed (b, ca, cb, 0) per index
+	void *seg = nullptr;        // 1/eps per 64-cell row segment (see rowSegments) or null
+	double epsBytes = 0;        // bytes per cell the E curl reads for 1/eps
 	DevSlab slab[2][6];
 	std::vector<DevObj> obj;
 	void *objTab = nullptr;     // ObjDev<T>[obj.size()] in device memory
@@ -989,6 +993,45 @@ void buildPieces(const std::vector<PostOp<T>> &v, int nadd, std::vector<PostPiec
 	}
 }
 
+// 1/eps per row segment of k_curl's tiles (64 cells along k): (x, y, z, 1) when every cell
+// of the segment holds the same triple (bit patterns), else zeros.  A wavefront of the E
+// curl covers one segment: on a uniform one it takes 1/eps from this table instead of
+// reading three arrays (most segments of a world lie inside one material; the values, and
+// so the results, are the same).
+template<class T>
+int rowSegments(alis_dev_world *D)
+{
+	const alis_dev_world_desc *W = &D->W;
+	const int nkb = (D->kNUM + 63) / 64;
+	const long nseg = (long)D->iNUM * D->jNUM * nkb;
+	std::vector<T> tab(4 * nseg, T(0));
+	long uniform = 0, mixedCells = 0;
+	for (int i = D->iMIN; i <= D->iMAX; i++)
+		for (int j = D->jMIN; j <= D->jMAX; j++)
+			for (int b = 0; b < nkb; b++) {
+				const int k0 = D->kMIN + 64 * b, k1 = std::min(k0 + 63, D->kMAX);
+				bool same = true;
+				for (int a = 0; a < 3 && same; a++) {
+					const float *row = W->eps[a][i][j];
+					for (int k = k0 + 1; k <= k1 && same; k++) same = !memcmp(&row[k], &row[k0], sizeof(float));
+				}
+				T *e = &tab[4 * (((long)(i - D->iMIN) * D->jNUM + (j - D->jMIN)) * nkb + b)];
+				if (same) {
+					for (int a = 0; a < 3; a++) e[a] = (T)W->eps[a][i][j][k0];
+					e[3] = T(1);
+					uniform++;
+				}
+				else mixedCells += k1 - k0 + 1;
+			}
+	if (devAlloc(D, &D->seg, tab.size() * sizeof(T))) return -1;
+	HIPCHECK(hipMemcpyAsync(D->seg, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	const double cells = (double)D->iNUM * D->jNUM * D->kNUM;
+	D->epsBytes = (mixedCells * 3.0 * D->esz + nseg * 4.0 * D->esz) / cells;
+	(void)uniform;
+	return 0;
+}
+
 // Commit: allocate and upload everything of the descriptor.
 template<class T>
 int commitWorld(alis_dev_world *D)
@@ -1013,6 +1056,7 @@ int commitWorld(alis_dev_world *D)
 			if (devAlloc(D, &D->C[a], D->fsize * sizeof(T))) return -1;
 			if (uploadBox<T>(D, D->C[a], D->L, b, hostBlock(W->eps[a], b))) return -1;
 		}
+		if (rowSegments<T>(D)) return -1;
 	}
 
 	// per-axis step ratios and CPML coefficients
@@ -1169,6 +1213,7 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 	a.F0 = (T *)D->F[h][m][0], a.F1 = (T *)D->F[h][m][1], a.F2 = (T *)D->F[h][m][2];
 	a.G0 = (const T *)D->F[1-h][m][0], a.G1 = (const T *)D->F[1-h][m][1], a.G2 = (const T *)D->F[1-h][m][2];
 	if (h == 0) a.C0 = (const T *)D->C[0], a.C1 = (const T *)D->C[1], a.C2 = (const T *)D->C[2];
+	if (h == 0) a.seg = (const typename Quad<T>::V *)D->seg;
 	a.dtx = (const T *)D->dt[h][0], a.dty = (const T *)D->dt[h][1], a.dtz = (const T *)D->dt[h][2];
 	a.L = D->L;
 	a.iMIN = D->iMIN, a.jMIN = D->jMIN, a.kMIN = D->kMIN;
@@ -1204,8 +1249,9 @@ int coreHalf(alis_dev_world *D, int h)
 	for (int m = 0; m <= D->cplx; m++) {
 		CurlArgs<T> a = curlArgs<T>(D, h, m);
 		hipEvent_t s0 = m == 0 ? e0 : nullptr, s1 = m == D->cplx ? e1 : nullptr;
-		if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, true>, grid, block, 0, D->st, s0, s1, 0, a);
-		else hipExtLaunchKernelGGL(k_curl<T, false>, grid, block, 0, D->st, s0, s1, 0, a);
+		if (h == 0 && D->seg) hipExtLaunchKernelGGL(k_curl<T, 2>, grid, block, 0, D->st, s0, s1, 0, a);
+		else if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, 1>, grid, block, 0, D->st, s0, s1, 0, a);
+		else hipExtLaunchKernelGGL(k_curl<T, 0>, grid, block, 0, D->st, s0, s1, 0, a);
 	}
 	if (e1) {
 		D->evPairs[h].push_back(e0);
@@ -1652,7 +1698,7 @@ struct alis_dev_batch {
 	void *curl[2] = {};         // CurlArgs<T>[] per half step
 	void *curlStart[2] = {};    // int[] workgroup starts
 	int curlN[2] = {}, curlBlocks[2] = {};
-	int curlFlat[2] = {-1, -1}, curlHasC[2] = {-1, -1};  // 1/0: every entry has it, -1: mixed
+	int curlFlat[2] = {-1, -1}, curlCM[2] = {-1, -1};  // shared by every entry, or -1: mixed
 	bool postReady = false;
 	struct Round {
 		std::vector<int> world;     // index into W
@@ -1701,7 +1747,7 @@ int batchBuildCurl(alis_dev_batch *B)
 		for (const CurlArgs<T> &a : args) nflat += a.flatPitch != 0, nc += a.C0 != nullptr;
 		const int n = (int)args.size();
 		B->curlFlat[h] = nflat == n ? 1 : nflat == 0 ? 0 : -1;
-		B->curlHasC[h] = nc == n ? 1 : nc == 0 ? 0 : -1;
+		B->curlCM[h] = nc == n ? 1 : nc == 0 ? 0 : -1;
 		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
 		if (batchAlloc(B, &B->curlStart[h], start.size() * sizeof(int))) return -1;
 		HIPCHECK(hipMemcpyAsync(B->curl[h], args.data(), args.size() * sizeof(CurlArgs<T>), hipMemcpyHostToDevice, B->st));
@@ -1757,12 +1803,14 @@ int batchCore(alis_dev_batch *B, int h)
 	const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h];
 	const int *start = (const int *)B->curlStart[h];
 	const dim3 grid(B->curlBlocks[h]), block(64, 4);
-	const int f = B->curlFlat[h], c = B->curlHasC[h];
-	if (f == 1 && c == 1) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, 1, 1>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h]);
-	else if (f == 1 && c == 0) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, 1, 0>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h]);
-	else if (f == 0 && c == 1) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, 0, 1>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h]);
-	else if (f == 0 && c == 0) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, 0, 0>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h]);
-	else hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h]);
+	const int f = B->curlFlat[h], c = B->curlCM[h];
+#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, F, C>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h])
+	if (f == 1 && c == 1) CURL_BATCH(1, 1);
+	else if (f == 1 && c == 0) CURL_BATCH(1, 0);
+	else if (f == 0 && c == 1) CURL_BATCH(0, 1);
+	else if (f == 0 && c == 0) CURL_BATCH(0, 0);
+	else CURL_BATCH(-1, -1);
+#undef CURL_BATCH
 	HIPCHECK(hipGetLastError());
 	if (e1) {
 		D0->evPairs[h].push_back(e0);
@@ -2519,6 +2567,11 @@ int alis_dev_synchronize(alis_dev_world *D)
 {
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
+}
+
+double alis_dev_eps_bytes(const alis_dev_world *D)
+{
+	return D->seg ? D->epsBytes : D->C[0] ? 3.0 * D->esz : 0.0;
 }
 
 int alis_dev_timing(alis_dev_world *D, int enable)

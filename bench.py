@@ -71,15 +71,20 @@ def load():
     alis.alis_dev_synchronize.argtypes = [ctypes.c_void_p]
     alis.alis_dev_bytes.restype = ctypes.c_long
     alis.alis_dev_bytes.argtypes = [ctypes.c_void_p]
+    alis.alis_dev_eps_bytes.restype = ctypes.c_double
+    alis.alis_dev_eps_bytes.argtypes = [ctypes.c_void_p]
     hip = ctypes.CDLL("libamdhip64.so")
     return alis, w, hip
 
 
 def curl_bytes(num, pml, s, has_eps):
-    """Algorithmic HBM bytes of one curl launch: E from H reads E, H (and 1/eps) and writes
-    E, 3 components each; H from E reads H, E and writes H.  CPML: each slab cell reads and
-    writes its two psi values.  Slabs are Num = pml-1 cells thick over the full cross
-    section (src/update.c:119-124)."""
+    """Algorithmic HBM bytes of one curl launch (the compulsory traffic: every state element
+    read and written once, every coefficient read once): E from H reads E, H (and 1/eps, 3
+    values per cell) and writes E, 3 components each; H from E reads H, E and writes H.
+    CPML: each slab cell reads and writes its two psi values.  Slabs are Num = pml-1 cells
+    thick over the full cross section (src/update.c:119-124).  The single-world E curl reads
+    1/eps from a per-segment table where 64 cells along z hold one material, so it moves
+    fewer bytes than this (the engine's figure is reported beside it)."""
     n = num[0] * num[1] * num[2]
     e = n * (12 if has_eps else 9) * s
     h = n * 9 * s
@@ -118,7 +123,8 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None):
     cplx = ctypes.c_int()
     nw = w.wl_worlds(ctx, ctypes.byref(cplx))
     out = {"cells": w.wl_cells(ctx), "sec": t1 - t0, "kern": kern, "num": list(num), "pml": list(pml),
-           "eps": eps, "nf": nf, "surf": surf.value, "hbm_bytes": alis.alis_dev_bytes(dev),
+           "eps": eps, "eps_bytes": alis.alis_dev_eps_bytes(dev), "nf": nf, "surf": surf.value,
+           "hbm_bytes": alis.alis_dev_bytes(dev),
            "worlds": nw, "parts": 1 + cplx.value}
     if farfield:
         import tempfile
@@ -151,6 +157,7 @@ def roofline(r, s, dft_s, steps):
         "bytes_per_launch": dom[1], "launch_ms": round(dom[2] * 1e3, 5), "launches": dom[3],
         "other_curl": {"bytes_per_launch": bh if dom[0] == "E" else be,
                        "launch_ms": round((th if dom[0] == "E" else te) * 1e3, 5)},
+        "eps_read_bytes_per_cell": round(r["eps_bytes"], 3),
         "step_bytes": step_bytes,
         "step_frac": round(step_bytes / (r["sec"] / steps) / PEAK_HBM, 4),
     }
@@ -325,6 +332,7 @@ def batched(alis, w, hip, args):
     with quiet():
         rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
         r1 = run(alis, w, hip, "kpoints", 1, 64, 64, args.warmup, ksteps)
+    rb["eps_bytes"] = 3 * 4 if rb["eps"] else 0       # batched launches read the 1/eps arrays
     vb = rb["cells"] * ksteps / rb["sec"] / 1e6
     v1 = r1["cells"] * ksteps / r1["sec"] / 1e6
     rlb = roofline(rb, 4, 8, ksteps)

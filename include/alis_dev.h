@@ -208,6 +208,12 @@ int alis_dev_synchronize(alis_dev_world *D);
 int alis_dev_timing(alis_dev_world *D, int enable);
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches);
 
+/* Average bytes per grid cell that the single-world E curl reads for 1/eps: 3 field-sized
+ * values in row segments (64 cells along z) that hold more than one (x, y, z) triple, one
+ * table entry per segment, 0 without objects.  Measurement helper (the algorithmic bytes of
+ * the E launch); no counterpart in the reference. */
+double alis_dev_eps_bytes(const alis_dev_world *D);
+
 /* Device-resident bytes of D: fields, coefficients, CPML and ADE state, source tables,
  * phaser accumulators and staging (not the TF/SF auxiliary worlds, which have their own). */
 long alis_dev_bytes(alis_dev_world *D);

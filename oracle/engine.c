@@ -593,6 +593,7 @@ int alis_dev_synchronize(alis_dev_world *D) { (void)D; return 0; }
 int alis_dev_timing(alis_dev_world *D, int enable) { (void)D, (void)enable; return 0; }
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches) { (void)D, (void)h; *ms = 0, *launches = 0; return 0; }
 long alis_dev_bytes(alis_dev_world *D) { return D->bytes; }
+double alis_dev_eps_bytes(const alis_dev_world *D) { return D->hasEps ? 3.0 * (D->fp64 ? 8 : 4) : 0.0; }
 
 /* ---- batches: the worlds one after the other (the oracle applies strips and sources as
  * they are recorded, so a batch half step is core of every world, then their ghosts) ---- */
