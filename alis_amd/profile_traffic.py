@@ -46,9 +46,12 @@ def main():
            "cells": cells, "fields": "fp32",
            "calibration": {"copy_bytes": nbytes, "FETCH_SIZE_kib": cF, "WRITE_SIZE_kib": cW,
                            "fetch_scale": round(fs, 5), "write_scale": round(ws, 5)}}
-    for name, pat in (("E", "k_curl<float, true"), ("H", "k_curl<float, false")):
-        f = [float(r["Counter_Value"]) for r in F if pat in r["Kernel_Name"] and int(r["Grid_Size"]) == cells]
-        w = [float(r["Counter_Value"]) for r in W if pat in r["Kernel_Name"] and int(r["Grid_Size"]) == cells]
+    # k_curl<float, CM, ...>: CM 0 is the H launch (no 1/eps), 1 or 2 the E launch
+    for name, pats in (("E", ("k_curl<float, 1", "k_curl<float, 2")), ("H", ("k_curl<float, 0",))):
+        def mine(r):
+            return any(p in r["Kernel_Name"] for p in pats) and int(r["Grid_Size"]) == cells
+        f = [float(r["Counter_Value"]) for r in F if mine(r)]
+        w = [float(r["Counter_Value"]) for r in W if mine(r)]
         fa, wa = sum(f) / len(f), sum(w) / len(w)
         out[name] = {"FETCH_SIZE_kib": round(fa, 1), "WRITE_SIZE_kib": round(wa, 1), "dispatches": len(f),
                      "traffic_bytes": round((fa * fs + wa * ws) * 1024)}
