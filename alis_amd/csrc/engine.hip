@@ -472,24 +472,38 @@ __device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
 		}
 }
 
-template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a)
+// Workgroup w of a one-dimensional phaser grid of ny frequency blocks per cell block: the
+// frequency blocks of one cell block run on one XCD (the dispatcher deals workgroup ids to
+// the 8 XCDs in turn), so that XCD's L2 fetches the cells' field lines once instead of every
+// XCD fetching them (the cell blocks are padded to a multiple of 8).
+__device__ inline void phaserBlock(int w, int ny, int &xb, int &yb)
 {
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= a.ncell) return;
-	phaserCell<T, TA>(a, t, (int)blockIdx.y * PHS_QPT);
+	xb = w % 8 + 8 * (w / (8 * ny));
+	yb = (w / 8) % ny;
 }
 
-// Several phasers (of several worlds) in one launch: entry e owns workgroups
-// [start[e], start[e+1]) along x; y runs over frequency pairs up to the largest entry.
 template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
+__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a, int ny)
 {
-	const int b = (int)blockIdx.x;
+	int xb, yb;
+	phaserBlock((int)blockIdx.x, ny, xb, yb);
+	const long t = (long)xb * 256 + threadIdx.x;
+	if (t >= a.ncell) return;
+	phaserCell<T, TA>(a, t, yb * PHS_QPT);
+}
+
+// Several phasers (of several worlds) in one launch: entry e owns cell blocks
+// [start[e], start[e+1]); ny frequency blocks up to the largest entry, laid out as in k_phaser.
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n, int nb, int ny)
+{
+	int b, yb;
+	phaserBlock((int)blockIdx.x, ny, b, yb);
+	if (b >= nb) return;
 	const int e = batchEntry(start, n, b);
 	const PhsArgs<T> &a = args[e];
 	const long t = (long)(b - start[e]) * 256 + threadIdx.x;
-	const int qa = (int)blockIdx.y * PHS_QPT;
+	const int qa = yb * PHS_QPT;
 	if (t >= a.ncell || qa >= a.nq) return;
 	phaserCell<T, TA>(a, t, qa);
 }
@@ -1601,18 +1615,20 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 			a.eE[q][0] = expE[2 * (q0 + q)], a.eE[q][1] = expE[2 * (q0 + q) + 1];
 			a.eH[q][0] = expH[2 * (q0 + q)], a.eH[q][1] = expH[2 * (q0 + q) + 1];
 		}
-		dim3 grid(blocksFor(P.ncell), (a.nq + PHS_QPT - 1) / PHS_QPT);
-		k_phaser<T, TA><<<grid, 256, 0, D->st>>>(a);
+		const int ny = (a.nq + PHS_QPT - 1) / PHS_QPT;
+		const long nb = (blocksFor(P.ncell) + 7) / 8 * 8;
+		k_phaser<T, TA><<<nb * ny, 256, 0, D->st>>>(a, ny);
 		HIPCHECK(hipGetLastError());
 	}
 	return 0;
 }
 
-// Copy launch arguments staged in pinned host memory to the device in stream order (k_fetch).
-void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st)
+// Copy launch arguments staged in pinned host memory to the device in stream order (k_fetch);
+// `done` (bound to the copy launch, no marker) tells when the staging can be rewritten.
+void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st, hipEvent_t done)
 {
 	const int words = (int)(bytes / 4), blocks = std::max(1, std::min(64, (words + 255) / 256));
-	k_fetch<<<blocks, 256, 0, st>>>((unsigned *)dev, (const unsigned *)pinned, words);
+	hipExtLaunchKernelGGL(k_fetch, dim3(blocks), dim3(256), 0, st, nullptr, done, 0, (unsigned *)dev, (const unsigned *)pinned, words);
 }
 
 // One sample into several phasers in one launch (entries of one field and accumulator type).
@@ -1645,10 +1661,9 @@ int phaserBatchUpdate(alis_dev_world *L, const std::vector<PhsArgs<T>> &args)
 	else HIPCHECK(hipEventSynchronize(L->phsDone));
 	memcpy(L->phsPinned, args.data(), args.size() * sizeof(PhsArgs<T>));
 	memcpy((char *)L->phsPinned + ab, start.data(), start.size() * sizeof(int));
-	fetchArgs(L->phsDev, L->phsPinned, bytes, L->st);
-	HIPCHECK(hipEventRecord(L->phsDone, L->st));
-	dim3 grid(blocks, ny);
-	k_phaser_batch<T, TA><<<grid, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsDev, (const int *)((char *)L->phsDev + ab), (int)args.size());
+	fetchArgs(L->phsDev, L->phsPinned, bytes, L->st, L->phsDone);
+	const long nb = (blocks + 7) / 8 * 8;
+	k_phaser_batch<T, TA><<<nb * ny, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsDev, (const int *)((char *)L->phsDev + ab), (int)args.size(), blocks, ny);
 	HIPCHECK(hipGetLastError());
 	return 0;
 }
@@ -1838,8 +1853,7 @@ int batchPost(alis_dev_batch *B, int h)
 			for (size_t w = 0; w < D->pendSrc[h].size(); w++) a[e].src[w] = D->pendSrc[h][w];
 			for (size_t q = 0; q < D->pendWave[h].size(); q++) a[e].wave[q] = (T)D->pendWave[h][q];
 		}
-		fetchArgs(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), B->st);
-		HIPCHECK(hipEventRecord(R.done, B->st));
+		fetchArgs(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), B->st, R.done);
 		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
 		HIPCHECK(hipGetLastError());
 	}
