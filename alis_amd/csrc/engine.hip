@@ -414,9 +414,10 @@ template<class T> struct PhsArgs {
 	double eE[MAXF][2], eH[MAXF][2];
 };
 
-// One face cell t of a phaser, frequencies qa .. qa+PHS_QPT-1 of the launch's range.
-template<class T, class TA>
-__device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
+// The sampled fields of face cell t of a phaser: the two tangential E components and the two
+// sums of the tangential H components across the face (src/phaser.c:83-110).
+template<class T>
+__device__ __forceinline__ void phaserFields(const PhsArgs<T> &a, long t, T &e1, T &e2, T &h1, T &h2)
 {
 	int f = 0;
 	while (f + 1 < a.nface && t >= a.face[f + 1].off) f++;
@@ -424,7 +425,6 @@ __device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
 	long local = t - g.off;
 	int u = (int)(local / g.n2), v = (int)(local % g.n2);
 	int i, j, k;
-	T e1, e2, h1, h2;
 	if (g.kind < 2) {
 		i = g.fixed, j = a.jMin + u, k = a.kMin + v;
 		long id = a.L.at(i, j, k), s = a.L.si;
@@ -443,6 +443,14 @@ __device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
 		e1 = a.E[0][id], e2 = a.E[1][id];
 		h1 = a.H[0][id] + a.H[0][id + 1], h2 = a.H[1][id] + a.H[1][id + 1];
 	}
+}
+
+// One face cell t of a phaser, frequencies qa .. qa+PHS_QPT-1 of the launch's range.
+template<class T, class TA>
+__device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
+{
+	T e1, e2, h1, h2;
+	phaserFields(a, t, e1, e2, h1, h2);
 	// one (re, im) pair per vector access; accumulators are streamed once per sample, so
 	// non-temporally.  All loads are issued before the first store: a store between them
 	// would make the next loads wait for it.
@@ -506,6 +514,62 @@ __global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restri
 	const int qa = yb * PHS_QPT;
 	if (t >= a.ncell || qa >= a.nq) return;
 	phaserCell<T, TA>(a, t, qa);
+}
+
+// Deferred phaser updates: a sample stores the face fields of one step (ring slot, [4][cell]);
+// a flush adds the pending samples to the accumulators in sample order, the same additions as
+// one k_phaser launch per sample, reading and writing the accumulators once for all of them.
+template<class T>
+__global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict__ slot)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= a.ncell) return;
+	T e1, e2, h1, h2;
+	phaserFields(a, t, e1, e2, h1, h2);
+	slot[t] = e1, slot[a.ncell + t] = e2, slot[2 * a.ncell + t] = h1, slot[3 * a.ncell + t] = h2;
+}
+
+// ring: ns samples [s][4][cell]; ex: per sample `stride` doubles, frequency q of the launch at
+// ex[s * stride + 4 q] = (eE re, eE im, eH re, eH im).
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phs_flush(PhsArgs<T> a, const T *__restrict__ ring, int ns,
+                                                   const double *__restrict__ ex, int stride, int ny)
+{
+	int xb, yb;
+	phaserBlock((int)blockIdx.x, ny, xb, yb);
+	const long t = (long)xb * 256 + threadIdx.x;
+	if (t >= a.ncell) return;
+	typedef typename Pair<TA>::V V;
+	V *acc = (V *)a.acc;
+	const TA half = TA(0.5);
+	const int qa = yb * PHS_QPT, nq = min(PHS_QPT, a.nq - qa);
+	V x[PHS_QPT][4];
+#pragma unroll
+	for (int u = 0; u < PHS_QPT; u++)
+		if (u < nq) {
+			const V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
+			for (int c = 0; c < 4; c++) x[u][c] = __builtin_nontemporal_load(p + c * a.ncell);
+		}
+	for (int s = 0; s < ns; s++) {
+		const T *r = ring + (long)s * 4 * a.ncell + t;
+		const T e1 = r[0], e2 = r[a.ncell], h1 = r[2 * a.ncell], h2 = r[3 * a.ncell];
+#pragma unroll
+		for (int u = 0; u < PHS_QPT; u++)
+			if (u < nq) {
+				const double *f = ex + (long)s * stride + 4 * (qa + u);
+				const TA er = TA(f[0]), ei = TA(f[1]), hr = TA(f[2]), hi = TA(f[3]);
+				x[u][0][0] = x[u][0][0] + er * TA(e1), x[u][0][1] = x[u][0][1] + ei * TA(e1);
+				x[u][1][0] = x[u][1][0] + er * TA(e2), x[u][1][1] = x[u][1][1] + ei * TA(e2);
+				x[u][2][0] = x[u][2][0] + hr * TA(h1) * half, x[u][2][1] = x[u][2][1] + hi * TA(h1) * half;
+				x[u][3][0] = x[u][3][0] + hr * TA(h2) * half, x[u][3][1] = x[u][3][1] + hi * TA(h2) * half;
+			}
+	}
+#pragma unroll
+	for (int u = 0; u < PHS_QPT; u++)
+		if (u < nq) {
+			V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
+			for (int c = 0; c < 4; c++) __builtin_nontemporal_store(x[u][c], p + c * a.ncell);
+		}
 }
 
 // Per-call launch arguments of batched launches from pinned host memory into device memory,
@@ -641,8 +705,16 @@ struct DevPhaser {
 	long ncell = 0;
 	void *acc = nullptr;        // TA
 	float *stage = nullptr;     // device float buffer for host syncs
-	size_t bytes = 0;           // device bytes of acc and stage
+	size_t bytes = 0;           // device bytes of acc and stage (and the ring)
+	// deferred updates: samples of the face fields not yet added to acc
+	void *ring = nullptr;       // [PHS_RING][4][ncell] T
+	double *exPinned = nullptr; // [PHS_RING][nf][4] their DFT factors, staged
+	double *exDev = nullptr;
+	hipEvent_t exDone = nullptr;  // the staging has been copied
+	int pend = 0, pq0 = 0, pnq = 0;  // samples pending and their frequency range
 };
+
+constexpr int PHS_RING = 8;     // samples per deferred flush
 
 struct alis_dev_world {
 	alis_dev_world_desc W{};    // copy of the descriptor (arrays below keep its tables alive)
@@ -1598,28 +1670,51 @@ int writeField(alis_dev_world *D, int h, int m, int axis, const double *in)
 
 // ---- phasers ----
 
+template<class T>
+PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH);
+void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st, hipEvent_t done);
+
+// Add the pending samples of P to its accumulators (one k_phs_flush launch).
+template<class T, class TA>
+int phaserFlush(alis_dev_world *D, DevPhaser &P)
+{
+	if (!P.pend) return 0;
+	fetchArgs(P.exDev, P.exPinned, (size_t)P.pend * 4 * P.pnq * sizeof(double), D->st, P.exDone);
+	PhsArgs<T> a = phaserArgs<T>(D, P, P.pq0, 0, nullptr, nullptr);
+	a.nq = P.pnq;
+	const int ny = (a.nq + PHS_QPT - 1) / PHS_QPT;
+	const long nb = (blocksFor(P.ncell) + 7) / 8 * 8;
+	k_phs_flush<T, TA><<<nb * ny, 256, 0, D->st>>>(a, (const T *)P.ring, P.pend, P.exDev, 4 * P.pnq, ny);
+	HIPCHECK(hipGetLastError());
+	P.pend = 0;
+	return 0;
+}
+
+int phaserFlushAny(alis_dev_world *D, DevPhaser &P)
+{
+	if (!P.pend) return 0;
+	if (D->fp64) return D->dft64 ? phaserFlush<double, double>(D, P) : phaserFlush<double, float>(D, P);
+	return D->dft64 ? phaserFlush<float, double>(D, P) : phaserFlush<float, float>(D, P);
+}
+
+// One sample of P (src/phaser.c:83-110): the face fields go to the ring and the factors to
+// the staging; the accumulators take the samples PHS_RING at a time, or before anything reads
+// them, in sample order (bitwise the running sums of one update per sample).
 template<class T, class TA>
 int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH)
 {
-	PhsArgs<T> a{};
-	for (int c = 0; c < 3; c++) a.E[c] = (const T *)D->F[0][0][c], a.H[c] = (const T *)D->F[1][0][c];
-	a.L = D->L;
-	a.iMin = P.iMin, a.jMin = P.jMin, a.kMin = P.kMin, a.nface = P.nface;
-	for (int f = 0; f < P.nface; f++) a.face[f] = P.face[f];
-	a.ncell = P.ncell;
-	a.acc = P.acc;
-	for (int q0 = 0; q0 < qn; q0 += MAXF) {
-		a.q0 = qa + q0;
-		a.nq = std::min(MAXF, qn - q0);
-		for (int q = 0; q < a.nq; q++) {
-			a.eE[q][0] = expE[2 * (q0 + q)], a.eE[q][1] = expE[2 * (q0 + q) + 1];
-			a.eH[q][0] = expH[2 * (q0 + q)], a.eH[q][1] = expH[2 * (q0 + q) + 1];
-		}
-		const int ny = (a.nq + PHS_QPT - 1) / PHS_QPT;
-		const long nb = (blocksFor(P.ncell) + 7) / 8 * 8;
-		k_phaser<T, TA><<<nb * ny, 256, 0, D->st>>>(a, ny);
-		HIPCHECK(hipGetLastError());
+	if (P.pend && (qa != P.pq0 || qn != P.pnq) && phaserFlush<T, TA>(D, P)) return -1;
+	if (!P.pend) {
+		HIPCHECK(hipEventSynchronize(P.exDone));   // the last flush has copied the staging
+		P.pq0 = qa, P.pnq = qn;
 	}
+	PhsArgs<T> a = phaserArgs<T>(D, P, qa, 0, nullptr, nullptr);
+	k_phs_sample<T><<<blocksFor(P.ncell), 256, 0, D->st>>>(a, (T *)P.ring + (size_t)P.pend * 4 * P.ncell);
+	HIPCHECK(hipGetLastError());
+	double *x = P.exPinned + (size_t)P.pend * 4 * qn;
+	for (int q = 0; q < qn; q++)
+		x[4 * q] = expE[2 * q], x[4 * q + 1] = expE[2 * q + 1], x[4 * q + 2] = expH[2 * q], x[4 * q + 3] = expH[2 * q + 1];
+	if (++P.pend == PHS_RING) return phaserFlush<T, TA>(D, P);
 	return 0;
 }
 
@@ -1937,7 +2032,11 @@ void alis_dev_release(alis_dev_world *D)
 			if (s.wavePinned) hipHostFree(s.wavePinned);
 			if (s.waveDone) hipEventDestroy(s.waveDone);
 		}
-	for (DevPhaser &P : D->phs) if (P.live) hipFree(P.acc), hipFree(P.stage);
+	for (DevPhaser &P : D->phs)
+		if (P.live) {
+			hipFree(P.acc), hipFree(P.stage), hipFree(P.ring), hipFree(P.exDev);
+			hipHostFree(P.exPinned), hipEventDestroy(P.exDone);
+		}
 	if (D->halfDone) hipEventDestroy(D->halfDone);
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
@@ -2116,7 +2215,13 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 	const size_t accBytes = (size_t)std::max(n, 1L) * (D->dft64 ? 8 : 4);
 	HIPCHECK(hipMalloc(&Q.acc, accBytes));
 	HIPCHECK(hipMalloc((void **)&Q.stage, (size_t)std::max(n, 1L) * sizeof(float)));
-	Q.bytes = accBytes + (size_t)std::max(n, 1L) * sizeof(float);
+	const size_t ringBytes = (size_t)PHS_RING * 4 * std::max(Q.ncell, 1L) * D->esz;
+	const size_t exBytes = (size_t)PHS_RING * 4 * std::max(Q.nf, 1) * sizeof(double);
+	HIPCHECK(hipMalloc(&Q.ring, ringBytes));
+	HIPCHECK(hipMalloc((void **)&Q.exDev, exBytes));
+	HIPCHECK(hipHostMalloc((void **)&Q.exPinned, exBytes, hipHostMallocDefault));
+	HIPCHECK(hipEventCreateWithFlags(&Q.exDone, hipEventDisableTiming));
+	Q.bytes = accBytes + (size_t)std::max(n, 1L) * sizeof(float) + ringBytes + exBytes;
 	D->bytes += Q.bytes;
 	// start from the host accumulators (normally zero)
 	std::vector<float> host(std::max(n, 1L));
@@ -2162,6 +2267,7 @@ int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const i
 			return fail("phasers updated together must share their field and accumulator precision");
 	}
 	for (int e = 1; e < n; e++) if (joinStream(D[e], D[0])) return -1;
+	for (int e = 0; e < n; e++) if (phaserFlushAny(D[e], D[e]->phs[id[e]])) return -1;
 	std::vector<PhsArgs<float>> af;
 	std::vector<PhsArgs<double>> ad;
 	for (int e = 0; e < n; e++) {
@@ -2189,6 +2295,7 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 	if (need < 0 || n < need) return fail("phaser read buffer too small");
 	DevPhaser &P = D->phs[id];
 	if (!need) return 0;
+	if (phaserFlushAny(D, P)) return -1;
 	if (ensureScratch(D, need)) return -1;
 	if (D->dft64) k_acc_to_double<double><<<blocksFor(need), 256, 0, D->st>>>((const double *)P.acc, (double *)D->scratch, need);
 	else k_acc_to_double<float><<<blocksFor(need), 256, 0, D->st>>>((const float *)P.acc, (double *)D->scratch, need);
@@ -2204,6 +2311,7 @@ int alis_dev_phaser_sync(alis_dev_world *D, int id)
 	if (need < 0) return fail("no such phaser");
 	DevPhaser &Q = D->phs[id];
 	if (!need) return 0;
+	if (phaserFlushAny(D, Q)) return -1;
 	if (D->dft64) k_acc_to_float<double><<<blocksFor(need), 256, 0, D->st>>>((const double *)Q.acc, Q.stage, need);
 	else HIPCHECK(hipMemcpyAsync(Q.stage, Q.acc, need * sizeof(float), hipMemcpyDeviceToDevice, D->st));
 	HIPCHECK(hipGetLastError());
@@ -2226,6 +2334,11 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 	HIPCHECK(hipStreamSynchronize(D->st));
 	HIPCHECK(hipFree(P.acc));
 	HIPCHECK(hipFree(P.stage));
+	HIPCHECK(hipFree(P.ring));
+	HIPCHECK(hipFree(P.exDev));
+	HIPCHECK(hipHostFree(P.exPinned));
+	HIPCHECK(hipEventDestroy(P.exDone));
+	P.pend = 0;
 	D->bytes -= P.bytes;
 	P.live = false;
 	return 0;
@@ -2395,6 +2508,7 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 	DevPhaser &Q = D->phs[id];
 	if (q < 0 || q >= Q.nf) return fail("phaser frequency out of range");
 	if (n < 0) return fail("negative column count");
+	if (phaserFlushAny(D, Q)) return -1;
 	memset(S, 0, (size_t)n * 96 * sizeof(double));
 	if (!n) return 0;
 	const int ni = Q.iMax - Q.iMin + 1, nj = Q.jMax - Q.jMin + 1, nk = Q.kMax - Q.kMin + 1;
