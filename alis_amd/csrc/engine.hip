@@ -385,10 +385,11 @@ __global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *__restric
 	postBlock(args[e], (long)(b - start[e]));
 }
 
-// Phaser running DFT (src/phaser.c:83-110): one thread per face cell and pair of
-// frequencies (grid y runs over frequency pairs, so even a small phaser box fills the chip).
-// The face components are read once per thread; accumulators stream coalesced per
-// [frequency][component] row.
+// Phaser running DFT (src/phaser.c:83-110): one thread per face cell and block of PHS_QPT
+// frequencies (the frequency blocks multiply the threads, so even a small phaser box fills
+// the chip).  The face components are read once per thread; accumulators stream coalesced
+// per [frequency][component] row.  A single world's updates are deferred (k_phs_sample,
+// k_phs_flush); batched updates add their sample at once (k_phaser_batch).
 struct PhsFaceGeom {
 	int kind;                   // 0 xMin 1 xMax 2 yMin 3 yMax 4 zMin 5 zMax
 	int fixed;                  // the face's fixed index
@@ -490,18 +491,8 @@ __device__ inline void phaserBlock(int w, int ny, int &xb, int &yb)
 	yb = (w / 8) % ny;
 }
 
-template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phaser(PhsArgs<T> a, int ny)
-{
-	int xb, yb;
-	phaserBlock((int)blockIdx.x, ny, xb, yb);
-	const long t = (long)xb * 256 + threadIdx.x;
-	if (t >= a.ncell) return;
-	phaserCell<T, TA>(a, t, yb * PHS_QPT);
-}
-
-// Several phasers (of several worlds) in one launch: entry e owns cell blocks
-// [start[e], start[e+1]); ny frequency blocks up to the largest entry, laid out as in k_phaser.
+// Several phasers (of several worlds) in one launch, one sample each: entry e owns cell blocks
+// [start[e], start[e+1]); ny frequency blocks up to the largest entry (see phaserBlock).
 template<class T, class TA>
 __global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n, int nb, int ny)
 {
