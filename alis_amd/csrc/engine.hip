@@ -49,7 +49,6 @@ int fail(const std::string &what)
 
 constexpr int ROW_ALIGN_BYTES = 128;
 constexpr int MAXPOLE = ALIS_DEV_MAXPOLES;
-constexpr int MAXF = 64;          // phaser frequencies per update launch
 
 // Index map of one device array: (i - i0)*si + (j - j0)*sj + (k - k0) + koff.
 struct Lay {
@@ -389,7 +388,7 @@ __global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *__restric
 // frequencies (the frequency blocks multiply the threads, so even a small phaser box fills
 // the chip).  The face components are read once per thread; accumulators stream coalesced
 // per [frequency][component] row.  A single world's updates are deferred (k_phs_sample,
-// k_phs_flush); batched updates add their sample at once (k_phaser_batch).
+// k_phs_flush, and k_phs_sample_batch / k_phs_flush_batch for several phasers at once).
 struct PhsFaceGeom {
 	int kind;                   // 0 xMin 1 xMax 2 yMin 3 yMax 4 zMin 5 zMax
 	int fixed;                  // the face's fixed index
@@ -411,8 +410,7 @@ template<class T> struct PhsArgs {
 	PhsFaceGeom face[6];
 	long ncell;
 	void *acc;                  // [freq][comp 4][cell] complex (TA pairs)
-	int q0, nq;
-	double eE[MAXF][2], eH[MAXF][2];
+	int q0, nq;                 // the launch's frequencies (of the accumulators)
 };
 
 // The sampled fields of face cell t of a phaser: the two tangential E components and the two
@@ -446,41 +444,6 @@ __device__ __forceinline__ void phaserFields(const PhsArgs<T> &a, long t, T &e1,
 	}
 }
 
-// One face cell t of a phaser, frequencies qa .. qa+PHS_QPT-1 of the launch's range.
-template<class T, class TA>
-__device__ __forceinline__ void phaserCell(const PhsArgs<T> &a, long t, int qa)
-{
-	T e1, e2, h1, h2;
-	phaserFields(a, t, e1, e2, h1, h2);
-	// one (re, im) pair per vector access; accumulators are streamed once per sample, so
-	// non-temporally.  All loads are issued before the first store: a store between them
-	// would make the next loads wait for it.
-	typedef typename Pair<TA>::V V;
-	V *acc = (V *)a.acc;
-	const TA half = TA(0.5);
-	const int nq = min(PHS_QPT, a.nq - qa);
-	V x[PHS_QPT][4];
-#pragma unroll
-	for (int u = 0; u < PHS_QPT; u++)
-		if (u < nq) {
-			const V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
-			for (int c = 0; c < 4; c++) x[u][c] = __builtin_nontemporal_load(p + c * a.ncell);
-		}
-#pragma unroll
-	for (int u = 0; u < PHS_QPT; u++)
-		if (u < nq) {
-			const int q = qa + u;
-			const TA er = TA(a.eE[q][0]), ei = TA(a.eE[q][1]);
-			const TA hr = TA(a.eH[q][0]), hi = TA(a.eH[q][1]);
-			x[u][0][0] = x[u][0][0] + er * TA(e1), x[u][0][1] = x[u][0][1] + ei * TA(e1);
-			x[u][1][0] = x[u][1][0] + er * TA(e2), x[u][1][1] = x[u][1][1] + ei * TA(e2);
-			x[u][2][0] = x[u][2][0] + hr * TA(h1) * half, x[u][2][1] = x[u][2][1] + hi * TA(h1) * half;
-			x[u][3][0] = x[u][3][0] + hr * TA(h2) * half, x[u][3][1] = x[u][3][1] + hi * TA(h2) * half;
-			V *p = acc + (long)(a.q0 + q) * 4 * a.ncell + t;
-			for (int c = 0; c < 4; c++) __builtin_nontemporal_store(x[u][c], p + c * a.ncell);
-		}
-}
-
 // Workgroup w of a one-dimensional phaser grid of ny frequency blocks per cell block: the
 // frequency blocks of one cell block run on one XCD (the dispatcher deals workgroup ids to
 // the 8 XCDs in turn), so that XCD's L2 fetches the cells' field lines once instead of every
@@ -489,22 +452,6 @@ __device__ inline void phaserBlock(int w, int ny, int &xb, int &yb)
 {
 	xb = w % 8 + 8 * (w / (8 * ny));
 	yb = (w / 8) % ny;
-}
-
-// Several phasers (of several worlds) in one launch, one sample each: entry e owns cell blocks
-// [start[e], start[e+1]); ny frequency blocks up to the largest entry (see phaserBlock).
-template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phaser_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n, int nb, int ny)
-{
-	int b, yb;
-	phaserBlock((int)blockIdx.x, ny, b, yb);
-	if (b >= nb) return;
-	const int e = batchEntry(start, n, b);
-	const PhsArgs<T> &a = args[e];
-	const long t = (long)(b - start[e]) * 256 + threadIdx.x;
-	const int qa = yb * PHS_QPT;
-	if (t >= a.ncell || qa >= a.nq) return;
-	phaserCell<T, TA>(a, t, qa);
 }
 
 // Deferred phaser updates: a sample stores the face fields of one step (ring slot, [4][cell]);
@@ -521,19 +468,15 @@ __global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict_
 }
 
 // ring: ns samples [s][4][cell]; ex: per sample `stride` doubles, frequency q of the launch at
-// ex[s * stride + 4 q] = (eE re, eE im, eH re, eH im).
+// ex[s * stride + 4 q] = (eE re, eE im, eH re, eH im).  Face cell t, frequencies qa ...
 template<class T, class TA>
-__global__ __launch_bounds__(256) void k_phs_flush(PhsArgs<T> a, const T *__restrict__ ring, int ns,
-                                                   const double *__restrict__ ex, int stride, int ny)
+__device__ __forceinline__ void phaserFlushCell(const PhsArgs<T> &a, long t, int qa, const T *__restrict__ ring, int ns,
+                                                const double *__restrict__ ex, int stride)
 {
-	int xb, yb;
-	phaserBlock((int)blockIdx.x, ny, xb, yb);
-	const long t = (long)xb * 256 + threadIdx.x;
-	if (t >= a.ncell) return;
 	typedef typename Pair<TA>::V V;
 	V *acc = (V *)a.acc;
 	const TA half = TA(0.5);
-	const int qa = yb * PHS_QPT, nq = min(PHS_QPT, a.nq - qa);
+	const int nq = min(PHS_QPT, a.nq - qa);
 	V x[PHS_QPT][4];
 #pragma unroll
 	for (int u = 0; u < PHS_QPT; u++)
@@ -561,6 +504,53 @@ __global__ __launch_bounds__(256) void k_phs_flush(PhsArgs<T> a, const T *__rest
 			V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
 			for (int c = 0; c < 4; c++) __builtin_nontemporal_store(x[u][c], p + c * a.ncell);
 		}
+}
+
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phs_flush(PhsArgs<T> a, const T *__restrict__ ring, int ns,
+                                                   const double *__restrict__ ex, int stride, int ny)
+{
+	int xb, yb;
+	phaserBlock((int)blockIdx.x, ny, xb, yb);
+	const long t = (long)xb * 256 + threadIdx.x;
+	if (t >= a.ncell) return;
+	phaserFlushCell<T, TA>(a, t, yb * PHS_QPT, ring, ns, ex, stride);
+}
+
+// The same for several phasers (of several worlds) in one launch each: the samples of every
+// entry (acc: its ring slot), and the flushes of the entries whose ring is full.
+template<class T>
+__global__ __launch_bounds__(256) void k_phs_sample_batch(const PhsArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
+{
+	const int b = (int)blockIdx.x;
+	const int e = batchEntry(start, n, b);
+	const PhsArgs<T> &a = args[e];
+	const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+	if (t >= a.ncell) return;
+	T e1, e2, h1, h2;
+	phaserFields(a, t, e1, e2, h1, h2);
+	T *slot = (T *)a.acc;
+	slot[t] = e1, slot[a.ncell + t] = e2, slot[2 * a.ncell + t] = h1, slot[3 * a.ncell + t] = h2;
+}
+
+template<class T> struct PhsFlush {
+	PhsArgs<T> a;
+	const T *ring;
+	const double *ex;
+	int ns, stride;
+};
+
+template<class T, class TA>
+__global__ __launch_bounds__(256) void k_phs_flush_batch(const PhsFlush<T> *__restrict__ args, const int *__restrict__ start, int n, int nb, int ny)
+{
+	int b, yb;
+	phaserBlock((int)blockIdx.x, ny, b, yb);
+	if (b >= nb) return;
+	const int e = batchEntry(start, n, b);
+	const PhsFlush<T> &F = args[e];
+	const long t = (long)(b - start[e]) * 256 + threadIdx.x;
+	if (t >= F.a.ncell || yb * PHS_QPT >= F.a.nq) return;
+	phaserFlushCell<T, TA>(F.a, t, yb * PHS_QPT, F.ring, F.ns, F.ex, F.stride);
 }
 
 // Per-call launch arguments of batched launches from pinned host memory into device memory,
@@ -687,6 +677,13 @@ struct DevTfsf {
 	alis_dev_world *aux = nullptr;
 };
 
+// Launch arguments staged in pinned host memory and their device copy (see fetchArgs).
+struct Staging {
+	void *pinned = nullptr, *dev = nullptr;
+	size_t bytes = 0;
+	hipEvent_t done = nullptr;       // the last copy from `pinned` has been made
+};
+
 struct DevPhaser {
 	bool live = false;
 	std::vector<float *> cf;    // host cfield data starts [q][face][comp]
@@ -742,10 +739,7 @@ struct alis_dev_world {
 	void *gatherDev = nullptr;       // probe copies: segment table + gathered values
 	float *gatherPinned = nullptr;
 	size_t gatherBytes = 0, gatherPinnedBytes = 0;
-	void *phsPinned = nullptr;       // staging of batched phaser updates led by this world
-	void *phsDev = nullptr;
-	size_t phsBytes = 0;
-	hipEvent_t phsDone = nullptr;    // the last staging copy has been made
+	Staging phsStage, phfStage;      // staging of batched phaser samples and flushes led by this world
 	rocblas_handle blas = nullptr;   // far-field products, created on first use
 	void *ffWork = nullptr;          // far-field workspace
 	size_t ffWorkBytes = 0;
@@ -1662,7 +1656,7 @@ int writeField(alis_dev_world *D, int h, int m, int axis, const double *in)
 // ---- phasers ----
 
 template<class T>
-PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH);
+PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn);
 void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st, hipEvent_t done);
 
 // Add the pending samples of P to its accumulators (one k_phs_flush launch).
@@ -1671,7 +1665,7 @@ int phaserFlush(alis_dev_world *D, DevPhaser &P)
 {
 	if (!P.pend) return 0;
 	fetchArgs(P.exDev, P.exPinned, (size_t)P.pend * 4 * P.pnq * sizeof(double), D->st, P.exDone);
-	PhsArgs<T> a = phaserArgs<T>(D, P, P.pq0, 0, nullptr, nullptr);
+	PhsArgs<T> a = phaserArgs<T>(D, P, P.pq0, P.pnq);
 	a.nq = P.pnq;
 	const int ny = (a.nq + PHS_QPT - 1) / PHS_QPT;
 	const long nb = (blocksFor(P.ncell) + 7) / 8 * 8;
@@ -1699,7 +1693,7 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 		HIPCHECK(hipEventSynchronize(P.exDone));   // the last flush has copied the staging
 		P.pq0 = qa, P.pnq = qn;
 	}
-	PhsArgs<T> a = phaserArgs<T>(D, P, qa, 0, nullptr, nullptr);
+	PhsArgs<T> a = phaserArgs<T>(D, P, qa, qn);
 	k_phs_sample<T><<<blocksFor(P.ncell), 256, 0, D->st>>>(a, (T *)P.ring + (size_t)P.pend * 4 * P.ncell);
 	HIPCHECK(hipGetLastError());
 	double *x = P.exPinned + (size_t)P.pend * 4 * qn;
@@ -1717,46 +1711,108 @@ void fetchArgs(void *dev, const void *pinned, size_t bytes, hipStream_t st, hipE
 	hipExtLaunchKernelGGL(k_fetch, dim3(blocks), dim3(256), 0, st, nullptr, done, 0, (unsigned *)dev, (const unsigned *)pinned, words);
 }
 
-// One sample into several phasers in one launch (entries of one field and accumulator type).
-// Arguments and workgroup starts go through L's pinned staging buffer; before it is
-// rewritten, the copy made from it by the previous call must have been made.
-template<class T, class TA>
-int phaserBatchUpdate(alis_dev_world *L, const std::vector<PhsArgs<T>> &args)
+// Room for `bytes` of launch arguments in S; waits until the last copy from it was made.
+int stagingFor(Staging &S, size_t bytes, hipStream_t st)
 {
-	if (args.empty()) return 0;
-	std::vector<int> start;
-	int blocks = 0, ny = 1;
-	for (const PhsArgs<T> &a : args) {
-		start.push_back(blocks);
-		blocks += (int)blocksFor(a.ncell);
-		ny = std::max(ny, (a.nq + PHS_QPT - 1) / PHS_QPT);
-	}
-	const size_t ab = (args.size() * sizeof(PhsArgs<T>) + 255) / 256 * 256, bytes = ab + start.size() * sizeof(int);
-	if (bytes > L->phsBytes) {
-		if (L->phsPinned) {
-			HIPCHECK(hipStreamSynchronize(L->st));
-			HIPCHECK(hipHostFree(L->phsPinned));
-			HIPCHECK(hipFree(L->phsDev));
+	if (bytes > S.bytes) {
+		if (S.pinned) {
+			HIPCHECK(hipStreamSynchronize(st));
+			HIPCHECK(hipHostFree(S.pinned));
+			HIPCHECK(hipFree(S.dev));
 		}
-		L->phsPinned = nullptr, L->phsDev = nullptr, L->phsBytes = 0;
-		HIPCHECK(hipHostMalloc(&L->phsPinned, bytes, hipHostMallocDefault));
-		HIPCHECK(hipMalloc(&L->phsDev, bytes));
-		L->phsBytes = bytes;
-		if (!L->phsDone) HIPCHECK(hipEventCreateWithFlags(&L->phsDone, hipEventDisableTiming));
+		S.pinned = nullptr, S.dev = nullptr, S.bytes = 0;
+		HIPCHECK(hipHostMalloc(&S.pinned, bytes, hipHostMallocDefault));
+		HIPCHECK(hipMalloc(&S.dev, bytes));
+		S.bytes = bytes;
+		if (!S.done) HIPCHECK(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
 	}
-	else HIPCHECK(hipEventSynchronize(L->phsDone));
-	memcpy(L->phsPinned, args.data(), args.size() * sizeof(PhsArgs<T>));
-	memcpy((char *)L->phsPinned + ab, start.data(), start.size() * sizeof(int));
-	fetchArgs(L->phsDev, L->phsPinned, bytes, L->st, L->phsDone);
-	const long nb = (blocks + 7) / 8 * 8;
-	k_phaser_batch<T, TA><<<nb * ny, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsDev, (const int *)((char *)L->phsDev + ab), (int)args.size(), blocks, ny);
+	else HIPCHECK(hipEventSynchronize(S.done));
+	return 0;
+}
+
+// One sample into each of several phasers (entries of one field and accumulator type; the
+// worlds share L's stream), deferred as phaserUpdate does it, in shared launches: one stores
+// every entry's sample, one flushes the entries whose ring is full.  Arguments, workgroup
+// starts and the flushed samples' DFT factors go through L's pinned staging.
+template<class T, class TA>
+int phaserBatchUpdate(alis_dev_world *L, alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
+                      const double *const *expE, const double *const *expH, int n)
+{
+	std::vector<PhsArgs<T>> sa;
+	std::vector<int> sstart, full;
+	int sblocks = 0;
+	for (int e = 0; e < n; e++) {
+		DevPhaser &P = D[e]->phs[id[e]];
+		if (!P.ncell || !nq[e]) continue;
+		if (P.pend && (q0[e] != P.pq0 || nq[e] != P.pnq) && phaserFlush<T, TA>(D[e], P)) return -1;
+		if (!P.pend) {
+			HIPCHECK(hipEventSynchronize(P.exDone));
+			P.pq0 = q0[e], P.pnq = nq[e];
+		}
+		PhsArgs<T> a = phaserArgs<T>(D[e], P, q0[e], nq[e]);
+		a.acc = (T *)P.ring + (size_t)P.pend * 4 * P.ncell;
+		sa.push_back(a);
+		sstart.push_back(sblocks);
+		sblocks += (int)blocksFor(P.ncell);
+		double *x = P.exPinned + (size_t)P.pend * 4 * P.pnq;
+		for (int q = 0; q < nq[e]; q++)
+			x[4 * q] = expE[e][2 * q], x[4 * q + 1] = expE[e][2 * q + 1], x[4 * q + 2] = expH[e][2 * q], x[4 * q + 3] = expH[e][2 * q + 1];
+		if (++P.pend == PHS_RING) full.push_back(e);
+	}
+	if (sa.empty()) return 0;
+	{
+		const size_t ab = (sa.size() * sizeof(PhsArgs<T>) + 255) / 256 * 256, bytes = ab + sstart.size() * sizeof(int);
+		if (stagingFor(L->phsStage, bytes, L->st)) return -1;
+		memcpy(L->phsStage.pinned, sa.data(), sa.size() * sizeof(PhsArgs<T>));
+		memcpy((char *)L->phsStage.pinned + ab, sstart.data(), sstart.size() * sizeof(int));
+		fetchArgs(L->phsStage.dev, L->phsStage.pinned, bytes, L->st, L->phsStage.done);
+		k_phs_sample_batch<T><<<sblocks, 256, 0, L->st>>>((const PhsArgs<T> *)L->phsStage.dev, (const int *)((char *)L->phsStage.dev + ab), (int)sa.size());
+		HIPCHECK(hipGetLastError());
+	}
+	if (full.empty()) return 0;
+	std::vector<PhsFlush<T>> fa;
+	std::vector<int> fstart;
+	int fblocks = 0, ny = 1;
+	size_t exBytes = 0;
+	for (int e : full) {
+		const DevPhaser &P = D[e]->phs[id[e]];
+		exBytes += (size_t)P.pend * 4 * P.pnq * sizeof(double);
+	}
+	const size_t ab = (full.size() * sizeof(PhsFlush<T>) + 255) / 256 * 256;
+	const size_t sb = (full.size() * sizeof(int) + 255) / 256 * 256, bytes = ab + sb + exBytes;
+	if (stagingFor(L->phfStage, bytes, L->st)) return -1;
+	size_t exOff = ab + sb;
+	for (int e : full) {
+		DevPhaser &P = D[e]->phs[id[e]];
+		PhsFlush<T> F;
+		F.a = phaserArgs<T>(D[e], P, P.pq0, P.pnq);
+		F.a.nq = P.pnq;
+		F.ring = (const T *)P.ring;
+		F.ex = (const double *)((char *)L->phfStage.dev + exOff);
+		F.ns = P.pend, F.stride = 4 * P.pnq;
+		const size_t xb = (size_t)P.pend * 4 * P.pnq * sizeof(double);
+		memcpy((char *)L->phfStage.pinned + exOff, P.exPinned, xb);
+		exOff += xb;
+		fa.push_back(F);
+		fstart.push_back(fblocks);
+		fblocks += (int)blocksFor(P.ncell);
+		ny = std::max(ny, (P.pnq + PHS_QPT - 1) / PHS_QPT);
+		P.pend = 0;
+	}
+	memcpy(L->phfStage.pinned, fa.data(), fa.size() * sizeof(PhsFlush<T>));
+	memcpy((char *)L->phfStage.pinned + ab, fstart.data(), fstart.size() * sizeof(int));
+	fetchArgs(L->phfStage.dev, L->phfStage.pinned, bytes, L->st, L->phfStage.done);
+	const long nb = (fblocks + 7) / 8 * 8;
+	const PhsFlush<T> *fargs = (const PhsFlush<T> *)L->phfStage.dev;
+	const int *fst = (const int *)((char *)L->phfStage.dev + ab);
+	k_phs_flush_batch<T, TA><<<nb * ny, 256, 0, L->st>>>(fargs, fst, (int)fa.size(), fblocks, ny);
 	HIPCHECK(hipGetLastError());
 	return 0;
 }
 
-// The arguments of one update launch of phaser P (frequencies qa .. qa+qn-1, qn <= MAXF).
+// The launch arguments of phaser P for frequencies qa .. qa+qn-1.
 template<class T>
-PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *expE, const double *expH)
+PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn)
 {
 	PhsArgs<T> a{};
 	for (int c = 0; c < 3; c++) a.E[c] = (const T *)D->F[0][0][c], a.H[c] = (const T *)D->F[1][0][c];
@@ -1766,10 +1822,6 @@ PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn, const dou
 	a.ncell = P.ncell;
 	a.acc = P.acc;
 	a.q0 = qa, a.nq = qn;
-	for (int q = 0; q < qn; q++) {
-		a.eE[q][0] = expE[2 * q], a.eE[q][1] = expE[2 * q + 1];
-		a.eH[q][0] = expH[2 * q], a.eH[q][1] = expH[2 * q + 1];
-	}
 	return a;
 }
 
@@ -2032,11 +2084,13 @@ void alis_dev_release(alis_dev_world *D)
 	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
-	if (D->phsPinned) hipHostFree(D->phsPinned);
+	for (Staging *S : {&D->phsStage, &D->phfStage}) {
+		if (S->pinned) hipHostFree(S->pinned);
+		if (S->dev) hipFree(S->dev);
+		if (S->done) hipEventDestroy(S->done);
+	}
 	if (D->gatherDev) hipFree(D->gatherDev);
 	if (D->gatherPinned) hipHostFree(D->gatherPinned);
-	if (D->phsDev) hipFree(D->phsDev);
-	if (D->phsDone) hipEventDestroy(D->phsDone);
 	if (D->blas) rocblas_destroy_handle(D->blas);
 	if (D->st && --*D->streamRefs == 0) {
 		hipStreamDestroy(D->st);
@@ -2258,20 +2312,10 @@ int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const i
 			return fail("phasers updated together must share their field and accumulator precision");
 	}
 	for (int e = 1; e < n; e++) if (joinStream(D[e], D[0])) return -1;
-	for (int e = 0; e < n; e++) if (phaserFlushAny(D[e], D[e]->phs[id[e]])) return -1;
-	std::vector<PhsArgs<float>> af;
-	std::vector<PhsArgs<double>> ad;
-	for (int e = 0; e < n; e++) {
-		DevPhaser &P = D[e]->phs[id[e]];
-		if (!P.ncell) continue;
-		for (int c = 0; c < nq[e]; c += MAXF) {
-			const int qn = std::min(MAXF, nq[e] - c);
-			if (D[0]->fp64) ad.push_back(phaserArgs<double>(D[e], P, q0[e] + c, qn, expE[e] + 2 * c, expH[e] + 2 * c));
-			else af.push_back(phaserArgs<float>(D[e], P, q0[e] + c, qn, expE[e] + 2 * c, expH[e] + 2 * c));
-		}
-	}
-	if (D[0]->fp64) return D[0]->dft64 ? phaserBatchUpdate<double, double>(D[0], ad) : phaserBatchUpdate<double, float>(D[0], ad);
-	return D[0]->dft64 ? phaserBatchUpdate<float, double>(D[0], af) : phaserBatchUpdate<float, float>(D[0], af);
+	if (D[0]->fp64) return D[0]->dft64 ? phaserBatchUpdate<double, double>(D[0], D, id, q0, nq, expE, expH, n)
+	                                   : phaserBatchUpdate<double, float>(D[0], D, id, q0, nq, expE, expH, n);
+	return D[0]->dft64 ? phaserBatchUpdate<float, double>(D[0], D, id, q0, nq, expE, expH, n)
+	                   : phaserBatchUpdate<float, float>(D[0], D, id, q0, nq, expE, expH, n);
 }
 
 long alis_dev_phaser_size(alis_dev_world *D, int id)
