@@ -456,7 +456,7 @@ __device__ inline void phaserBlock(int w, int ny, int &xb, int &yb)
 
 // Deferred phaser updates: a sample stores the face fields of one step (ring slot, [4][cell]);
 // a flush adds the pending samples to the accumulators in sample order, the same additions as
-// one k_phaser launch per sample, reading and writing the accumulators once for all of them.
+// one immediate update per sample, reading and writing the accumulators once for all of them.
 template<class T>
 __global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict__ slot)
 {
