@@ -9,6 +9,7 @@ oracle.
   spectra agree with the float fixture to 1e-5 and with the FP64 oracle to 1e-5.
 * FP64 worlds: the final device state agrees with the FP64 oracle to 1e-12 relative.
 * The closed-form dispersion recurrence of a periodic vacuum box, FP64 on the device.
+* Phaser reads between deferred flushes (13 steps), alone and batched: bitwise the oracle.
 * Full 256^3 nanoblock size: the GPU's state after 300 steps has the same checksum as the
   reference's own C code run on the host (oracle/_ref/wl_ref), and FP64 tracks FP32.
 """
@@ -135,3 +136,19 @@ def test_batch_equals_alone(precision):
         if precision == "ref":
             bad = compare(golden(n), mix[n])
             assert not bad, n + ": " + "\n".join(bad[:10])
+
+
+@pytest.mark.parametrize("name", WITH_PHASERS)
+def test_phaser_read_between_flushes(name):
+    """Phaser updates reach the accumulators eight samples at a time; a read after 13 steps
+    (five samples pending) returns the running sums of every update: in float accumulation
+    the same bits as the oracle's one update per step, alone and in a batch (whose phasers
+    are updated together)."""
+    print(f"[gpu] {name}: phaser read after 13 steps", flush=True)
+    fg, pg = run_raw("hip", name, 13, "ref")
+    fo, po = run_raw("oracle", name, 13, "ref")
+    assert pg.size and np.array_equal(pg, po) and np.array_equal(fg, fo)
+    names = sorted(WITH_PHASERS)
+    mix = _split(run_scenario("hip", ",".join(names), 13, "ref"), names)
+    bad = compare(run_scenario("oracle", name, 13, "ref"), mix[name])
+    assert not bad, "\n".join(bad[:10])
