@@ -1856,13 +1856,16 @@ struct alis_dev_batch {
 	struct Round {
 		std::vector<int> world;     // index into W
 		std::vector<int> seg;       // segment of that world
-		void *args = nullptr;       // PostArgs<T>[] (device)
-		void *pinned = nullptr;     // staging (host, pinned)
-		hipEvent_t done = nullptr;  // staging copy finished
-		void *start = nullptr;      // long[] workgroup starts (device)
+		void *args = nullptr;       // PostArgs<T>[] (device, in postArgs)
+		void *pinned = nullptr;     // its staging (in postPinned)
+		void *start = nullptr;      // int[] workgroup starts (device)
 		long blocks = 0;
 	};
 	std::vector<Round> rounds[2];
+	// every round's arguments of a half step: staged together, fetched in one copy
+	void *postArgs[2] = {}, *postPinned[2] = {};
+	size_t postBytes[2] = {};
+	hipEvent_t postDone[2] = {};    // the last copy from postPinned has been made
 	std::vector<void *> allocs;
 };
 
@@ -1930,13 +1933,22 @@ int batchBuildPost(alis_dev_batch *B)
 				R.blocks += D->post[h][r].total / 256;
 			}
 			if (R.world.empty()) continue;
-			const size_t bytes = R.world.size() * sizeof(PostArgs<T>);
-			if (batchAlloc(B, &R.args, bytes)) return -1;
 			if (batchAlloc(B, &R.start, start.size() * sizeof(int))) return -1;
-			HIPCHECK(hipHostMalloc(&R.pinned, bytes, hipHostMallocDefault));
-			HIPCHECK(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
-			HIPCHECK(hipEventRecord(R.done, B->st));
 			HIPCHECK(hipMemcpyAsync(R.start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, B->st));
+		}
+		size_t bytes = 0;
+		for (const alis_dev_batch::Round &R : B->rounds[h]) bytes += (R.world.size() * sizeof(PostArgs<T>) + 255) / 256 * 256;
+		if (!bytes) continue;
+		if (batchAlloc(B, &B->postArgs[h], bytes)) return -1;
+		HIPCHECK(hipHostMalloc(&B->postPinned[h], bytes, hipHostMallocDefault));
+		HIPCHECK(hipEventCreateWithFlags(&B->postDone[h], hipEventDisableTiming));
+		HIPCHECK(hipEventRecord(B->postDone[h], B->st));
+		B->postBytes[h] = bytes;
+		size_t off = 0;
+		for (alis_dev_batch::Round &R : B->rounds[h]) {
+			if (R.world.empty()) continue;
+			R.args = (char *)B->postArgs[h] + off, R.pinned = (char *)B->postPinned[h] + off;
+			off += (R.world.size() * sizeof(PostArgs<T>) + 255) / 256 * 256;
 		}
 	}
 	HIPCHECK(hipStreamSynchronize(B->st));
@@ -1976,9 +1988,10 @@ template<class T>
 int batchPost(alis_dev_batch *B, int h)
 {
 	if (!B->postReady && batchBuildPost<T>(B)) return -1;
+	if (!B->postBytes[h]) return 0;
+	HIPCHECK(hipEventSynchronize(B->postDone[h]));
 	for (alis_dev_batch::Round &R : B->rounds[h]) {
 		if (R.world.empty()) continue;
-		HIPCHECK(hipEventSynchronize(R.done));
 		PostArgs<T> *a = (PostArgs<T> *)R.pinned;
 		for (size_t e = 0; e < R.world.size(); e++) {
 			alis_dev_world *D = B->W[R.world[e]];
@@ -1991,7 +2004,10 @@ int batchPost(alis_dev_batch *B, int h)
 			for (size_t w = 0; w < D->pendSrc[h].size(); w++) a[e].src[w] = D->pendSrc[h][w];
 			for (size_t q = 0; q < D->pendWave[h].size(); q++) a[e].wave[q] = (T)D->pendWave[h][q];
 		}
-		fetchArgs(R.args, R.pinned, R.world.size() * sizeof(PostArgs<T>), B->st, R.done);
+	}
+	fetchArgs(B->postArgs[h], B->postPinned[h], B->postBytes[h], B->st, B->postDone[h]);
+	for (alis_dev_batch::Round &R : B->rounds[h]) {
+		if (R.world.empty()) continue;
 		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
 		HIPCHECK(hipGetLastError());
 	}
@@ -2718,11 +2734,10 @@ void alis_dev_batch_release(alis_dev_batch *B)
 	if (!B) return;
 	for (alis_dev_world *W : B->W) hipStreamSynchronize(W->st);   // the batch's launches ran on one of these
 	for (void *p : B->allocs) hipFree(p);
-	for (int h = 0; h < 2; h++)
-		for (alis_dev_batch::Round &R : B->rounds[h]) {
-			if (R.pinned) hipHostFree(R.pinned);
-			if (R.done) hipEventDestroy(R.done);
-		}
+	for (int h = 0; h < 2; h++) {
+		if (B->postPinned[h]) hipHostFree(B->postPinned[h]);
+		if (B->postDone[h]) hipEventDestroy(B->postDone[h]);
+	}
 	delete B;
 }
 
