@@ -640,6 +640,54 @@ __global__ __launch_bounds__(256) void k_acc_to_double(const TA *src, double *ds
 
 inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// Inner sums of the net flux Re(E x H*) of one frequency over the phaser faces
+// (src/phaser.c:142-223).  Term t of the table (FLUX_TERM ints: kind, first cell, rows na, row
+// length nb, components x and y, weight offset, output offset, first workgroup) sums
+// w * Re(x conj(y)) over a face stored row by row (cell a * nb + b):
+//  * kind 0 along each row: a wavefront per row, lane l adds b = l, l + 64, ... in order and
+//    writes its partial sum to out[a * 64 + l] (the host adds the 64 in order);
+//  * kind 1 down each column: a thread per column b adds a = 0 .. na - 1 in order.
+// Both read every row with consecutive lanes on consecutive cells and use no cross-lane
+// operation.  The accumulators are read once: about 2 x 16 bytes per face cell and term.
+constexpr int FLUX_TERM = 9;
+
+template<class TA>
+__global__ __launch_bounds__(256) void k_flux_sums(const typename Pair<TA>::V *__restrict__ acc, long ncell,
+                                                   const int *__restrict__ terms, int nterm,
+                                                   const double *__restrict__ w, double *__restrict__ out)
+{
+	typedef typename Pair<TA>::V V;
+	const int b = (int)blockIdx.x;
+	int t = 0;
+	while (t + 1 < nterm && b >= terms[FLUX_TERM * (t + 1) + 8]) t++;
+	const int *T = terms + FLUX_TERM * t;
+	const int na = T[2], nb = T[3], lb = b - T[8];
+	const V *X = acc + (long)T[4] * ncell + T[1], *Y = acc + (long)T[5] * ncell + T[1];
+	const double *wt = w + T[6];
+	double *o = out + T[7];
+	if (T[0] == 0) {
+		const int a = lb * 4 + (int)threadIdx.x / 64, l = (int)threadIdx.x % 64;
+		if (a >= na) return;
+		double r = 0;
+		for (int c = l; c < nb; c += 64) {
+			const V x = X[(long)a * nb + c], y = Y[(long)a * nb + c];
+			r += wt[c] * ((double)x[0] * (double)y[0] + (double)x[1] * (double)y[1]);
+		}
+		o[(long)a * 64 + l] = r;
+	}
+	else {
+		const int c = lb * 256 + (int)threadIdx.x;
+		if (c >= nb) return;
+		double r = 0;
+		for (int a = 0; a < na; a++) {
+			const V x = X[(long)a * nb + c], y = Y[(long)a * nb + c];
+			r += wt[a] * ((double)x[0] * (double)y[0] + (double)x[1] * (double)y[1]);
+		}
+		o[c] = r;
+	}
+}
+
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -2365,6 +2413,40 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 	else k_acc_to_double<float><<<blocksFor(need), 256, 0, D->st>>>((const float *)P.acc, (double *)D->scratch, need);
 	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipMemcpyAsync(out, D->scratch, need * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                              const double *w, long nw, double *out, long nout)
+{
+	if (alis_dev_phaser_size(D, id) < 0) return fail("no such phaser");
+	DevPhaser &P = D->phs[id];
+	if (q < 0 || q >= P.nf || nterm < 1 || nterm > 12 || nw < 1 || nout < 1) return fail("flux sums: bad frequency, term count or sizes");
+	std::vector<int> t((size_t)FLUX_TERM * nterm);
+	long blocks = 0;
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms + 8 * n;
+		const long na = u[2], nb = u[3], len = u[0] == 0 ? na * 64 : nb, wl = u[0] == 0 ? nb : na;
+		if ((u[0] != 0 && u[0] != 1) || na < 1 || nb < 1 || u[1] < 0 || u[1] + na * nb > P.ncell || u[4] < 0 || u[4] > 3
+		    || u[5] < 0 || u[5] > 3 || u[6] < 0 || u[6] + wl > nw || u[7] < 0 || u[7] + len > nout)
+			return fail("flux sums: term outside the phaser faces or the buffers");
+		std::copy(u, u + 8, t.begin() + (size_t)FLUX_TERM * n);
+		t[(size_t)FLUX_TERM * n + 8] = (int)blocks;
+		blocks += u[0] == 0 ? (na + 3) / 4 : (nb + 255) / 256;
+	}
+	if (phaserFlushAny(D, P)) return -1;
+	const long tab = ((long)t.size() + 1) / 2;     // doubles that hold the term table
+	if (ensureScratch(D, nout + nw + tab)) return -1;
+	double *dOut = (double *)D->scratch, *dW = dOut + nout;
+	int *dT = (int *)(dW + nw);
+	HIPCHECK(hipMemcpyAsync(dW, w, nw * sizeof(double), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemcpyAsync(dT, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
+	const long base = (long)q * 4 * P.ncell;
+	if (D->dft64) k_flux_sums<double><<<(unsigned)blocks, 256, 0, D->st>>>((const Pair<double>::V *)P.acc + base, P.ncell, dT, nterm, dW, dOut);
+	else k_flux_sums<float><<<(unsigned)blocks, 256, 0, D->st>>>((const Pair<float>::V *)P.acc + base, P.ncell, dT, nterm, dW, dOut);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, dOut, nout * sizeof(double), hipMemcpyDeviceToHost, D->st));
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
 }

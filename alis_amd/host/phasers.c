@@ -325,6 +325,76 @@ static double faceFlux(phaser P, const double *v, const phaserGeom *g, int q, in
 	return s + c;
 }
 
+/* Trapezoid weights of the E nodes lo..hi along one axis (half weight at both ends). */
+static void trapezoid(double *w, const float *d, int lo, int hi)
+{
+	for (int n = lo; n <= hi; n++) w[n - lo] = (n == lo || n == hi) ? 0.5 * d[n] : d[n];
+}
+
+static double rowSum(const double *rows, int a)
+{
+	double r = 0;
+	for (int l = 0; l < 64; l++) r += rows[(long)a * 64 + l];
+	return r;
+}
+
+/* faceFlux with the inner trapezoid sums computed where the accumulators live
+ * (alis_dev_phaser_flux_sums): each face is one row term (inner axis = the face's fast axis)
+ * and one column term; rows holds 64 partial sums per row, cols one sum per column. */
+static double faceFluxSums(phaser P, int kind, const double *rows, const double *cols)
+{
+	world W = P->W;
+	double s = 0, c = 0;
+	if (kind < 2) {          /* a = j, b = k: E1 x H2 along k (rows), E2 x H1 along j (columns) */
+		for (int j = P->jMin+1; j <= P->jMax; j++) neumaierAdd(&s, &c, W->CH->dy[j] * rowSum(rows, j - P->jMin));
+		for (int k = P->kMin+1; k <= P->kMax; k++) neumaierAdd(&s, &c, W->CH->dz[k] * -cols[k - P->kMin]);
+	}
+	else if (kind < 4) {     /* a = i, b = k: E1 x H2 along i (columns), E2 x H1 along k (rows) */
+		for (int k = P->kMin+1; k <= P->kMax; k++) neumaierAdd(&s, &c, W->CH->dz[k] * cols[k - P->kMin]);
+		for (int i = P->iMin+1; i <= P->iMax; i++) neumaierAdd(&s, &c, W->CH->dx[i] * -rowSum(rows, i - P->iMin));
+	}
+	else {                   /* a = i, b = j: E1 x H2 along j (rows), E2 x H1 along i (columns) */
+		for (int i = P->iMin+1; i <= P->iMax; i++) neumaierAdd(&s, &c, W->CH->dx[i] * rowSum(rows, i - P->iMin));
+		for (int j = P->jMin+1; j <= P->jMax; j++) neumaierAdd(&s, &c, W->CH->dy[j] * -cols[j - P->jMin]);
+	}
+	return s + c;
+}
+
+/* The six faces' fluxes of P from the device sums: one call per frequency, which reads that
+ * frequency's accumulators once and returns about 64 + 1 doubles per face row. */
+static void faceFluxesDev(phaser P, phaser B, const phaserGeom *g, int q, double *face)
+{
+	world W = P->W;
+	const int ni = B->iMax-B->iMin+1, nj = B->jMax-B->jMin+1, nk = B->kMax-B->kMin+1;
+	double *w = malloc((size_t)(ni + nj + nk) * sizeof(double));
+	const int wx = 0, wy = ni, wz = ni + nj;
+	trapezoid(w + wx, W->CE->dx, B->iMin, B->iMax);
+	trapezoid(w + wy, W->CE->dy, B->jMin, B->jMax);
+	trapezoid(w + wz, W->CE->dz, B->kMin, B->kMax);
+	int terms[12][8], nterm = 0;
+	long rowOff[6] = {0}, colOff[6] = {0}, nout = 0;
+	for (int kind = 0; kind < 6; kind++) {
+		face[kind] = 0;
+		if (!g->on[kind]) continue;
+		const int na = kind < 4 ? (kind < 2 ? nj : ni) : ni, nb = g->n2[kind], off = (int)g->off[kind];
+		/* row term: E1 x H2 on x and z faces, E2 x H1 on y faces; column term the other */
+		const int rx = kind < 2 || kind >= 4 ? 0 : 1, ry = kind < 2 || kind >= 4 ? 3 : 2;
+		const int rw = kind < 4 ? wz : wy, cw = kind < 2 ? wy : wx;
+		rowOff[kind] = nout, nout += (long)na * 64;
+		colOff[kind] = nout, nout += nb;
+		int *r = terms[nterm++], *t = terms[nterm++];
+		r[0] = 0, r[1] = off, r[2] = na, r[3] = nb, r[4] = rx, r[5] = ry, r[6] = rw, r[7] = (int)rowOff[kind];
+		t[0] = 1, t[1] = off, t[2] = na, t[3] = nb, t[4] = rx ? 0 : 1, t[5] = rx ? 3 : 2, t[6] = cw, t[7] = (int)colOff[kind];
+	}
+	if (!nterm) { free(w); return; }
+	double *out = malloc((size_t)nout * sizeof(double));
+	alisFatal("phaser flux", alis_dev_phaser_flux_sums(W->dev, B->devId, q, nterm, &terms[0][0], w, ni + nj + nk, out, nout));
+	for (int kind = 0; kind < 6; kind++)
+		if (g->on[kind]) face[kind] = faceFluxSums(P, kind, out + rowOff[kind], out + colOff[kind]);
+	free(out);
+	free(w);
+}
+
 /* Net outward flux of phaser P normalised by A^2; 2-D worlds divide by the unit length of
  * the flat axis, symmetry planes double the result (src/phaser.c:228-245). */
 double phaserFluxD(phaser P)
@@ -332,16 +402,24 @@ double phaserFluxD(phaser P)
 	phaser B = phaserBase(P);
 	if (!B) return 0;
 	phaserGeom g;
-	const double *v = phaserDoubles(B, &g);
 	int q = (int)(P - B);
 	world W = P->W;
+	double face[6];
+	if (B->devId >= 0 && W->dev) {
+		phaserFaces(B, &g);
+		faceFluxesDev(P, B, &g, q, face);
+	}
+	else {
+		const double *v = phaserDoubles(B, &g);
+		for (int kind = 0; kind < 6; kind++) face[kind] = faceFlux(P, v, &g, q, kind);
+	}
 	double s = 0, c = 0;
-	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 1));
-	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 0));
-	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 3));
-	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 2));
-	neumaierAdd(&s, &c, faceFlux(P, v, &g, q, 5));
-	neumaierAdd(&s, &c, -faceFlux(P, v, &g, q, 4));
+	neumaierAdd(&s, &c, face[1]);
+	neumaierAdd(&s, &c, -face[0]);
+	neumaierAdd(&s, &c, face[3]);
+	neumaierAdd(&s, &c, -face[2]);
+	neumaierAdd(&s, &c, face[5]);
+	neumaierAdd(&s, &c, -face[4]);
 	double Sum = s + c;
 	if (W->Dom.x.Min==W->Dom.x.Max) Sum /= W->CH->dx[P->iMax]+W->CH->dx[P->iMax-1];
 	if (W->Dom.y.Min==W->Dom.y.Max) Sum /= W->CH->dy[P->jMax]+W->CH->dy[P->jMax-1];

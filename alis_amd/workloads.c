@@ -222,6 +222,17 @@ int wl_phaser_info(void *ctx, long *surfaceCells)
 	return n;
 }
 
+/* Net outward flux of every frequency of the workload's phaser (phaserPoyntingOut,
+ * src/phaser.c:228-245) into out[0..n-1]; returns the frequencies written. */
+int wl_flux(void *ctx, double *out, int n)
+{
+	wl *c = ctx;
+	int m = 0;
+	if (!c->P) return 0;
+	while (m < n && c->P[m].f) out[m] = phaserPoyntingOut(c->P + m), m++;
+	return m;
+}
+
 /* Grid nodes and CPML layer thickness (Num+1 cells) per face, 0 where the face is not PML;
  * returns 1 when the E update reads 1/eps (objects were put, src/update.c:109 vs :114). */
 int wl_geometry(void *ctx, int *num, int *pml)

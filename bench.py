@@ -20,6 +20,7 @@ every curl launch of the timed region.  `cpu_baseline` times the reference's own
 import argparse
 import ctypes
 import json
+import math
 import os
 import subprocess
 import sys
@@ -60,6 +61,7 @@ def load():
     w.wl_phaser_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
     w.wl_geometry.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     w.wl_device.restype = ctypes.c_void_p
+    w.wl_flux.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     w.wl_device.argtypes = [ctypes.c_void_p]
     w.wl_free.argtypes = [ctypes.c_void_p]
     w.wl_checksum.restype = ctypes.c_double
@@ -97,7 +99,7 @@ def curl_bytes(num, pml, s, has_eps):
     return e + slab * 4 * s, h + slab * 4 * s
 
 
-def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None):
+def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None, flux=False):
     ctx = w.wl_create(name.encode(), cells, fbits, dbits)
     if not ctx:
         sys.exit(f"bench: unknown workload {name}")
@@ -126,6 +128,17 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None):
            "eps": eps, "eps_bytes": alis.alis_dev_eps_bytes(dev), "nf": nf, "surf": surf.value,
            "hbm_bytes": alis.alis_dev_bytes(dev),
            "worlds": nw, "parts": 1 + cplx.value}
+    if flux:
+        # net flux of every frequency (phaserPoyntingOut): one k_flux_sums launch per frequency
+        # reads that frequency's face accumulators; the host adds the returned row sums
+        vals = (ctypes.c_double * 64)()
+        w.wl_flux(ctx, vals, 1)                          # flushes the pending phaser samples
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        n = w.wl_flux(ctx, vals, 64)
+        out["flux"] = {"frequencies": n, "ms_per_frequency": round((time.perf_counter() - t0) / max(n, 1) * 1e3, 4),
+                       "accumulator_bytes_per_frequency": surf.value * 4 * 2 * (8 if dbits == 64 else 4),
+                       "finite": all(math.isfinite(vals[q]) for q in range(n))}
     if farfield:
         import tempfile
         with tempfile.TemporaryDirectory() as d:
@@ -295,7 +308,7 @@ def main():
         print(json.dumps({"section": args.only, **out}), flush=True)
         return
     with quiet():
-        r32 = run(alis, w, hip, "nanoblock", args.cells, 32, 64, args.warmup, args.steps)
+        r32 = run(alis, w, hip, "nanoblock", args.cells, 32, 64, args.warmup, args.steps, flux=True)
     value = r32["cells"] * args.steps / r32["sec"] / 1e6
     rl = roofline(r32, 4, 8, args.steps)
     rl["traffic"] = measured_traffic(r32["cells"], "fp32")
@@ -309,6 +322,7 @@ def main():
                                f"60x60x40 nm block on n=1.5 substrate, TF/SF 400-900 nm band, 32-frequency phaser",
                    "cells": r32["cells"], "fields": "fp32", "dft": "fp64", "hbm_resident_bytes": r32["hbm_bytes"]},
         "roofline": rl,
+        "flux": r32["flux"],
     }
     if not args.no_fp64:
         with quiet():

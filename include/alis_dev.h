@@ -148,6 +148,18 @@ int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const i
 int alis_dev_phaser_sync(alis_dev_world *D, int id);
 long alis_dev_phaser_size(alis_dev_world *D, int id);
 int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
+/* Inner sums of the net flux of frequency q of phaser id (src/phaser.c:142-223 — the
+ * trapezoid loops of phaserPoyntingX/Y/Z; phaserPoyntingOut, src/phaser.c:228-245, adds them
+ * with the outer weights on the host).  nterm (1..12) terms of 8 ints each: kind (0: sums
+ * along the rows of a face, 64 in-order partial sums per row, out[o + a*64 + l] for lane l
+ * adding cells b = l, l+64, ...; 1: one sum per column, out[o + b], adding rows a = 0.. in
+ * order), the face's first cell, rows na, row length nb (cell a*nb + b), the components x and
+ * y (0..3, the device read layout), the offset of the weights in w (per row cell b for kind
+ * 0, per row a for kind 1) and the output offset o.  Each sum adds
+ * w * (Re x Re y + Im x Im y) in double.  Returns -1 for a term outside the faces or the
+ * buffers. */
+int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                              const double *w, long nw, double *out, long nout);
 int alis_dev_phaser_remove(alis_dev_world *D, int id);
 
 /* Near-to-far projection, the surface sums (src/farfld.c:248-459).  For each of n

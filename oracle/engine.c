@@ -464,6 +464,40 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 	return 0;
 }
 
+/* The flux's inner trapezoid sums (src/phaser.c:142-223) in the term form of the device
+ * entry: each output adds its cells in the stated order, so the two agree to the bit. */
+int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                              const double *w, long nw, double *out, long nout)
+{
+	phs *Q = getPhaser(D, id);
+	if (!Q) return fail("no such phaser");
+	if (q < 0 || q >= Q->nf || nterm < 1 || nterm > 12 || nw < 1 || nout < 1) return fail("flux sums: bad frequency, term count or sizes");
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms + 8 * n;
+		const long na = u[2], nb = u[3], len = u[0] == 0 ? na * 64 : nb, wl = u[0] == 0 ? nb : na;
+		if ((u[0] != 0 && u[0] != 1) || na < 1 || nb < 1 || u[1] < 0 || u[1] + na * nb > Q->ncell || u[4] < 0 || u[4] > 3
+		    || u[5] < 0 || u[5] > 3 || u[6] < 0 || u[6] + wl > nw || u[7] < 0 || u[7] + len > nout)
+			return fail("flux sums: term outside the phaser faces or the buffers");
+	}
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms + 8 * n;
+		const long na = u[2], nb = u[3];
+		const long bx = 2 * (((long)q * 4 + u[4]) * Q->ncell + u[1]), by = 2 * (((long)q * 4 + u[5]) * Q->ncell + u[1]);
+		double *o = out + u[7];
+		if (u[0] == 0) for (long c = 0; c < na * 64; c++) o[c] = 0;
+		else for (long c = 0; c < nb; c++) o[c] = 0;
+		for (long a = 0; a < na; a++)
+			for (long b = 0; b < nb; b++) {
+				const long px = bx + 2 * (a * nb + b), py = by + 2 * (a * nb + b);
+				const double xr = Q->dft64 ? Q->accD[px] : Q->accF[px], xi = Q->dft64 ? Q->accD[px + 1] : Q->accF[px + 1];
+				const double yr = Q->dft64 ? Q->accD[py] : Q->accF[py], yi = Q->dft64 ? Q->accD[py + 1] : Q->accF[py + 1];
+				if (u[0] == 0) o[a * 64 + b % 64] += w[u[6] + b] * (xr * yr + xi * yi);
+				else o[b] += w[u[6] + a] * (xr * yr + xi * yi);
+			}
+	}
+	return 0;
+}
+
 int alis_dev_phaser_remove(alis_dev_world *D, int id)
 {
 	phs *Q = getPhaser(D, id);

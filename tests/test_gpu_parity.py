@@ -138,6 +138,21 @@ def test_batch_equals_alone(precision):
             assert not bad, n + ": " + "\n".join(bad[:10])
 
 
+@pytest.mark.parametrize("prec", ["ref", "fp32"])
+def test_flux_device_sums_bitwise(prec):
+    """The net flux (phaserPoyntingOut, src/phaser.c:228-245) takes its inner trapezoid sums
+    from k_flux_sums, which adds the cells in the order the oracle's restatement does: every
+    phaser's flux of every phaser scenario equals the oracle's to the bit (the float the API
+    returns, printed to round-trip), after 30 steps (six samples pending in the ring)."""
+    for name in sorted(WITH_PHASERS):
+        print(f"[gpu] {name} {prec}: flux sums", flush=True)
+        g, o = run_scenario("hip", name, 30, prec), run_scenario("oracle", name, 30, prec)
+        keys = [k for k in o if k.endswith(".flux")]
+        assert keys and all(k in g for k in keys)
+        assert all(g[k] == o[k] for k in keys), [(k, g[k], o[k]) for k in keys if g[k] != o[k]][:4]
+        assert any(o[k] != 0 for k in keys)
+
+
 @pytest.mark.parametrize("name", WITH_PHASERS)
 def test_phaser_read_between_flushes(name):
     """Phaser updates reach the accumulators eight samples at a time; a read after 13 steps
