@@ -143,7 +143,7 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // dealt out per XCD (see xcdTile), otherwise grid (k tiles, y tiles, x planes).
 __device__ inline int xcdTile(int wid, int nwg);
 
-template<class T, int CM, bool NT = true, bool NTG = false>
+template<class T, int CM, bool NT = true, bool NTG = false, bool MZ = false>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k);
 
 template<class T, int CM, bool NT = true, bool NTG = false, int BY = 4>
@@ -160,10 +160,10 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 // neighbour values, and the psi slabs go through the cache (the short rows of small worlds
 // cover a slab with partial lines that neighbouring tiles complete).  Measured on 64 k-point
 // worlds: 5 % (E) and 14 % (H) faster than reloading and streaming; slower in k_curl.
-template<class T, int CM, bool NT, bool NTG>
+template<class T, int CM, bool NT, bool NTG, bool MZ>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
-	constexpr bool SHARE = true, NTP = false;
+	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ;
 #include "curl_cell.inc"
 }
 
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
-	constexpr bool SHARE = false, NTP = NT;
+	constexpr bool SHARE = false, NTP = NT, MERGEZ = false;
 #include "curl_cell.inc"
 }
 
@@ -225,8 +225,8 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 			const int off = (t % per) * 256 + lid, jj = off / pitch - 1, kk = off - (jj + 1) * pitch;
 			if (jj < 0 || jj >= a.nj || kk < 1 || kk > a.nk) continue;
 			const int i = a.iMIN + t / per, j = a.jMIN + jj, k = a.kMIN - 1 + kk;
-			if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1>(a, i, j, k);
-			else curlAt<T, 0>(a, i, j, k);
+			if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true>(a, i, j, k);
+			else curlAt<T, 0, true, false, true>(a, i, j, k);
 		}
 		return;
 	}
