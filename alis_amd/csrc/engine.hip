@@ -23,6 +23,7 @@
 #include <cmath>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <rocblas/rocblas.h>
 
 #include "alis_dev.h"
@@ -640,6 +641,15 @@ __global__ __launch_bounds__(256) void k_acc_to_double(const TA *src, double *ds
 
 inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// A fill kernel for buffers that must start from a value (used instead of a memset node,
+// which callers capturing the work into a graph should not meet).
+template<class T>
+__global__ __launch_bounds__(256) void k_fill(T *p, long n, T v)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) p[t] = v;
+}
+
 // Inner sums of the net flux Re(E x H*) of one frequency over the phaser faces
 // (src/phaser.c:142-223).  Term t of the table (FLUX_TERM ints: kind, first cell, rows na, row
 // length nb, components x and y, weight offset, output offset, first workgroup) sums
@@ -650,6 +660,10 @@ inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 
 // Both read every row with consecutive lanes on consecutive cells and use no cross-lane
 // operation.  The accumulators are read once: about 2 x 16 bytes per face cell and term.
 constexpr int FLUX_TERM = 9;
+
+// Doubles of the file-scope `scratch` array that alis_dev_phaser_flux_sums_noalloc uses for
+// its outputs, weights and term table (4 MiB per GPU, held until the process ends).
+constexpr long FLUX_SCRATCH_DOUBLES = 1L << 19;
 
 template<class TA>
 __global__ __launch_bounds__(256) void k_flux_sums(const typename Pair<TA>::V *__restrict__ acc, long ncell,
@@ -689,6 +703,11 @@ __global__ __launch_bounds__(256) void k_flux_sums(const typename Pair<TA>::V *_
 
 
 }  // namespace
+
+// Scratch memory of alis_dev_phaser_flux_sums_noalloc, for callers that cannot allocate:
+// one copy per GPU, shared by every such call on that GPU (calls that overlap on one GPU
+// race).  Host code reaches it only through the address that hipGetSymbolAddress returns.
+static __device__ __align__(256) double scratch[FLUX_SCRATCH_DOUBLES];
 
 // ---------------------------------------------------------------------------------------
 // Device world
@@ -771,6 +790,7 @@ struct alis_dev_world {
 	std::vector<alis_dev_src> srcDesc[2];
 	std::vector<alis_dev_tfsf> tfDesc;
 	int fp64 = 0, dft64 = 0, cplx = 0;
+	int device = 0;             // the GPU that holds the world (current at commit)
 	size_t esz = 4;
 	int iMIN, iMAX, jMIN, jMAX, kMIN, kMAX, iMin, iMax, jMin, jMax, kMin, kMax, iNUM, jNUM, kNUM;
 	Lay L{};
@@ -2104,6 +2124,10 @@ int alis_dev_commit(const alis_dev_world_desc *W, int fp64, int dft64, alis_dev_
 	}
 	D->tfDesc.assign(W->tfsf, W->tfsf + W->ntfsf);
 	D->W.tfsf = D->tfDesc.data();
+	if (hipGetDevice(&D->device) != hipSuccess) {
+		delete D;
+		return fail("hipGetDevice failed");
+	}
 	D->fp64 = fp64 ? 1 : 0, D->dft64 = dft64 ? 1 : 0;
 	D->cplx = W->complexField ? 1 : 0;
 	D->esz = D->fp64 ? 8 : 4;
@@ -2417,38 +2441,107 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 	return 0;
 }
 
-int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
-                              const double *w, long nw, double *out, long nout)
+extern "C++" {
+namespace {
+
+// A checked flux-sums request: the kernel's term table and the doubles of device memory it
+// needs (outputs, weights, then the table).
+struct FluxPlan {
+	int id = 0, q = 0, nterm = 0;
+	long nw = 0, nout = 0, blocks = 0, doubles = 0;
+	std::vector<int> t;
+};
+
+int fluxPlan(alis_dev_world *D, int id, int q, int nterm, const int *terms, long nw, long nout, FluxPlan &F)
 {
 	if (alis_dev_phaser_size(D, id) < 0) return fail("no such phaser");
-	DevPhaser &P = D->phs[id];
+	const DevPhaser &P = D->phs[id];
 	if (q < 0 || q >= P.nf || nterm < 1 || nterm > 12 || nw < 1 || nout < 1) return fail("flux sums: bad frequency, term count or sizes");
-	std::vector<int> t((size_t)FLUX_TERM * nterm);
-	long blocks = 0;
+	F.id = id, F.q = q, F.nterm = nterm, F.nw = nw, F.nout = nout;
+	F.t.assign((size_t)FLUX_TERM * nterm, 0);
+	F.blocks = 0;
 	for (int n = 0; n < nterm; n++) {
 		const int *u = terms + 8 * n;
 		const long na = u[2], nb = u[3], len = u[0] == 0 ? na * 64 : nb, wl = u[0] == 0 ? nb : na;
 		if ((u[0] != 0 && u[0] != 1) || na < 1 || nb < 1 || u[1] < 0 || u[1] + na * nb > P.ncell || u[4] < 0 || u[4] > 3
 		    || u[5] < 0 || u[5] > 3 || u[6] < 0 || u[6] + wl > nw || u[7] < 0 || u[7] + len > nout)
 			return fail("flux sums: term outside the phaser faces or the buffers");
-		std::copy(u, u + 8, t.begin() + (size_t)FLUX_TERM * n);
-		t[(size_t)FLUX_TERM * n + 8] = (int)blocks;
-		blocks += u[0] == 0 ? (na + 3) / 4 : (nb + 255) / 256;
+		std::copy(u, u + 8, F.t.begin() + (size_t)FLUX_TERM * n);
+		F.t[(size_t)FLUX_TERM * n + 8] = (int)F.blocks;
+		F.blocks += u[0] == 0 ? (na + 3) / 4 : (nb + 255) / 256;
 	}
+	F.doubles = nout + nw + ((long)F.t.size() + 1) / 2;
+	return 0;
+}
+
+// Run a planned request with `dev` (F.doubles doubles of device memory) holding the outputs,
+// the weights and the term table.  Outputs no term writes read back as zero.
+int fluxRun(alis_dev_world *D, const FluxPlan &F, const double *w, double *out, double *dev)
+{
+	DevPhaser &P = D->phs[F.id];
 	if (phaserFlushAny(D, P)) return -1;
-	const long tab = ((long)t.size() + 1) / 2;     // doubles that hold the term table
-	if (ensureScratch(D, nout + nw + tab)) return -1;
-	double *dOut = (double *)D->scratch, *dW = dOut + nout;
-	int *dT = (int *)(dW + nw);
-	HIPCHECK(hipMemcpyAsync(dW, w, nw * sizeof(double), hipMemcpyHostToDevice, D->st));
-	HIPCHECK(hipMemcpyAsync(dT, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
-	const long base = (long)q * 4 * P.ncell;
-	if (D->dft64) k_flux_sums<double><<<(unsigned)blocks, 256, 0, D->st>>>((const Pair<double>::V *)P.acc + base, P.ncell, dT, nterm, dW, dOut);
-	else k_flux_sums<float><<<(unsigned)blocks, 256, 0, D->st>>>((const Pair<float>::V *)P.acc + base, P.ncell, dT, nterm, dW, dOut);
+	double *dOut = dev, *dW = dOut + F.nout;
+	int *dT = (int *)(dW + F.nw);
+	k_fill<double><<<blocksFor(F.nout), 256, 0, D->st>>>(dOut, F.nout, 0.0);
 	HIPCHECK(hipGetLastError());
-	HIPCHECK(hipMemcpyAsync(out, dOut, nout * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipMemcpyAsync(dW, w, F.nw * sizeof(double), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemcpyAsync(dT, F.t.data(), F.t.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
+	const long base = (long)F.q * 4 * P.ncell;
+	if (D->dft64) k_flux_sums<double><<<(unsigned)F.blocks, 256, 0, D->st>>>((const Pair<double>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut);
+	else k_flux_sums<float><<<(unsigned)F.blocks, 256, 0, D->st>>>((const Pair<float>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, dOut, F.nout * sizeof(double), hipMemcpyDeviceToHost, D->st));
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
+}
+
+// Device address of this GPU's copy of `scratch`: looked up with hipGetSymbolAddress on the
+// first call per device (with that device current) and kept.  In host code the array's name
+// only stands for the symbol; kernels must be given this address.
+int fluxScratch(alis_dev_world *D, double **out)
+{
+	static std::mutex lock;
+	static std::map<int, double *> addr;
+	std::lock_guard<std::mutex> g(lock);
+	auto it = addr.find(D->device);
+	if (it == addr.end()) {
+		int cur = 0;
+		HIPCHECK(hipGetDevice(&cur));
+		if (cur != D->device) HIPCHECK(hipSetDevice(D->device));
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (cur != D->device) HIPCHECK(hipSetDevice(cur));
+		if (e != hipSuccess) return fail("hipGetSymbolAddress(scratch)", e);
+		if (!p) return fail("hipGetSymbolAddress(scratch) returned no address");
+		it = addr.emplace(D->device, (double *)p).first;
+	}
+	*out = it->second;
+	return 0;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                              const double *w, long nw, double *out, long nout)
+{
+	FluxPlan F;
+	if (fluxPlan(D, id, q, nterm, terms, nw, nout, F)) return -1;
+	if (ensureScratch(D, F.doubles)) return -1;
+	return fluxRun(D, F, w, out, (double *)D->scratch);
+}
+
+int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                                      const double *w, long nw, double *out, long nout)
+{
+	FluxPlan F;
+	if (fluxPlan(D, id, q, nterm, terms, nw, nout, F)) return -1;
+	if (F.doubles > FLUX_SCRATCH_DOUBLES)
+		return fail("flux sums without allocation: " + std::to_string(F.doubles * 8) + " bytes of outputs, weights and terms exceed the " +
+		            std::to_string(FLUX_SCRATCH_DOUBLES * 8) + "-byte scratch array");
+	double *s = nullptr;
+	if (fluxScratch(D, &s)) return -1;
+	return fluxRun(D, F, w, out, s);
 }
 
 int alis_dev_phaser_sync(alis_dev_world *D, int id)

@@ -160,6 +160,15 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
  * buffers. */
 int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
                               const double *w, long nw, double *out, long nout);
+/* The same sums, allocating nothing: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under graph capture, tight serving loops).  Outputs,
+ * weights and the term table live in a file-scope device array of 4 MiB per GPU (held until
+ * the process ends), so nout + nw + (9 * nterm + 1) / 2 must not exceed 524288 doubles
+ * (-1 otherwise).  Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Results are those of alis_dev_phaser_flux_sums; outputs that no term writes are
+ * zero in both. */
+int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                                      const double *w, long nw, double *out, long nout);
 int alis_dev_phaser_remove(alis_dev_world *D, int id);
 
 /* Near-to-far projection, the surface sums (src/farfld.c:248-459).  For each of n

@@ -479,6 +479,7 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
 		    || u[5] < 0 || u[5] > 3 || u[6] < 0 || u[6] + wl > nw || u[7] < 0 || u[7] + len > nout)
 			return fail("flux sums: term outside the phaser faces or the buffers");
 	}
+	for (long c = 0; c < nout; c++) out[c] = 0;
 	for (int n = 0; n < nterm; n++) {
 		const int *u = terms + 8 * n;
 		const long na = u[2], nb = u[3];
@@ -496,6 +497,15 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
 			}
 	}
 	return 0;
+}
+
+/* The allocation-free form of the device entry: the same sums, refused beyond the size of
+ * the device's scratch array (outputs, weights and term table in 524288 doubles). */
+int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nterm, const int *terms,
+                                      const double *w, long nw, double *out, long nout)
+{
+	if (nterm >= 1 && nout + nw + (9L * nterm + 1) / 2 > (1L << 19)) return fail("flux sums without allocation: request exceeds the scratch array");
+	return alis_dev_phaser_flux_sums(D, id, q, nterm, terms, w, nw, out, nout);
 }
 
 int alis_dev_phaser_remove(alis_dev_world *D, int id)

@@ -578,6 +578,28 @@ int alis_physics_dispersion(int cells, int m, int p, int steps, int fp64, double
 	return c;
 }
 
+/* Run a scenario and keep its worlds alive (for tests that call the engine ABI directly):
+ * the device world of its first world, the device index of that world's phaser array and
+ * the array's frequency count.  alis_drive_release frees them. */
+int alis_drive_hold(const char *name, int steps, void **dev, int *pid, int *nf)
+{
+	OUT = fopen("/dev/null", "w");
+	int rc = run(name, steps);
+	fclose(OUT);
+	OUT = 0;
+	if (rc || !LAST || !LAST->dev || !LASTP) return rc ? rc : 3;
+	if (LASTP->devId < 0) return 4;
+	*dev = LAST->dev;
+	*pid = LASTP->devId;
+	*nf = LASTP->count;
+	return 0;
+}
+
+void alis_drive_release(void)
+{
+	releaseRun();
+}
+
 /* The host library's double-precision far field (farFieldD, alis_amd/host/farfield.c). */
 double complex farFieldD(phaser P, tfield pol(phaser,float,float), double theta, double phi);
 
