@@ -7,6 +7,7 @@ Products (all git-ignored, shipped to GPU boxes with the tree):
   tests/golden/libdrive.so         parity scenarios (tests/golden/drive.c) over libalis.so
   oracle/liboracle.so              CPU restatement of the engine ABI under the host library
   tests/golden/libdrive_oracle.so  the same scenarios over liboracle.so
+  oracle/libwl_oracle.so           the benchmark workloads over liboracle.so (parity at full size)
   oracle/_ref/                     the reference built from its sources (oracle/Makefile)
 """
 import os
@@ -81,6 +82,7 @@ def clients(libdir, soname, libname):
         client(ROOT / "tests" / "golden" / "drive.c", ROOT / "tests" / "golden" / "libdrive.so", libdir, libname)
     else:
         client(ROOT / "tests" / "golden" / "drive.c", ROOT / "tests" / "golden" / "libdrive_oracle.so", libdir, libname)
+        client(PKG / "workloads.c", ROOT / "oracle" / "libwl_oracle.so", libdir, libname)
 
 
 def build_oracle():

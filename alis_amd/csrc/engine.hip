@@ -575,16 +575,17 @@ __global__ __launch_bounds__(256) void k_fetch(unsigned *__restrict__ dst, const
 	for (int w = (int)(blockIdx.x * blockDim.x + threadIdx.x); w < words; w += (int)(gridDim.x * blockDim.x)) dst[w] = src[w];
 }
 
-// Several boxes of several arrays into one float buffer: segment s covers dst[off[s] ..
-// off[s+1]) with the cells of its box, k fastest.
+// Several boxes of several arrays (of one or several worlds, float or double) into one float
+// buffer: segment s covers dst[off[s] .. off[s+1]) with the cells of its box, k fastest.
 struct GatherSeg {
 	const void *src;
 	Lay L;
 	Box b;
 	long off;
+	int dbl;                    // the source array holds doubles
+	int pad;                    // zero: tables are compared byte for byte
 };
 
-template<class T>
 __global__ __launch_bounds__(256) void k_gather(const GatherSeg *__restrict__ seg, int nseg, long total, float *dst)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -599,7 +600,8 @@ __global__ __launch_bounds__(256) void k_gather(const GatherSeg *__restrict__ se
 	const long l = t - g.off;
 	const int nk = g.b.k1 - g.b.k0 + 1, nj = g.b.j1 - g.b.j0 + 1;
 	const int dk = (int)(l % nk), dj = (int)(l / nk % nj), di = (int)(l / nk / nj);
-	dst[t] = (float)((const T *)g.src)[g.L.at(g.b.i0 + di, g.b.j0 + dj, g.b.k0 + dk)];
+	const long at = g.L.at(g.b.i0 + di, g.b.j0 + dj, g.b.k0 + dk);
+	dst[t] = g.dbl ? (float)((const double *)g.src)[at] : ((const float *)g.src)[at];
 }
 
 // Pack a box of a device array into a compact buffer (float or double), and back.
@@ -648,6 +650,51 @@ __global__ __launch_bounds__(256) void k_fill(T *p, long n, T v)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t < n) p[t] = v;
+}
+
+// Probe spectra (the running DFT of writeSpectrum, src/output.c:177-230) in FP64 with
+// compensated (Neumaier) summation.  acc holds, per probe p and frequency bin b, the running
+// sum (re, im) at acc[2 (p nb + b)] and its compensation at acc[2 (p nb + b) + 1]; a thread
+// takes one bin and up to SPEC_PPT probes and adds ns staged samples (tau, v[0..np-1]) in
+// sample order, each with the factor e^{i 2 pi tau (nTop - b)} (one sincos per sample and
+// bin, shared by the thread's probes).
+constexpr int SPEC_PPT = 8;
+
+__device__ inline void neumaier(double &s, double &c, double x)
+{
+	const double t = s + x;
+	c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+	s = t;
+}
+
+__global__ __launch_bounds__(256) void k_spectrum_add(double2 *__restrict__ acc, int np, int nb, int nTop,
+                                                      const double *__restrict__ samp, int ns)
+{
+	const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	const int b = t % nb, p0 = t / nb * SPEC_PPT;
+	if (p0 >= np) return;
+	const int pn = min(SPEC_PPT, np - p0);
+	double2 sum[SPEC_PPT], cmp[SPEC_PPT];
+#pragma unroll
+	for (int u = 0; u < SPEC_PPT; u++)
+		if (u < pn) sum[u] = acc[2 * ((long)(p0 + u) * nb + b)], cmp[u] = acc[2 * ((long)(p0 + u) * nb + b) + 1];
+	const double w = 2 * M_PI;
+	const int harmonic = nTop - b;
+	for (int q = 0; q < ns; q++) {
+		const double *r = samp + (long)q * (1 + np);
+		double sn, cs;
+		sincos(w * r[0] * harmonic, &sn, &cs);
+#pragma unroll
+		for (int u = 0; u < SPEC_PPT; u++)
+			if (u < pn) {
+				const double v = r[1 + p0 + u];
+				neumaier(sum[u].x, cmp[u].x, v * cs);
+				neumaier(sum[u].y, cmp[u].y, v * sn);
+			}
+	}
+#pragma unroll
+	for (int u = 0; u < SPEC_PPT; u++)
+		if (u < pn) acc[2 * ((long)(p0 + u) * nb + b)] = sum[u], acc[2 * ((long)(p0 + u) * nb + b) + 1] = cmp[u];
 }
 
 // Inner sums of the net flux Re(E x H*) of one frequency over the phaser faces
@@ -817,9 +864,6 @@ struct alis_dev_world {
 	std::vector<DevPhaser> phs;
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
-	void *gatherDev = nullptr;       // probe copies: segment table + gathered values
-	float *gatherPinned = nullptr;
-	size_t gatherBytes = 0, gatherPinnedBytes = 0;
 	Staging phsStage, phfStage;      // staging of batched phaser samples and flushes led by this world
 	rocblas_handle blas = nullptr;   // far-field products, created on first use
 	void *ffWork = nullptr;          // far-field workspace
@@ -2190,8 +2234,6 @@ void alis_dev_release(alis_dev_world *D)
 		if (S->dev) hipFree(S->dev);
 		if (S->done) hipEventDestroy(S->done);
 	}
-	if (D->gatherDev) hipFree(D->gatherDev);
-	if (D->gatherPinned) hipHostFree(D->gatherPinned);
 	if (D->blas) rocblas_destroy_handle(D->blas);
 	if (D->st && --*D->streamRefs == 0) {
 		hipStreamDestroy(D->st);
@@ -2230,94 +2272,104 @@ int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int
 	return D->fp64 ? syncHost<double>(D, b) : syncHost<float>(D, b);
 }
 
-// One world's share of alis_dev_sync_boxes: segments of E, H (every part) and the currents
-// of dispersive objects inside each box, gathered on the device and copied into pinned
-// memory (the caller waits for the stream and scatters).
+// Probe copies (alis_dev_sync_boxes): the segments of E, H (every part) and of the currents
+// of dispersive objects inside each box, for every world, gathered by one launch into one
+// buffer and copied by one transfer into pinned memory; the host then scatters the values
+// into the descriptors' arrays.  The segment table is uploaded only when it changes (a
+// program reads the same probes every step).
 extern "C++" {
 struct GatherHost { float ***host; Box b; long off; };
 
-template<class T>
-int gatherBoxes(alis_dev_world *D, const std::vector<Box> &boxes, std::vector<GatherHost> &out)
+struct GatherState {
+	void *dev = nullptr;        // segment table, then the gathered values
+	size_t devBytes = 0, tabBytes = 0;
+	float *pinned = nullptr;
+	size_t pinnedBytes = 0;
+	std::vector<GatherSeg> table;   // the table on the device (valid when tabBytes > 0)
+	hipEvent_t ev = nullptr;
+};
+GatherState g_gather;
+
+void gatherWorld(alis_dev_world *D, const Box &box, std::vector<GatherSeg> &seg, std::vector<GatherHost> &out, long &total)
 {
 	const alis_dev_world_desc *W = &D->W;
 	const Box fb = fieldBox(D);
-	std::vector<GatherSeg> seg;
-	long total = 0;
 	auto add = [&](const void *src, const Lay &L, const Box &b, float ***host) {
 		if (!b.count()) return;
-		seg.push_back(GatherSeg{src, L, b, total});
+		seg.push_back(GatherSeg{src, L, b, total, D->fp64, 0});
 		out.push_back(GatherHost{host, b, total});
 		total += b.count();
 	};
-	for (const Box &box : boxes) {
-		const Box b{std::max(box.i0, fb.i0), std::min(box.i1, fb.i1), std::max(box.j0, fb.j0), std::min(box.j1, fb.j1),
-		            std::max(box.k0, fb.k0), std::min(box.k1, fb.k1)};
-		for (int h = 0; h < 2; h++)
-			for (int m = 0; m <= D->cplx; m++)
-				for (int a = 0; a < 3; a++) add(D->F[h][m][a], D->L, b, W->field[h][m][a]);
-		for (size_t n = 0; n < D->obj.size(); n++) {
-			const DevObj &O = D->obj[n];
-			const Box c{std::max(box.i0, O.box.i0), std::min(box.i1, O.box.i1), std::max(box.j0, O.box.j0), std::min(box.j1, O.box.j1),
-			            std::max(box.k0, O.box.k0), std::min(box.k1, O.box.k1)};
-			if (c.i0 > c.i1 || c.j0 > c.j1 || c.k0 > c.k1) continue;
-			for (int m = 0; m <= D->cplx; m++)
-				for (int q = 0; q < 3; q++)
-					for (int p = 0; p < O.NK; p++) add(O.J[m][q][p], O.LO, c, W->obj[n].J[m][q][p]);
-		}
+	const Box b{std::max(box.i0, fb.i0), std::min(box.i1, fb.i1), std::max(box.j0, fb.j0), std::min(box.j1, fb.j1),
+	            std::max(box.k0, fb.k0), std::min(box.k1, fb.k1)};
+	for (int h = 0; h < 2; h++)
+		for (int m = 0; m <= D->cplx; m++)
+			for (int a = 0; a < 3; a++) add(D->F[h][m][a], D->L, b, W->field[h][m][a]);
+	for (size_t n = 0; n < D->obj.size(); n++) {
+		const DevObj &O = D->obj[n];
+		const Box c{std::max(box.i0, O.box.i0), std::min(box.i1, O.box.i1), std::max(box.j0, O.box.j0), std::min(box.j1, O.box.j1),
+		            std::max(box.k0, O.box.k0), std::min(box.k1, O.box.k1)};
+		if (c.i0 > c.i1 || c.j0 > c.j1 || c.k0 > c.k1) continue;
+		for (int m = 0; m <= D->cplx; m++)
+			for (int q = 0; q < 3; q++)
+				for (int p = 0; p < O.NK; p++) add(O.J[m][q][p], O.LO, c, W->obj[n].J[m][q][p]);
 	}
-	if (!total) return 0;
-	const size_t segB = (seg.size() * sizeof(GatherSeg) + 255) / 256 * 256, need = segB + total * sizeof(float);
-	if (need > D->gatherBytes) {
-		HIPCHECK(hipStreamSynchronize(D->st));
-		if (D->gatherDev) HIPCHECK(hipFree(D->gatherDev));
-		D->gatherDev = nullptr, D->gatherBytes = 0;
-		HIPCHECK(hipMalloc(&D->gatherDev, need));
-		D->gatherBytes = need;
-	}
-	if ((size_t)total * sizeof(float) > D->gatherPinnedBytes) {
-		if (D->gatherPinned) HIPCHECK(hipHostFree(D->gatherPinned));
-		D->gatherPinned = nullptr, D->gatherPinnedBytes = 0;
-		HIPCHECK(hipHostMalloc((void **)&D->gatherPinned, total * sizeof(float), hipHostMallocDefault));
-		D->gatherPinnedBytes = total * sizeof(float);
-	}
-	float *vals = (float *)((char *)D->gatherDev + segB);
-	HIPCHECK(hipMemcpyAsync(D->gatherDev, seg.data(), seg.size() * sizeof(GatherSeg), hipMemcpyHostToDevice, D->st));
-	k_gather<T><<<blocksFor(total), 256, 0, D->st>>>((const GatherSeg *)D->gatherDev, (int)seg.size(), total, vals);
-	HIPCHECK(hipGetLastError());
-	HIPCHECK(hipMemcpyAsync(D->gatherPinned, vals, total * sizeof(float), hipMemcpyDeviceToHost, D->st));
-	return 0;
 }
 }  // extern "C++"
 
 int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 {
-	// the worlds in order of first appearance, each with its boxes
-	std::vector<alis_dev_world *> worlds;
-	std::vector<std::vector<Box>> boxes;
+	if (n < 1) return 0;
+	std::vector<GatherSeg> seg;
+	std::vector<GatherHost> out;
+	long total = 0;
 	for (int e = 0; e < n; e++) {
 		const int *b = box + 6 * e;
-		size_t w = 0;
-		while (w < worlds.size() && worlds[w] != D[e]) w++;
-		if (w == worlds.size()) worlds.push_back(D[e]), boxes.emplace_back();
-		boxes[w].push_back(Box{b[0], b[1], b[2], b[3], b[4], b[5]});
+		gatherWorld(D[e], Box{b[0], b[1], b[2], b[3], b[4], b[5]}, seg, out, total);
 	}
-	std::vector<std::vector<GatherHost>> out(worlds.size());
-	for (size_t w = 0; w < worlds.size(); w++) {
-		const int rc = worlds[w]->fp64 ? gatherBoxes<double>(worlds[w], boxes[w], out[w]) : gatherBoxes<float>(worlds[w], boxes[w], out[w]);
-		if (rc) return rc;
+	if (!total) return 0;
+	GatherState &G = g_gather;
+	// the gather runs on the first world's stream after the work of every other world's stream
+	const hipStream_t st = D[0]->st;
+	if (!G.ev) HIPCHECK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
+	std::vector<hipStream_t> joined{st};
+	for (int e = 1; e < n; e++) {
+		if (std::find(joined.begin(), joined.end(), D[e]->st) != joined.end()) continue;
+		joined.push_back(D[e]->st);
+		HIPCHECK(hipEventRecord(G.ev, D[e]->st));
+		HIPCHECK(hipStreamWaitEvent(st, G.ev, 0));
 	}
-	for (size_t w = 0; w < worlds.size(); w++) {
-		bool seen = false;
-		for (size_t v = 0; v < w; v++) seen |= worlds[v]->st == worlds[w]->st;
-		if (!seen) HIPCHECK(hipStreamSynchronize(worlds[w]->st));
+	const size_t segB = (seg.size() * sizeof(GatherSeg) + 255) / 256 * 256, need = segB + total * sizeof(float);
+	if (need > G.devBytes) {
+		HIPCHECK(hipDeviceSynchronize());
+		if (G.dev) HIPCHECK(hipFree(G.dev));
+		G.dev = nullptr, G.devBytes = 0, G.tabBytes = 0;
+		HIPCHECK(hipMalloc(&G.dev, need));
+		G.devBytes = need;
 	}
-	for (size_t w = 0; w < worlds.size(); w++)
-		for (const GatherHost &g : out[w]) {
-			const float *v = worlds[w]->gatherPinned + g.off;
-			const int nk = g.b.k1 - g.b.k0 + 1;
-			for (int i = g.b.i0; i <= g.b.i1; i++)
-				for (int j = g.b.j0; j <= g.b.j1; j++, v += nk) memcpy(&g.host[i][j][g.b.k0], v, nk * sizeof(float));
-		}
+	if ((size_t)total * sizeof(float) > G.pinnedBytes) {
+		if (G.pinned) HIPCHECK(hipHostFree(G.pinned));
+		G.pinned = nullptr, G.pinnedBytes = 0;
+		HIPCHECK(hipHostMalloc((void **)&G.pinned, total * sizeof(float), hipHostMallocDefault));
+		G.pinnedBytes = total * sizeof(float);
+	}
+	const bool same = G.tabBytes == segB && G.table.size() == seg.size() && !memcmp(G.table.data(), seg.data(), seg.size() * sizeof(GatherSeg));
+	if (!same) {
+		G.table = seg;
+		G.tabBytes = segB;
+		HIPCHECK(hipMemcpyAsync(G.dev, G.table.data(), G.table.size() * sizeof(GatherSeg), hipMemcpyHostToDevice, st));
+	}
+	float *vals = (float *)((char *)G.dev + segB);
+	k_gather<<<blocksFor(total), 256, 0, st>>>((const GatherSeg *)G.dev, (int)seg.size(), total, vals);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(G.pinned, vals, total * sizeof(float), hipMemcpyDeviceToHost, st));
+	HIPCHECK(hipStreamSynchronize(st));
+	for (const GatherHost &g : out) {
+		const float *v = G.pinned + g.off;
+		const int nk = g.b.k1 - g.b.k0 + 1;
+		for (int i = g.b.i0; i <= g.b.i1; i++)
+			for (int j = g.b.j0; j <= g.b.j1; j++, v += nk) memcpy(&g.host[i][j][g.b.k0], v, nk * sizeof(float));
+	}
 	return 0;
 }
 
@@ -2927,6 +2979,104 @@ void alis_dev_batch_release(alis_dev_batch *B)
 		if (B->postDone[h]) hipEventDestroy(B->postDone[h]);
 	}
 	delete B;
+}
+
+// ---- probe spectra ----
+
+constexpr int SPEC_RING = 256;  // samples staged per k_spectrum_add launch
+
+struct alis_dev_spectrum {
+	int np = 0, nb = 0, nTop = 0;
+	double2 *acc = nullptr;     // [probe][bin] (sum, compensation)
+	double *stage[2] = {};      // device copies of the staged samples
+	double *pinned[2] = {};     // samples (tau, v[np]) staged in pinned host memory
+	hipEvent_t done[2] = {};    // the last launch that read stage/pinned[s] has ended
+	int cur = 0, pend = 0;
+};
+
+namespace {
+
+// Spectra share one stream of their own, beside the worlds' streams.
+hipStream_t g_specStream = nullptr;
+int g_specRefs = 0;
+
+int spectrumFlush(alis_dev_spectrum *S)
+{
+	if (!S->pend) return 0;
+	const int c = S->cur;
+	HIPCHECK(hipMemcpyAsync(S->stage[c], S->pinned[c], (size_t)S->pend * (1 + S->np) * sizeof(double), hipMemcpyHostToDevice, g_specStream));
+	const long threads = (long)S->nb * ((S->np + SPEC_PPT - 1) / SPEC_PPT);
+	k_spectrum_add<<<blocksFor(threads), 256, 0, g_specStream>>>(S->acc, S->np, S->nb, S->nTop, S->stage[c], S->pend);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipEventRecord(S->done[c], g_specStream));
+	S->cur ^= 1, S->pend = 0;
+	return 0;
+}
+
+}  // namespace
+
+int alis_dev_spectrum_create(int nprobe, int nbin, int nTop, alis_dev_spectrum **out)
+{
+	*out = nullptr;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no GPU is visible to the engine");
+	if (nprobe < 1 || nbin < 1) return fail("spectrum: no probes or no frequency bins");
+	if (!g_specStream) HIPCHECK(hipStreamCreateWithFlags(&g_specStream, hipStreamNonBlocking));
+	alis_dev_spectrum *S = new alis_dev_spectrum;
+	g_specRefs++;
+	S->np = nprobe, S->nb = nbin, S->nTop = nTop;
+	const long n = 2L * nprobe * nbin;
+	auto bad = [&](const char *what, hipError_t e) { alis_dev_spectrum_release(S); return fail(what, e); };
+	hipError_t e;
+	if ((e = hipMalloc((void **)&S->acc, n * sizeof(double2))) != hipSuccess) return bad("hipMalloc", e);
+	k_fill<double><<<blocksFor(2 * n), 256, 0, g_specStream>>>((double *)S->acc, 2 * n, 0.0);
+	for (int c = 0; c < 2; c++) {
+		const size_t bytes = (size_t)SPEC_RING * (1 + nprobe) * sizeof(double);
+		if ((e = hipMalloc((void **)&S->stage[c], bytes)) != hipSuccess) return bad("hipMalloc", e);
+		if ((e = hipHostMalloc((void **)&S->pinned[c], bytes, hipHostMallocDefault)) != hipSuccess) return bad("hipHostMalloc", e);
+		if ((e = hipEventCreateWithFlags(&S->done[c], hipEventDisableTiming)) != hipSuccess) return bad("hipEventCreate", e);
+		if ((e = hipEventRecord(S->done[c], g_specStream)) != hipSuccess) return bad("hipEventRecord", e);
+	}
+	*out = S;
+	return 0;
+}
+
+int alis_dev_spectrum_add(alis_dev_spectrum *S, double tau, const double *v)
+{
+	if (!S->pend) HIPCHECK(hipEventSynchronize(S->done[S->cur]));   // the last launch from this staging ended
+	double *r = S->pinned[S->cur] + (size_t)S->pend * (1 + S->np);
+	r[0] = tau;
+	memcpy(r + 1, v, S->np * sizeof(double));
+	if (++S->pend == SPEC_RING) return spectrumFlush(S);
+	return 0;
+}
+
+int alis_dev_spectrum_read(alis_dev_spectrum *S, double *out)
+{
+	if (spectrumFlush(S)) return -1;
+	const long n = 2L * S->np * S->nb;
+	std::vector<double2> a(n);
+	HIPCHECK(hipMemcpyAsync(a.data(), S->acc, n * sizeof(double2), hipMemcpyDeviceToHost, g_specStream));
+	HIPCHECK(hipStreamSynchronize(g_specStream));
+	for (long e = 0; e < n / 2; e++) out[2 * e] = a[2 * e].x + a[2 * e + 1].x, out[2 * e + 1] = a[2 * e].y + a[2 * e + 1].y;
+	return 0;
+}
+
+void alis_dev_spectrum_release(alis_dev_spectrum *S)
+{
+	if (!S) return;
+	if (g_specStream) (void)hipStreamSynchronize(g_specStream);
+	if (S->acc) (void)hipFree(S->acc);
+	for (int c = 0; c < 2; c++) {
+		if (S->stage[c]) (void)hipFree(S->stage[c]);
+		if (S->pinned[c]) (void)hipHostFree(S->pinned[c]);
+		if (S->done[c]) (void)hipEventDestroy(S->done[c]);
+	}
+	delete S;
+	if (--g_specRefs == 0 && g_specStream) {
+		(void)hipStreamDestroy(g_specStream);
+		g_specStream = nullptr;
+	}
 }
 
 int alis_dev_synchronize(alis_dev_world *D)

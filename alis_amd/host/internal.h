@@ -37,6 +37,9 @@ void alisForgetPhaser(phaser P);
 void alisFatal(const char *what, int rc);
 phaser phaserBase(phaser P);     /* first element of the array holding P, or 0 */
 
+/* observers.c */
+void alisForgetSpectra(world W);  /* drop the writeSpectrum records of a world */
+
 /* describe.c */
 void describeWorld(world W, alis_dev_world_desc *d);
 void freeWorldDesc(alis_dev_world_desc *d);

@@ -17,7 +17,6 @@ struct queue {
 	FILE *File;
 	unsigned long int N;
 	unsigned long long int timeStamp;
-	float complex **Spectrum;
 	queue Q;
 };
 
@@ -52,7 +51,12 @@ float get(world W, field F, float x, float y, float z)
 	if (W->dev && W->hostStale) {
 		int readsAux = F==SoEx || F==SoEy || F==SoEz || F==SoEE || F==ScEx || F==ScEy || F==ScEz || F==ScEE;
 		if (readsAux || W->isAux) syncWorld(W);
-		else alisProbe(W, MIN(i,-i)-2, MAX(i,-i)+2, MIN(j,-j)-2, MAX(j,-j)+2, MIN(k,-k)-2, MAX(k,-k)+2);
+		else {
+			/* the accessors read within two cells of the node, mirrored across a symmetry
+			 * plane when its index is negative */
+			int ci = i < 0 && W->xMinSur==SYM ? -i : i, cj = j < 0 && W->yMinSur==SYM ? -j : j, ck = k < 0 && W->zMinSur==SYM ? -k : k;
+			alisProbe(W, ci-2, ci+2, cj-2, cj+2, ck-2, ck+2);
+		}
 	}
 	return F(W, i, j, k);
 }
@@ -145,60 +149,192 @@ void (writeRow)(world W, char *format, ...)
 	va_end(ap);
 }
 
-/* Running DFT of probe values over N calls between wavelengths LMin and LMax, normalised by
- * the source spectrum; on the N-th call writes wavelength, |S/S0| and arg(S/S0)/pi per
- * probe (src/output.c:177-230). */
+/*
+ * Probe spectra (writeSpectrum, src/output.c:177-230): the running DFT of the values a
+ * program passes each call, over bins n = 0 .. nMax-nMin of the wavelengths
+ * dL nMin nMax / (nMax - n), normalised by the spectrum of the world's source, written as
+ * "wavelength |S/S0|... arg(S/S0)/pi..." on the N-th call.  One record per (world, name).
+ * In the reference precision the sums are the reference's float complex ones, term for
+ * term; in the FP32/FP64 precisions the sample sums run on the device in FP64 with
+ * compensated summation (alis_dev_spectrum), and the source spectrum in double on the host.
+ */
+typedef struct spectrum {
+	world W;
+	char *name;                 /* the name as called, before W->ID is prefixed */
+	unsigned long calls;
+	int M, nMin, nMax;          /* probes (values per call) and the bin range */
+	float dL;
+	int f64;
+	float complex *acc;         /* reference precision: [1+M][bins], row 0 the source */
+	double complex *norm;       /* FP32/FP64: the source's spectrum per bin */
+	alis_dev_spectrum *dev;     /* FP32/FP64: the probe sums */
+	struct spectrum *next;
+} spectrum;
+
+static spectrum *spectra;
+
+/* Drop the spectra of a world that goes away (a later world may reuse its address). */
+void alisForgetSpectra(world W)
+{
+	for (spectrum **p = &spectra; *p;) {
+		spectrum *q = *p;
+		if (q->W != W) { p = &q->next; continue; }
+		*p = q->next;
+		alis_dev_spectrum_release(q->dev);
+		free(q->name), free(q->acc), free(q->norm), free(q);
+	}
+}
+
+static spectrum *spectrumOf(world W, const char *name)
+{
+	for (spectrum *q = spectra; q; q = q->next) if (q->W == W && !strcmp(q->name, name)) return q;
+	return 0;
+}
+
+/* A new record: bins, the source spectrum and the accumulators of W's precision. */
+static spectrum *spectrumNew(world W, const char *name, int N, float LMin, float LMax, int M)
+{
+	spectrum *q = calloc(1, sizeof *q);
+	q->W = W, q->name = strdup(name), q->M = M;
+	q->dL = LMin * LMax / (N * W->dt);
+	q->nMin = floor(LMin/q->dL), q->nMax = ceil(LMax/q->dL);
+	q->f64 = W->devDftPrec != 0;
+	const int nF = 1 + q->nMax - q->nMin;
+	const float dL = q->dL;
+	const int nMin = q->nMin, nMax = q->nMax;
+	source S = normalisingSource(W);
+	if (!q->f64) {
+		q->acc = calloc((size_t)(1 + M) * nF, sizeof(float complex));
+		for (int n = 0; S && W->N && n < nF; n++) {
+			float f = (nMax-n) / (dL * nMin * nMax);
+			float complex s0 = 0;
+			for (int m = 1; m < W->N; m++) s0 += S->waveForm(W, S, W->dt*m, S->phase) * cexpf(2*PI*I*f*W->dt*m);
+			if (cabsf(s0) < 1E-3) s0 = 0;
+			else if (W->sourceType == 2) s0 *= sqrtf(f * S->wavelength);
+			else if (W->sourceType == 3) s0 *= f * S->wavelength;
+			q->acc[n] = s0;
+		}
+	}
+	else {
+		q->norm = calloc(nF, sizeof(double complex));
+		for (int n = 0; S && W->N && n < nF; n++) {
+			const double f = (nMax - n) / ((double)dL * nMin * nMax);
+			double complex s0 = 0;
+			for (int m = 1; m < W->N; m++) s0 += (double)S->waveForm(W, S, W->dt*m, S->phase) * cexp(2 * M_PI * I * f * ((double)W->dt * m));
+			if (cabs(s0) < 1E-3) s0 = 0;
+			else if (W->sourceType == 2) s0 *= sqrt(f * S->wavelength);
+			else if (W->sourceType == 3) s0 *= f * S->wavelength;
+			q->norm[n] = s0;
+		}
+		alisFatal("spectrum", alis_dev_spectrum_create(M, nF, nMax, &q->dev));
+	}
+	q->next = spectra, spectra = q;
+	return q;
+}
+
+/* Sums per bin of probe p (0: the source's spectrum), reference precision or FP64. */
+static double complex spectrumAt(spectrum *q, const double *dev, int p, int n)
+{
+	const int nF = 1 + q->nMax - q->nMin;
+	if (!q->f64) return q->acc[(size_t)p * nF + n];
+	if (!p) return q->norm[n];
+	return dev[2 * ((size_t)(p-1) * nF + n)] + I * dev[2 * ((size_t)(p-1) * nF + n) + 1];
+}
+
+static void spectrumWrite(spectrum *q, const char *path)
+{
+	const int nF = 1 + q->nMax - q->nMin, M = q->M;
+	double *dev = 0;
+	if (q->f64) {
+		dev = malloc(2 * sizeof(double) * M * nF);
+		alisFatal("spectrum read", alis_dev_spectrum_read(q->dev, dev));
+	}
+	char file[1100];
+	snprintf(file, sizeof file, "%s.txt", path);
+	FILE *fp = fopen(file, "w");
+	for (int n = 0; fp && n < nF; n++) {
+		fprintf(fp, "%g", (q->dL * q->nMin * q->nMax) / (q->nMax-n));
+		for (int part = 0; part < 2; part++)
+			for (int p = 1; p <= M; p++) {
+				if (!q->f64) {
+					float complex s0 = q->acc[n], r = s0 ? q->acc[(size_t)p * nF + n] / s0 : 0;
+					fprintf(fp, "\t%g", s0 ? (part ? cargf(r)/PI : cabsf(r)) : 0);
+				}
+				else {
+					double complex s0 = q->norm[n], r = s0 ? spectrumAt(q, dev, p, n) / s0 : 0;
+					fprintf(fp, "\t%g", s0 ? (part ? carg(r)/M_PI : cabs(r)) : 0);
+				}
+			}
+		fprintf(fp, "\n");
+	}
+	if (fp) fclose(fp);
+	free(dev);
+}
+
+/* Raw sums of the spectrum (W, name): for p = 0 (the source's spectrum) and every probe,
+ * every bin as (re, im) doubles; returns the doubles written (0 if there is no such
+ * spectrum or n is too small).  For tests; not part of the reference's API. */
+long alisSpectrumRead(world W, const char *name, double *out, long n)
+{
+	spectrum *q = spectrumOf(W, name);
+	if (!q) return 0;
+	const int nF = 1 + q->nMax - q->nMin;
+	const long need = 2L * (1 + q->M) * nF;
+	if (n < need) return 0;
+	double *dev = 0;
+	if (q->f64) {
+		dev = malloc(2 * sizeof(double) * q->M * nF);
+		alisFatal("spectrum read", alis_dev_spectrum_read(q->dev, dev));
+	}
+	for (int p = 0; p <= q->M; p++)
+		for (int b = 0; b < nF; b++) {
+			double complex v = spectrumAt(q, dev, p, b);
+			out[2 * ((long)p * nF + b)] = creal(v), out[2 * ((long)p * nF + b) + 1] = cimag(v);
+		}
+	free(dev);
+	return need;
+}
+
 void (writeSpectrum)(world W, int N, float LMin, float LMax, char *format, ...)
 {
-	static queue head;
-	char filename[1024];
-	double d;
-	float dL = LMin * LMax / (N * W->dt);
-	int M = 0, nMin = floor(LMin/dL), nMax = ceil(LMax/dL);
+	char name[1024];
 	va_list ap;
 	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	do d = va_arg(ap, double), M++;
-	while (!NAN2INT(d));
-	va_end(ap);
-
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	queue Q = findQueue(W, filename, &head);
-	int nF = 1-nMin+nMax;
-	if (!Q->Spectrum) {
-		source S = normalisingSource(W);
-		MALLOC(Q->Spectrum, 0, M);
-		MALLOC(Q->Spectrum[0], 0, M*nF);
-		for (int m = 1; m < M; m++) Q->Spectrum[m] = Q->Spectrum[0]+m*nF;
-		if (S && W->N) {
-			for (int n = 0; n <= nMax-nMin; n++) {
-				float f = (nMax-n) / (dL * nMin * nMax);
-				for (int m = 1; m < W->N; m++) Q->Spectrum[0][n] += S->waveForm(W, S, W->dt*m, S->phase) * cexpf(2*PI*I*f*W->dt*m);
-				if (cabsf(Q->Spectrum[0][n]) < 1E-3) Q->Spectrum[0][n] = 0;
-				else if (W->sourceType == 2) Q->Spectrum[0][n] *= sqrtf(f * S->wavelength);
-				else if (W->sourceType == 3) Q->Spectrum[0][n] *= f * S->wavelength;
-			}
+	vsnprintf(name, sizeof name, format, ap);
+	int M = 0;
+	double v[256];
+	for (double d = va_arg(ap, double); !NAN2INT(d); d = va_arg(ap, double)) {
+		if (M == 256) {
+			fprintf(stderr, "writeSpectrum: more than 256 values per call\n");
+			exit(1);
 		}
-	}
-	for (int m = 1; m < M; m++) {
-		float t = W->t / (dL * nMin * nMax);
-		d = va_arg(ap, double);
-		for (int n = 0; n <= nMax-nMin; n++) Q->Spectrum[m][n] += (float) d * cexpf(2*PI*I*t*(nMax-n));
+		v[M++] = d;
 	}
 	va_end(ap);
-
-	if (Q->N == (unsigned long)N) {
-		strcat(filename, ".txt");
-		FILE *fp = fopen(filename, "w");
-		if (!fp) return;
-		for (int n = 0; n <= nMax-nMin; n++) {
-			fprintf(fp, "%g", (dL * nMin * nMax) / (nMax-n));
-			for (int m = 1; m < M; m++) fprintf(fp, "\t%g", Q->Spectrum[0][n] ? cabsf(Q->Spectrum[m][n]/Q->Spectrum[0][n]) : 0);
-			for (int m = 1; m < M; m++) fprintf(fp, "\t%g", Q->Spectrum[0][n] ? cargf(Q->Spectrum[m][n]/Q->Spectrum[0][n])/PI : 0);
-			fprintf(fp, "\n");
-		}
-		fclose(fp);
+	spectrum *q = spectrumOf(W, name);
+	if (!q) {
+		q = spectrumNew(W, name, N, LMin, LMax, M);
+		if (name[0]=='/') mkdir(W->ID, 0755);
+		char dir[1100];
+		snprintf(dir, sizeof dir, "%s%s", W->ID, name);
+		if (dir[0] && dir[strlen(dir)-1]=='/') mkdir(dir, 0755);
+	}
+	if (M != q->M) {
+		fprintf(stderr, "writeSpectrum: %d values per call, %d before\n", M, q->M);
+		exit(1);
+	}
+	q->calls++;
+	const int nF = 1 + q->nMax - q->nMin;
+	if (!q->f64) {
+		float t = W->t / (q->dL * q->nMin * q->nMax);
+		for (int m = 1; m <= M; m++)
+			for (int n = 0; n < nF; n++) q->acc[(size_t)m * nF + n] += (float) v[m-1] * cexpf(2*PI*I*t*(q->nMax-n));
+	}
+	else alisFatal("spectrum", alis_dev_spectrum_add(q->dev, (double)W->t / ((double)q->dL * q->nMin * q->nMax), v));
+	if (q->calls == (unsigned long)N) {
+		char path[1100];
+		snprintf(path, sizeof path, "%s%s", W->ID, name);
+		spectrumWrite(q, path);
 	}
 }
 

@@ -94,6 +94,7 @@ static void forgetProbes(world W);
 void alisForgetWorld(world W)
 {
 	forgetProbes(W);
+	alisForgetSpectra(W);
 	if (W->dev) {
 		dropBatch();
 		alis_dev_release(W->dev);

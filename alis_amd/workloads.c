@@ -15,8 +15,11 @@
  *              (dx = 10 nm; Bloch-periodic x and y with period 640 nm = 2a, a = 320 nm; an
  *              eps = 12 slab 0.5a thick with air holes of radius 0.3a on the square lattice
  *              of period a; complex fields), 8 point dipoles per world at positions and
- *              phases drawn from a splitmix64 stream (seed 2024).  All worlds step together
- *              with updateWorldsE/updateWorldsH.
+ *              phases drawn from a splitmix64 stream (seed 2024), and 8 probes per world at
+ *              positions from the same stream (Hz, Ez alternately) whose values feed one
+ *              writeSpectrum per world every step: 256 frequency bins over 145.5-2000 nm
+ *              for an 8000-step run.  All worlds step together with
+ *              updateWorldsE/updateWorldsH.
  *   metasurface an 8 x 8 array of TiO2 pillars (eps 6.25, 600 nm tall, radii drawn uniformly
  *              from 60-120 nm by a splitmix64 stream, seed 2024) on glass (eps 2.25) in a
  *              periodic cell of 8 x 400 nm: dx = 5 nm, Courant 0.5, periodic x and y, 16-cell
@@ -25,12 +28,17 @@
  *              x/y period in cells (640: 641 x 641 x 320 nodes including CPML, z edge half
  *              of it); the pitch stays 400 nm, so the array has cells/80 pillars a side.
  *
- * One step is updateE + updateH (+ updatePhaser for nanoblock), src/update.c:275-341.
+ * One step is updateE + updateH (+ updatePhaser for nanoblock; the probes and their spectra
+ * for dipole and kpoints), src/update.c:275-341.  Every probe value read is kept
+ * (wl_trace).
  */
 #include "alis.h"
 #include <string.h>
 #include <time.h>
 #include <stdint.h>
+#ifndef ALIS_REF
+#include "alis_dev.h"
+#endif
 
 typedef struct {
 	world W;
@@ -38,6 +46,9 @@ typedef struct {
 	int probes;
 	float pts[8][3];
 	double probeSum;
+	double *trace;              /* every probe value read, in order */
+	long ntrace, captrace;
+	float (*kprobe)[8][3];      /* kpoints: probe nodes per world */
 	int nw;                     /* worlds stepped together (W = Ws[0]) */
 	world *Ws;
 	object *parts;              /* objects the world refers to after putObjects */
@@ -52,6 +63,18 @@ static uint64_t splitmix(uint64_t *s)
 }
 
 static double uniform(uint64_t *s) { return (splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* Record one probe value: it enters the checksum and the trace. */
+static float keep(wl *c, float v)
+{
+	if (c->ntrace == c->captrace) {
+		c->captrace = c->captrace ? 2 * c->captrace : 4096;
+		c->trace = realloc(c->trace, c->captrace * sizeof(double));
+	}
+	c->trace[c->ntrace++] = v;
+	c->probeSum += v;
+	return v;
+}
 
 /* One k-point cell of the band sweep: q-th of nq points along Gamma-X-M. */
 static world kpointWorld(int q, int nq, uint64_t *seed)
@@ -150,6 +173,13 @@ void *wl_create(const char *name, int cells, int fieldBits, int dftBits)
 		c->Ws = calloc(cells, sizeof(world));
 		for (int q = 0; q < cells; q++) c->Ws[q] = kpointWorld(q, cells, &seed);
 		c->W = c->Ws[0];
+		c->kprobe = calloc(cells, sizeof *c->kprobe);
+		for (int q = 0; q < cells; q++)
+			for (int p = 0; p < 8; p++) {
+				c->kprobe[q][p][0] = (float)(uniform(&seed) * 600 - 300);
+				c->kprobe[q][p][1] = (float)(uniform(&seed) * 600 - 300);
+				c->kprobe[q][p][2] = (float)(uniform(&seed) * 200 - 100);
+			}
 	}
 	else {
 		free(c);
@@ -182,11 +212,53 @@ void wl_step(void *ctx, int n)
 		}
 		if (c->P) updatePhaser(c->W, c->P);
 		for (int p = 0; p < c->probes; p++) {
-			c->probeSum += get(c->W, Ez, c->pts[p][0], c->pts[p][1], c->pts[p][2]);
-			c->probeSum += get(c->W, Hy, c->pts[p][0], c->pts[p][1], c->pts[p][2]);
+			keep(c, get(c->W, Ez, c->pts[p][0], c->pts[p][1], c->pts[p][2]));
+			keep(c, get(c->W, Hy, c->pts[p][0], c->pts[p][1], c->pts[p][2]));
+		}
+		for (int q = 0; c->kprobe && q < c->nw; q++) {
+			world W = c->Ws[q];
+			float v[8];
+			for (int p = 0; p < 8; p++) v[p] = keep(c, get(W, p % 2 ? Ez : Hz, c->kprobe[q][p][0], c->kprobe[q][p][1], c->kprobe[q][p][2]));
+			writeSpectrum(W, 8000, 145.5, 2000, "kpoints-spectrum-%d", q, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
 		}
 	}
 }
+
+/* Probe values read by the last wl_step calls (in order) into out[0..n-1]; returns how many
+ * were read in total. */
+long wl_trace(void *ctx, double *out, long n)
+{
+	wl *c = ctx;
+	for (long a = 0; a < n && a < c->ntrace; a++) out[a] = c->trace[a];
+	return c->ntrace;
+}
+
+#ifndef ALIS_REF
+long alisSpectrumRead(world W, const char *name, double *out, long n);
+
+/* Raw sums of world q's probe spectrum (kpoints): the source row, then the 8 probes, 256 bins
+ * each as (re, im); returns the doubles written. */
+long wl_spectrum(void *ctx, int q, double *out, long n)
+{
+	wl *c = ctx;
+	char name[64];
+	if (!c->kprobe || q < 0 || q >= c->nw) return 0;
+	snprintf(name, sizeof name, "kpoints-spectrum-%d", q);
+	return alisSpectrumRead(c->Ws[q], name, out, n);
+}
+
+/* One field array of world q as doubles over the ghost-inclusive grid (alis_dev_read_field):
+ * h 0 E / 1 H, part m, axis a; returns the doubles written. */
+long wl_field(void *ctx, int q, int h, int m, int a, double *out, long n)
+{
+	wl *c = ctx;
+	if (q < 0 || q >= c->nw || !c->Ws[q]->dev) return 0;
+	world W = c->Ws[q];
+	long per = (long)(W->iNUM + 2) * (W->jNUM + 2) * (W->kNUM + 2);
+	if (n < per || m > W->complexField) return 0;
+	return alis_dev_read_field(W->dev, h, m, a, out) ? 0 : per;
+}
+#endif
 
 /* Grid nodes of the stepped worlds including CPML (the cell count of one step; a complex
  * cell of a Bloch world counts once). */
@@ -304,6 +376,8 @@ void wl_free(void *ctx)
 	for (int q = 0; q < c->nw; q++) deleteWorld(c->Ws[q]);
 	free(c->Ws);
 	free(c->parts);
+	free(c->trace);
+	free(c->kprobe);
 	free(c);
 }
 

@@ -79,16 +79,17 @@ def load():
     return alis, w, hip
 
 
-def curl_bytes(num, pml, s, has_eps):
+def curl_bytes(num, pml, s, eps_bytes):
     """Algorithmic HBM bytes of one curl launch (the compulsory traffic: every state element
-    read and written once, every coefficient read once): E from H reads E, H (and 1/eps, 3
-    values per cell) and writes E, 3 components each; H from E reads H, E and writes H.
+    read and written once, every coefficient read once): E from H reads E, H and writes E,
+    3 components each, plus `eps_bytes` per cell of 1/eps; H from E reads H, E and writes H.
     CPML: each slab cell reads and writes its two psi values.  Slabs are Num = pml-1 cells
     thick over the full cross section (src/update.c:119-124).  The single-world E curl reads
-    1/eps from a per-segment table where 64 cells along z hold one material, so it moves
-    fewer bytes than this (the engine's figure is reported beside it)."""
+    1/eps from a per-segment table where 64 cells along z hold one material (the engine's
+    figure, alis_dev_eps_bytes: 0.34 bytes per cell on the nanoblock); batched launches read
+    the three 1/eps arrays (3 s per cell)."""
     n = num[0] * num[1] * num[2]
-    e = n * (12 if has_eps else 9) * s
+    e = n * (9 * s + eps_bytes)
     h = n * 9 * s
     slab = 0
     for f in range(6):
@@ -154,7 +155,8 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None, f
 
 
 def roofline(r, s, dft_s, steps):
-    be, bh = curl_bytes(r["num"], r["pml"], s, r["eps"] > 0)
+    be, bh = curl_bytes(r["num"], r["pml"], s, r["eps_bytes"])
+    be_ref, _ = curl_bytes(r["num"], r["pml"], s, 3 * s if r["eps"] else 0)
     be, bh = be * r["worlds"] * r["parts"], bh * r["worlds"] * r["parts"]    # one launch covers all
     (mse, ne), (msh, nh) = r["kern"]
     te, th = mse / max(ne, 1) * 1e-3, msh / max(nh, 1) * 1e-3
@@ -171,6 +173,9 @@ def roofline(r, s, dft_s, steps):
         "other_curl": {"bytes_per_launch": bh if dom[0] == "E" else be,
                        "launch_ms": round((th if dom[0] == "E" else te) * 1e3, 5)},
         "eps_read_bytes_per_cell": round(r["eps_bytes"], 3),
+        # the same launch priced with the reference's representation (three 1/eps values per
+        # cell): an effective rate, not the bytes this kernel moves
+        "effective_gbs_three_eps_arrays": round(be_ref * r["worlds"] * r["parts"] / te / 1e9, 1) if te and dom[0] == "E" else None,
         "step_bytes": step_bytes,
         "step_frac": round(step_bytes / (r["sec"] / steps) / PEAK_HBM, 4),
     }
@@ -203,26 +208,45 @@ def measured_traffic(cells, fields):
     return m["E"]["traffic_bytes"]
 
 
-def cpu_baseline(cells, steps):
-    exe = ROOT / "oracle" / "_ref" / "wl_ref"
-    if not exe.exists():
-        return None
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = min(cores, 16)
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-    r = subprocess.run([str(exe), "nanoblock", str(cells), "2", str(steps)], capture_output=True, text=True,
-                       env=env, timeout=600)
-    if r.returncode:
+    return model, cores
+
+
+def cpu_baseline(cells, steps, runs=5):
+    """The reference's own C code (oracle/_ref/wl_ref: its sources compiled with its flags)
+    on the same workload, with OpenMP on every core of this process's affinity set: median
+    Mcell-updates/s of `runs` runs of `steps` steps each (after 2 warm-up steps)."""
+    exe = ROOT / "oracle" / "_ref" / "wl_ref"
+    if not exe.exists():
         return None
-    sec, ncell, threads = r.stdout.split()[:3]
-    sec, ncell = float(sec), int(ncell)
-    return {"value": round(ncell / sec / 1e6, 2), "unit": "Mcell-updates/s", "cores": int(threads),
-            "kind": "reference",
+    model, cores = host_cpu()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    vals, threads = [], cores
+    for _ in range(runs):
+        r = subprocess.run([str(exe), "nanoblock", str(cells), "2", str(steps)], capture_output=True, text=True,
+                           env=env, timeout=600)
+        if r.returncode:
+            return None
+        sec, ncell, threads = r.stdout.split()[:3]
+        vals.append(int(ncell) / float(sec) / 1e6)
+    vals.sort()
+    return {"value": round(vals[len(vals) // 2], 2), "unit": "Mcell-updates/s", "cores": int(threads),
+            "kind": "reference", "runs": [round(v, 1) for v in vals], "cpu": model, "nproc": os.cpu_count(),
             "sample": f"reference C (gcc -O3 -fopenmp, its own sources) on the same {cells}^3 nanoblock workload, "
-                      f"{steps} steps after 2 warm-up steps, OMP_NUM_THREADS={threads}"}
+                      f"median of {runs} runs of {steps} steps after 2 warm-up steps, OMP_NUM_THREADS={threads}"}
 
 
 def farfield_cpu_baseline(X, Y, cells=160):
@@ -233,7 +257,7 @@ def farfield_cpu_baseline(X, Y, cells=160):
     if not exe.exists():
         return None
     import tempfile
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    _, cores = host_cpu()
     env = dict(os.environ, OMP_NUM_THREADS=str(cores))
     with tempfile.TemporaryDirectory() as d:
         r = subprocess.run([str(exe), "metasurface", str(cells), "0", "0", str(X), str(Y), d],
@@ -277,7 +301,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cells", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=300, help="steps per CPU baseline run (median of 5 runs)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kpoints", type=int, default=64, help="worlds of the batched k-point sweep (0: skip)")
     ap.add_argument("--serve", type=int, default=24, help="requests of the serving run (0: skip)")
@@ -330,8 +354,10 @@ def main():
         v64 = r64["cells"] * max(args.steps // 2, 1) / r64["sec"] / 1e6
         line["fp64"] = {"value": round(v64, 1), "ms_per_step": round(r64["sec"] / max(args.steps // 2, 1) * 1e3, 4),
                         "roofline_frac": roofline(r64, 8, 8, max(args.steps // 2, 1))["frac"], "fp32_over_fp64": round(value / v64, 2)}
+    line["single_fp64"] = single_fp64(alis, w, hip, args)
     if args.kpoints > 0:
         line["batched"] = batched(alis, w, hip, args)
+        line["batched"]["batched_fp32_over_single_fp64"] = round(line["batched"]["value"] / line["single_fp64"]["value"], 2)
     if args.serve > 0:
         line["serving"] = serving(alis, args)
     if args.metasurface > 0:
@@ -340,25 +366,36 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def single_fp64(alis, w, hip, args):
+    """The plain FP64 single solve: the 64^3 dipole in vacuum (10-cell CPML, Ez point dipole,
+    8 probes reading Ez and Hy every step), FP64 fields and DFT, one world per launch."""
+    steps = max(args.steps, 200)
+    with quiet():
+        r = run(alis, w, hip, "dipole", 64, 64, 64, args.warmup, steps)
+    return {"workload": "dipole 64^3 (incl. 10-cell CPML), dx 10 nm, Courant 0.5, Ez dipole 600 nm pulse, 8 probes (Ez, Hy) per step",
+            "value": round(r["cells"] * steps / r["sec"] / 1e6, 1), "unit": "Mcell-updates/s", "dtype": "f64",
+            "ms_per_step": round(r["sec"] / steps * 1e3, 4), "steps": steps}
+
+
 def batched(alis, w, hip, args):
-    """The k-point band sweep: all worlds in shared launches, against one world alone in FP64."""
-    ksteps = max(args.steps // 2, 1)
+    """The k-point band sweep: all worlds in shared launches, with 8 probes per world read
+    every step into a 256-bin FP64 probe spectrum (writeSpectrum); at least 512 steps, so the
+    device adds the staged spectrum samples twice in the timed region."""
+    ksteps = max(args.steps // 2, 512)
     with quiet():
         rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
-        r1 = run(alis, w, hip, "kpoints", 1, 64, 64, args.warmup, ksteps)
     rb["eps_bytes"] = 3 * 4 if rb["eps"] else 0       # batched launches read the 1/eps arrays
     vb = rb["cells"] * ksteps / rb["sec"] / 1e6
-    v1 = r1["cells"] * ksteps / r1["sec"] / 1e6
     rlb = roofline(rb, 4, 8, ksteps)
     return {
         "workload": f"{args.kpoints} k-point worlds along Gamma-X-M, each 65x65x50 (64x64x31 cells + 8-cell CPML in z), "
-                    "Bloch-periodic x/y, complex fields, eps=12 PhC slab with air holes, 8 random-phase dipoles",
+                    "Bloch-periodic x/y, complex fields, eps=12 PhC slab with air holes, 8 random-phase dipoles, "
+                    "8 probes per world read every step into a 256-bin FP64 writeSpectrum",
         "value": round(vb, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rb["sec"] / ksteps * 1e3, 4),
+        "steps": ksteps,
         "roofline": {"bound": "hbm", "achieved": rlb["achieved"], "peak": rlb["peak"], "unit": "GB/s", "frac": rlb["frac"],
                      "kernel": "k_curl_batch (E half step of all worlds, both parts)", "bytes_per_launch": rlb["bytes_per_launch"],
                      "launch_ms": rlb["launch_ms"]},
-        "fp64_single_world": {"value": round(v1, 1), "ms_per_step": round(r1["sec"] / ksteps * 1e3, 4)},
-        "batched_fp32_over_single_fp64": round(vb / v1, 2),
     }
 
 
@@ -384,8 +421,9 @@ def metasurface(alis, w, hip, args):
     fc = farfield_cpu_baseline(X, 8) if args.cpu_steps > 0 else None
     if fc:
         scale = rm["num"][0] * rm["num"][1] / fc["face_cells"]
-        fc["estimated_sec_full"] = round(fc["sec_per_direction"] * scale * X * Y, 1)
-        fc["scaled_to"] = f"{rm['num'][0]}x{rm['num'][1]} face cells"
+        fc["extrapolated_sec_full"] = round(fc["sec_per_direction"] * scale * X * Y, 1)
+        fc["extrapolation"] = (f"measured per direction on {fc['face_cells']} face cells, scaled by face cells to "
+                               f"{rm['num'][0]}x{rm['num'][1]} and by {X}x{Y} directions (not measured at full size)")
         ms["farfield"]["cpu_baseline"] = fc
     return ms
 

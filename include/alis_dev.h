@@ -221,6 +221,19 @@ int alis_dev_batch_core(alis_dev_batch *B, int h);
 int alis_dev_batch_post(alis_dev_batch *B, int h);
 void alis_dev_batch_release(alis_dev_batch *B);
 
+/* Probe spectra: the running DFT of writeSpectrum (src/output.c:177-230, whose loop at
+ * :212-216 adds one sample of every probe to every frequency bin per call) in FP64 with
+ * compensated summation.  A spectrum has nprobe series and nbin bins; the sample of scaled
+ * time tau adds v[p] e^{i 2 pi tau (nTop - b)} to bin b of probe p.  Samples are staged on
+ * the host and added on the device in call order, 256 at a time, on a stream of their own
+ * (no world is involved); read flushes the staged samples and returns the sums as
+ * out[(p nbin + b) 2 + re/im].  Replaces the float complex accumulation of the reference. */
+typedef struct alis_dev_spectrum alis_dev_spectrum;
+int alis_dev_spectrum_create(int nprobe, int nbin, int nTop, alis_dev_spectrum **out);
+int alis_dev_spectrum_add(alis_dev_spectrum *S, double tau, const double *v);
+int alis_dev_spectrum_read(alis_dev_spectrum *S, double *out);
+void alis_dev_spectrum_release(alis_dev_spectrum *S);
+
 /* Wait for all work of D. */
 int alis_dev_synchronize(alis_dev_world *D);
 

@@ -633,6 +633,58 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 	return 0;
 }
 
+/* ---- probe spectra (src/output.c:177-230) ----
+ * The running DFT of writeSpectrum in double with compensated (Neumaier) summation: every
+ * sample is added to every bin as it arrives, in call order (the device stages samples and
+ * adds them later in the same order). */
+struct alis_dev_spectrum {
+	int np, nb, nTop;
+	double *sum, *cmp;          /* [probe][bin][re, im] */
+};
+
+static void neumaier(double *s, double *c, double x)
+{
+	const double t = *s + x;
+	*c += fabs(*s) >= fabs(x) ? (*s - t) + x : (x - t) + *s;
+	*s = t;
+}
+
+int alis_dev_spectrum_create(int nprobe, int nbin, int nTop, alis_dev_spectrum **out)
+{
+	if (nprobe < 1 || nbin < 1) return fail("spectrum: no probes or no frequency bins");
+	alis_dev_spectrum *S = calloc(1, sizeof *S);
+	S->np = nprobe, S->nb = nbin, S->nTop = nTop;
+	S->sum = calloc(2L * nprobe * nbin, sizeof(double));
+	S->cmp = calloc(2L * nprobe * nbin, sizeof(double));
+	*out = S;
+	return 0;
+}
+
+int alis_dev_spectrum_add(alis_dev_spectrum *S, double tau, const double *v)
+{
+	for (int b = 0; b < S->nb; b++) {
+		const double ph = 2 * M_PI * tau * (S->nTop - b), cs = cos(ph), sn = sin(ph);
+		for (int p = 0; p < S->np; p++) {
+			const long e = 2 * ((long)p * S->nb + b);
+			neumaier(&S->sum[e], &S->cmp[e], v[p] * cs);
+			neumaier(&S->sum[e + 1], &S->cmp[e + 1], v[p] * sn);
+		}
+	}
+	return 0;
+}
+
+int alis_dev_spectrum_read(alis_dev_spectrum *S, double *out)
+{
+	for (long e = 0; e < 2L * S->np * S->nb; e++) out[e] = S->sum[e] + S->cmp[e];
+	return 0;
+}
+
+void alis_dev_spectrum_release(alis_dev_spectrum *S)
+{
+	if (!S) return;
+	free(S->sum), free(S->cmp), free(S);
+}
+
 int alis_dev_synchronize(alis_dev_world *D) { (void)D; return 0; }
 int alis_dev_timing(alis_dev_world *D, int enable) { (void)D, (void)enable; return 0; }
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches) { (void)D, (void)h; *ms = 0, *launches = 0; return 0; }

@@ -1,7 +1,10 @@
 /*
  * Half-step arithmetic of the oracle, instantiated for T = float and T = double by
- * engine.c (SUF(name) appends the type suffix).  Plain sequential C loops in the
- * reference's order; every function cites the reference lines it restates.
+ * engine.c (SUF(name) appends the type suffix).  Plain C loops in the reference's order;
+ * every function cites the reference lines it restates.  Loops whose cells are independent
+ * (each iteration writes only its own cell and reads only arrays the loop does not write:
+ * the curl, the CPML faces, the object terms) run their outer index on OpenMP threads,
+ * which leaves every cell's arithmetic and order as in the sequential loop.
  */
 
 /* ---- curl and CPML: src/update.c:48-125 ---- */
@@ -25,6 +28,7 @@ static void SUF(cpmlFace)(world *D, int h, int m, int f, int alt, int hasC)
 	/* target components and the source components of their curl terms */
 	const int ta = (ax+1) % 3, tb = (ax+2) % 3;        /* x: (y,z)  y: (z,x)  z: (x,y) */
 	const int di = ax == 0 ? alt : 0, dj = ax == 1 ? alt : 0, dk = ax == 2 ? alt : 0;
+	#pragma omp parallel for schedule(static)
 	for (int i = i0; i <= i1; i++)
 		for (int j = j0; j <= j1; j++)
 			for (int k = k0; k <= k1; k++) {
@@ -48,6 +52,7 @@ static void SUF(curl)(world *D, int h, int m)
 	const int alt = h ? -1 : 1, hasC = !h && D->hasEps;
 	arr *F = D->F[h][m], *G = D->F[1-h][m];
 	const float *dx = W->dt[h][0], *dy = W->dt[h][1], *dz = W->dt[h][2];
+	#pragma omp parallel for schedule(static)
 	for (int i = W->iMIN; i <= W->iMAX; i++)
 		for (int j = W->jMIN; j <= W->jMAX; j++)
 			for (int k = W->kMIN; k <= W->kMAX; k++) {

@@ -15,8 +15,12 @@
 static FILE *OUT;
 static world LAST;      /* the (first) scenario's world and phaser array, for the raw dumps */
 static phaser LASTP;
-static world RUNW[8];   /* everything a run created, freed by releaseRun */
-static phaser RUNP[8];
+#define MAXSCEN 64
+static world RUNW[MAXSCEN];   /* everything a run created, freed by releaseRun */
+static phaser RUNP[MAXSCEN];
+static char OUTDIR[512] = ".";   /* where scenarios write their spectrum files */
+static int RUNSTEPS;            /* steps of the current run */
+static const char *RUNNAME[MAXSCEN];
 static int NRUN;
 
 /* Free the worlds of the last run.  Against the reference library they are left alive: its
@@ -244,6 +248,8 @@ typedef struct {
 	field *F[8];
 	float pts[8][3];
 	trace T;
+	int kq;                     /* k-point scenarios: their index, whose spectrum they write */
+	float spec[3][3];           /* and the nodes of their spectrum's probes */
 } scen;
 static void dipoleSetup(scen *c)
 {
@@ -396,10 +402,59 @@ static void multipoleSetup(scen *c)
 	memcpy(c->F, F, sizeof F);
 	memcpy(c->pts, pts, sizeof pts);
 }
+/* The q-th of NKP k-points along Gamma-X-M of a small photonic-crystal slab (the band
+ * sweep at small size): a cell of period 2a = 160 nm (a = 80 nm, dx = 10 nm) with
+ * Bloch-periodic x and y, an eps = 12 slab 40 nm thick with air holes of radius 0.3a on the
+ * square lattice of period a, 8-cell CPML above and below, complex fields.  Two point
+ * dipoles (Pulse 300 nm) at positions and phases drawn from a splitmix64 stream seeded by
+ * q; three probes feed writeSpectrum over 100-800 nm, written on the run's last step. */
+#define NKP 40
+static uint64_t splitmix(uint64_t *s)
+{
+	uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+	return z ^ (z >> 31);
+}
+static float unitDraw(uint64_t *s) { return (float)((splitmix(s) >> 11) * (1.0 / 9007199254740992.0)); }
+static void kpointSetup(scen *c, int q)
+{
+	const float a = 80, L = 2 * a;
+	float t = (q + 0.5f) / NKP * 2, kx, ky;
+	if (t < 1) kx = 0.5f * t, ky = 0;
+	else kx = 0.5f, ky = 0.5f * (t - 1);
+	dom D = {{-L/2, L/2}, {-L/2, L/2}, {-60, 60}};
+	res R = {5, {10}, {1240}};
+	sur S = {{BBC, a/kx}, {ky > 0 ? BBC : PBC, ky > 0 ? a/ky : INF}, {PML}, {8}};
+	world W = createWorld(D, R, S, "");
+	object Slab = {Difference, {2}, objects {
+		{Box, {{-INF, INF}, {-INF, INF}, {-20, 20}}},
+		{Lattice, {{a}, {a}}, objects {{RodZ, {{-24, 24}, {-24, 24}, {-INF, INF}}}}},
+	}};
+	putObjects(W, n(sqrtf(12)), Slab, Air);
+	uint64_t seed = 2024 + q;
+	for (int d = 0; d < 2; d++) {
+		float x = unitDraw(&seed) * 120 - 60, y = unitDraw(&seed) * 120 - 60, z = unitDraw(&seed) * 30 - 15;
+		pointDipole(W, d ? Ez : Hz, x, y, z, Pulse, 300, 200, 1, unitDraw(&seed) * 360);
+	}
+	field *F[3] = { Hz, iEz, Ex };
+	float pts[3][3] = {{20, -30, 0}, {-40, 10, 30}, {50, 50, -10}};
+	float spec[3][3] = {{30, 30, 0}, {-30, 0, 10}, {0, -50, -30}};
+	c->W = W, c->P = 0, c->np = 3, c->kq = q;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
+	memcpy(c->spec, spec, sizeof spec);
+}
 static int setupScenario(scen *c, const char *name)
 {
 	memset(c, 0, sizeof *c);
 	c->name = name;
+	c->kq = -1;
+	if (!strncmp(name, "kp", 2) && name[2] >= '0' && name[2] <= '9') {
+		int q = atoi(name + 2);
+		if (q < 0 || q >= NKP) return fprintf(stderr, "k-point %d outside 0..%d\n", q, NKP - 1), 2;
+		return kpointSetup(c, q), 0;
+	}
 	if (!strcmp(name, "dipole")) return dipoleSetup(c), 0;
 	if (!strcmp(name, "symband")) return symbandSetup(c), 0;
 	if (!strcmp(name, "nanoblock")) return nanoblockSetup(c), 0;
@@ -415,6 +470,35 @@ static void stepScenario(scen *c, int phaserDone)
 {
 	if (c->P && !phaserDone) updatePhaser(c->W, c->P);
 	sample(&c->T, c->W, c->F, c->pts, c->np);
+	if (c->kq >= 0) {
+		float (*p)[3] = c->spec;
+		writeSpectrum(c->W, RUNSTEPS, 100, 800, "%s/kp%d-spectrum", OUTDIR, c->kq, get(c->W, Hz, p[0][0], p[0][1], p[0][2]),
+			get(c->W, iHz, p[1][0], p[1][1], p[1][2]), get(c->W, Ez, p[2][0], p[2][1], p[2][2]));
+	}
+}
+
+/* The spectrum file a k-point scenario wrote on its last step: hash of its bytes and the sum
+ * of its numbers. */
+static void putSpectrum(scen *c)
+{
+	char path[700];
+	snprintf(path, sizeof path, "%s/kp%d-spectrum.txt", OUTDIR, c->kq);
+	FILE *f = fopen(path, "rb");
+	if (!f) {
+		fprintf(OUT, "%s.spectrum missing\n", c->name);
+		return;
+	}
+	static char text[1 << 16];
+	size_t n = fread(text, 1, sizeof text - 1, f);
+	fclose(f);
+	text[n] = 0;
+	double sum = 0;
+	for (char *p = text, *e; *p; p = e) {
+		double v = strtod(p, &e);
+		if (e == p) e = p + 1;
+		else sum += v;
+	}
+	fprintf(OUT, "%s.spectrum %016llx %.9g\n", c->name, (unsigned long long)fnv(1469598103934665603ULL, text, n), sum);
 }
 
 /* The far field is evaluated from stepped worlds only (on the device, which a zero-step
@@ -423,6 +507,7 @@ static void dumpScenario(scen *c, int stepped)
 {
 	if (c->P) sync(c->W);
 	putTrace(c->name, &c->T);
+	if (c->kq >= 0 && stepped) putSpectrum(c);
 	dumpState(c->name, c->W);
 	if (c->P) dumpPhasers(c->name, c->P, stepped);
 }
@@ -431,20 +516,30 @@ static void dumpScenario(scen *c, int stepped)
  * several step together with updateWorldsE/updateWorldsH (shared launches). */
 static int run(const char *names, int steps)
 {
-	scen c[8];
+	static scen c[MAXSCEN];
 	int n = 0;
-	char buf[256];
-	snprintf(buf, sizeof buf, "%s", names);
+	/* "kpoints" stands for the whole k-point sweep kp0,...,kp39 */
+	static char buf[4096];
+	buf[0] = 0;
+	char tmp[1024];
+	snprintf(tmp, sizeof tmp, "%s", names);
+	for (char *tok = strtok(tmp, ","); tok; tok = strtok(0, ",")) {
+		size_t l = strlen(buf);
+		if (!strcmp(tok, "kpoints"))
+			for (int q = 0; q < NKP; q++) l += snprintf(buf + l, sizeof buf - l, "%skp%d", l ? "," : "", q);
+		else snprintf(buf + l, sizeof buf - l, "%s%s", l ? "," : "", tok);
+	}
+	RUNSTEPS = steps;
 	for (char *tok = strtok(buf, ","); tok; tok = strtok(0, ",")) {
-		if (n == 8) return fprintf(stderr, "too many scenarios\n"), 2;
+		if (n == MAXSCEN) return fprintf(stderr, "too many scenarios\n"), 2;
 		if (setupScenario(&c[n], tok)) return 2;
-		static char label[8][32];
+		static char label[MAXSCEN][32];
 		snprintf(label[n], sizeof label[n], "%s", tok);
 		c[n].name = label[n];
 		dumpSetup(c[n].name, c[n].W);
 		n++;
 	}
-	world Ws[8];
+	world Ws[MAXSCEN];
 	for (int a = 0; a < n; a++) Ws[a] = c[a].W;
 	for (int s = 1; s <= steps; s++) {
 		if (n == 1) {
@@ -457,8 +552,8 @@ static int run(const char *names, int steps)
 			/* several scenarios: shared launches for the half steps and the phasers */
 			updateWorldsE(Ws, n);
 			updateWorldsH(Ws, n);
-			world PW[8];
-			phaser PP[8];
+			world PW[MAXSCEN];
+			phaser PP[MAXSCEN];
 			int np = 0;
 			for (int a = 0; a < n; a++) if (c[a].P) PW[np] = c[a].W, PP[np++] = c[a].P;
 			updatePhaserArrays(PW, PP, np);
@@ -470,14 +565,24 @@ static int run(const char *names, int steps)
 		for (int a = 0; a < n; a++) stepScenario(&c[a], phasersDone);
 	}
 	for (int a = 0; a < n; a++) dumpScenario(&c[a], steps > 0);
-	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P;
+	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P, RUNNAME[a] = c[a].name;
 	NRUN = n;
 	LAST = c[0].W, LASTP = c[0].P;
 	return 0;
 }
 
+/* Spectrum files go next to the output file. */
+static void outDir(const char *path)
+{
+	snprintf(OUTDIR, sizeof OUTDIR, "%s", path);
+	char *slash = strrchr(OUTDIR, '/');
+	if (slash) *slash = 0;
+	else snprintf(OUTDIR, sizeof OUTDIR, ".");
+}
+
 int alis_drive(const char *name, int steps, const char *path)
 {
+	outDir(path);
 	OUT = fopen(path, "w");
 	if (!OUT) return 1;
 	int rc = run(name, steps);
@@ -489,12 +594,18 @@ int alis_drive(const char *name, int steps, const char *path)
 
 #ifndef ALIS_REF
 #include "alis_dev.h"
-/* Run a scenario and write the final device state of its main world as doubles: for E then
- * H, every component of the real (and imaginary) part over the ghost-inclusive grid, then
- * the phaser accumulators if the scenario has phasers.  Header: 8 longs (iNUM, jNUM, kNUM,
- * complexField, field doubles, phaser doubles, 0, 0). */
+/* Raw sums of a writeSpectrum record (alis_amd/host/observers.c). */
+long alisSpectrumRead(world W, const char *name, double *out, long n);
+
+/* Run scenarios and write the final device state as doubles: for every world of the run, for
+ * E then H, every component of the real (and imaginary) part over the ghost-inclusive grid;
+ * then the first world's phaser accumulators if it has phasers; then the raw sums of every
+ * k-point world's spectrum (source row first, then each probe, bins as re/im).  Header: 8
+ * longs (iNUM, jNUM, kNUM, complexField of the first world, field doubles, phaser doubles,
+ * spectrum doubles, worlds). */
 int alis_drive_raw(const char *name, int steps, const char *path)
 {
+	outDir(path);
 	OUT = fopen("/dev/null", "w");
 	int rc = run(name, steps);
 	fclose(OUT);
@@ -503,8 +614,9 @@ int alis_drive_raw(const char *name, int steps, const char *path)
 	FILE *f = fopen(path, "wb");
 	if (!f) return 1;
 	world W = LAST;
-	long per = (long)(W->iNUM+2) * (W->jNUM+2) * (W->kNUM+2);
-	long nf = 6 * per * (1 + W->complexField);
+	long nf = 0;
+	for (int a = 0; a < NRUN; a++)
+		nf += 6L * (RUNW[a]->iNUM+2) * (RUNW[a]->jNUM+2) * (RUNW[a]->kNUM+2) * (1 + RUNW[a]->complexField);
 	long np = 0;
 	int pid = -1;
 	if (LASTP) {
@@ -512,22 +624,38 @@ int alis_drive_raw(const char *name, int steps, const char *path)
 		pid = B->devId;
 		if (pid >= 0) np = alis_dev_phaser_size(W->dev, pid);
 	}
-	long head[8] = { W->iNUM, W->jNUM, W->kNUM, W->complexField, nf, np > 0 ? np : 0, 0, 0 };
+	static double spec[1 << 16];
+	long ns = 0;
+	for (int a = 0; a < NRUN; a++) {
+		int q = -1;
+		if (sscanf(RUNNAME[a], "kp%d", &q) != 1) continue;
+		char nm[700];
+		snprintf(nm, sizeof nm, "%s/kp%d-spectrum", OUTDIR, q);
+		long got = alisSpectrumRead(RUNW[a], nm, spec + ns, (long)(sizeof spec / sizeof spec[0]) - ns);
+		if (got <= 0) rc = 5;
+		ns += got > 0 ? got : 0;
+	}
+	long head[8] = { W->iNUM, W->jNUM, W->kNUM, W->complexField, nf, np > 0 ? np : 0, ns, NRUN };
 	fwrite(head, sizeof head, 1, f);
-	double *buf = malloc(per * sizeof(double));
-	for (int h = 0; h < 2; h++)
-		for (int m = 0; m <= W->complexField; m++)
-			for (int a = 0; a < 3; a++) {
-				if (alis_dev_read_field(W->dev, h, m, a, buf)) rc = 4;
-				fwrite(buf, sizeof(double), per, f);
-			}
-	free(buf);
+	for (int a = 0; a < NRUN; a++) {
+		world V = RUNW[a];
+		long per = (long)(V->iNUM+2) * (V->jNUM+2) * (V->kNUM+2);
+		double *buf = malloc(per * sizeof(double));
+		for (int h = 0; h < 2; h++)
+			for (int m = 0; m <= V->complexField; m++)
+				for (int x = 0; x < 3; x++) {
+					if (alis_dev_read_field(V->dev, h, m, x, buf)) rc = 4;
+					fwrite(buf, sizeof(double), per, f);
+				}
+		free(buf);
+	}
 	if (np > 0) {
 		double *pb = malloc(np * sizeof(double));
 		if (alis_dev_phaser_read(W->dev, pid, pb, np)) rc = 4;
 		fwrite(pb, sizeof(double), np, f);
 		free(pb);
 	}
+	fwrite(spec, sizeof(double), ns, f);
 	fclose(f);
 	releaseRun();
 	return rc;

@@ -6,12 +6,14 @@ reference's own flags) into oracle/_ref/libref.a and links tests/golden/drive.c 
 tests/golden/<scenario>.txt, and manifest.json records the step counts and a SHA-256 of the
 reference sources the fixtures came from.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [SCENARIO ...]     (default: all)
 """
 import hashlib
 import json
+import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -28,6 +30,9 @@ SCENARIOS = {
     "hexagonal": 200,
     "multipole": 200,
     "pillars": 200,
+    # 40 Bloch k-point worlds of a photonic-crystal slab (kp0..kp39), stepped together on the
+    # GPU (80 batch entries): probes and writeSpectrum files
+    "kpoints": 200,
 }
 
 
@@ -36,10 +41,17 @@ def main():
         sys.exit("reference sources not found; fixtures can only be regenerated where they are")
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "ref", "ref-programs"], check=True)
     drive = ROOT / "oracle" / "_ref" / "drive_ref"
+    only = sys.argv[1:]
     for name, steps in SCENARIOS.items():
+        if only and name not in only:
+            continue
         out = HERE / f"{name}.txt"
-        subprocess.run([str(drive), name, str(steps), str(out)], check=True, stdout=subprocess.DEVNULL,
-                       env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"})
+        with tempfile.TemporaryDirectory() as d:
+            # scenarios write their spectrum files next to the output file
+            tmp = Path(d) / "out.txt"
+            subprocess.run([str(drive), name, str(steps), str(tmp)], check=True, stdout=subprocess.DEVNULL,
+                           env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"}, cwd=d)
+            shutil.copyfile(tmp, out)
         print(f"{name}: {sum(1 for _ in out.open())} lines")
     h = hashlib.sha256()
     for f in sorted(REF.glob("*.[ch]")):

@@ -15,6 +15,25 @@ GOLDEN = ROOT / "tests" / "golden"
 MANIFEST = json.loads((GOLDEN / "manifest.json").read_text())
 SCENARIOS = MANIFEST["scenarios"]
 WITH_PHASERS = ("symband", "nanoblock", "oblique", "pillars")
+# "kpoints" runs the 40 worlds kp0..kp39 of the k-point sweep together; their output keys
+# start with their own names
+KPOINTS = tuple(f"kp{q}" for q in range(40))
+
+
+def members(name):
+    """The world names whose output lines a scenario name produces."""
+    return KPOINTS if name == "kpoints" else (name,)
+
+
+def split_run(got, names):
+    """The lines of a run of several scenarios, per scenario name."""
+    owner = {m: n for n in names for m in members(n)}
+    out = {n: {} for n in names}
+    for k, v in got.items():
+        n = owner.get(k.split(".")[0].split("#")[0])
+        if n is not None:
+            out[n][k] = v
+    return out
 
 LIBS = {"hip": GOLDEN / "libdrive.so", "oracle": GOLDEN / "libdrive_oracle.so"}
 _loaded = {}
@@ -46,8 +65,9 @@ def run_scenario(kind, name, steps, precision):
         return parse(out.read_text())
 
 
-def run_raw(kind, name, steps, precision):
-    """Final device state of a scenario as doubles: (fields, phaser accumulators)."""
+def run_raw(kind, name, steps, precision, spectra=False):
+    """Final device state of a scenario as doubles: (fields of every world, the first world's
+    phaser accumulators[, the raw sums of every k-point world's probe spectrum])."""
     os.environ["ALIS_PRECISION"] = precision
     with tempfile.TemporaryDirectory() as d:
         out = Path(d) / "raw.bin"
@@ -56,7 +76,10 @@ def run_raw(kind, name, steps, precision):
         b = out.read_bytes()
     head = np.frombuffer(b[:64], dtype=np.int64)
     data = np.frombuffer(b[64:], dtype=np.float64)
-    return data[:head[4]].copy(), data[head[4]:head[4] + head[5]].copy()
+    fields, phaser = data[:head[4]].copy(), data[head[4]:head[4] + head[5]].copy()
+    if spectra:
+        return fields, phaser, data[head[4] + head[5]:head[4] + head[5] + head[6]].copy()
+    return fields, phaser
 
 
 # directions of the far-field checks (degrees): normal, oblique, both sides of the 90-degree
@@ -113,7 +136,8 @@ def compare(ref, got, phaser_exact=True, post_tol=1e-4, phaser_tol=1e-5):
     phaser_exact) phaser accumulators must be identical; flux and far-field values are
     computed in double here and in float by the reference, so they agree to post_tol
     relative to the largest magnitude on their line.  Without phaser_exact (FP64
-    accumulators) the accumulators' sums of squares agree to phaser_tol."""
+    accumulators) the accumulators' sums of squares agree to phaser_tol, and the numbers of a
+    spectrum file (writeSpectrum) sum to within 1e-4."""
     bad = []
     if set(ref) != set(got):
         bad.append(f"keys differ: only in fixture {sorted(set(ref) - set(got))[:5]}, only in run {sorted(set(got) - set(ref))[:5]}")
@@ -124,6 +148,11 @@ def compare(ref, got, phaser_exact=True, post_tol=1e-4, phaser_tol=1e-5):
             scale = max(np.abs(x).max(), np.abs(y).max(), 1e-300)
             if x.shape != y.shape or np.abs(x - y).max() > post_tol * scale:
                 bad.append(f"{key}: {a} vs {b}")
+        elif not phaser_exact and key.endswith(".spectrum"):
+            # the spectrum file's numbers (six digits) from double instead of float sums
+            x, y = float(a[1]), float(b[1])
+            if not abs(x - y) <= 1e-4 * max(abs(x), 1e-300):
+                bad.append(f"{key}: sum of the file's numbers {x} vs {y}")
         elif not phaser_exact and _PHASER_DATA.search(key):
             x, y = float(a[1]), float(b[1])
             if abs(x - y) > phaser_tol * max(abs(x), 1e-300):
@@ -134,7 +163,7 @@ def compare(ref, got, phaser_exact=True, post_tol=1e-4, phaser_tol=1e-5):
 
 
 # Keys written before the first time step (setup arrays, extents, coefficients).
-_SETUP = re.compile(r"^[a-z]+(\.aux\d+)*\.(grid|phase|clock|dt[EH]\.[xyz]|cpml[EH]\.[xyz](#\d)?|eps\.[xyz]|obj\d+.*|src\d+.*|tfsf\d+)$")
+_SETUP = re.compile(r"^[a-z]+\d*(\.aux\d+)*\.(grid|phase|clock|dt[EH]\.[xyz]|cpml[EH]\.[xyz](#\d)?|eps\.[xyz]|obj\d+.*|src\d+.*|tfsf\d+)$")
 
 
 def setup_keys(d):

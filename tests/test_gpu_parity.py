@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from scenarios import ROOT, SCENARIOS, WITH_PHASERS, compare, dispersion, golden, run_raw, run_scenario
+from scenarios import split_run, ROOT, SCENARIOS, WITH_PHASERS, compare, dispersion, golden, run_raw, run_scenario
 
 pytestmark = pytest.mark.gpu
 
@@ -48,24 +48,31 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("name", sorted(SCENARIOS))
 def test_fp64_matches_fp64_oracle(name):
+    """FP64 worlds: every world's final state, the phaser accumulators and (k-point worlds)
+    the probe spectra of writeSpectrum within 1e-12 of the FP64 oracle."""
     print(f"[gpu] {name} fp64 vs oracle", flush=True)
-    fg, pg = run_raw("hip", name, SCENARIOS[name], "fp64")
-    fo, po = run_raw("oracle", name, SCENARIOS[name], "fp64")
-    assert fg.shape == fo.shape and pg.shape == po.shape
+    fg, pg, sg = run_raw("hip", name, SCENARIOS[name], "fp64", spectra=True)
+    fo, po, so = run_raw("oracle", name, SCENARIOS[name], "fp64", spectra=True)
+    assert fg.shape == fo.shape and pg.shape == po.shape and sg.shape == so.shape
     assert _rel(fg, fo) <= 1e-12, _rel(fg, fo)
     if po.size:
         assert _rel(pg, po) <= 1e-12, _rel(pg, po)
+    if name == "kpoints":
+        assert so.size and np.abs(so).max() > 0
+        assert _rel(sg, so) <= 1e-12, _rel(sg, so)
 
 
-@pytest.mark.parametrize("name", WITH_PHASERS)
+@pytest.mark.parametrize("name", WITH_PHASERS + ("kpoints",))
 def test_fp32_spectra_vs_fp64_oracle(name):
-    """The production precision: DFT spectra within 1e-5 of the FP64 oracle."""
+    """The production precision: DFT spectra (phaser accumulators; writeSpectrum sums of the
+    k-point worlds) within 1e-5 of the FP64 oracle."""
     print(f"[gpu] {name} fp32 spectra vs fp64 oracle", flush=True)
-    fg, pg = run_raw("hip", name, SCENARIOS[name], "fp32")
-    fo, po = run_raw("oracle", name, SCENARIOS[name], "fp64")
-    err_f, err_p = _rel(fg, fo), _rel(pg, po)
+    fg, pg, sg = run_raw("hip", name, SCENARIOS[name], "fp32", spectra=True)
+    fo, po, so = run_raw("oracle", name, SCENARIOS[name], "fp64", spectra=True)
+    err_f = _rel(fg, fo)
+    err_p = _rel(pg, po) if po.size else _rel(sg, so)
     print(f"[gpu] {name}: fp32 fields err {err_f:.2e}, spectra err {err_p:.2e} (vs fp64 oracle)")
-    assert err_p <= 1e-5
+    assert (po.size or so.size) and err_p <= 1e-5
 
 
 @pytest.mark.parametrize("cells,m,p", [(17, 3, 2), (21, 8, 1)])
@@ -116,8 +123,6 @@ def test_full_size_fp64_tracks_fp32():
     assert s32 > 0 and abs(s64 - s32) <= 1e-4 * s32
 
 
-def _split(got, names):
-    return {n: {k: v for k, v in got.items() if k.split(".")[0].split("#")[0] == n} for n in names}
 
 
 @pytest.mark.parametrize("precision", ["ref", "fp32"])
@@ -129,7 +134,7 @@ def test_batch_equals_alone(precision):
     names = sorted(SCENARIOS)
     steps = SCENARIOS[names[0]]
     print(f"[gpu] batch of {len(names)} scenarios, {precision}", flush=True)
-    mix = _split(run_scenario("hip", ",".join(names), steps, precision), names)
+    mix = split_run(run_scenario("hip", ",".join(names), steps, precision), names)
     for n in names:
         alone = run_scenario("hip", n, steps, precision)
         assert mix[n] == alone, n
@@ -164,6 +169,6 @@ def test_phaser_read_between_flushes(name):
     fo, po = run_raw("oracle", name, 13, "ref")
     assert pg.size and np.array_equal(pg, po) and np.array_equal(fg, fo)
     names = sorted(WITH_PHASERS)
-    mix = _split(run_scenario("hip", ",".join(names), 13, "ref"), names)
+    mix = split_run(run_scenario("hip", ",".join(names), 13, "ref"), names)
     bad = compare(run_scenario("oracle", name, 13, "ref"), mix[name])
     assert not bad, "\n".join(bad[:10])

@@ -5,7 +5,7 @@ auxiliary worlds, probe traces, phaser accumulators); flux and far field, evalua
 double here, agree to 1e-4 of their scale."""
 import pytest
 
-from scenarios import SCENARIOS, compare, golden, run_scenario
+from scenarios import split_run, SCENARIOS, compare, golden, run_scenario
 
 
 @pytest.mark.parametrize("name", sorted(SCENARIOS))
@@ -34,15 +34,13 @@ def test_fixture_catches_planted_mistake(tmp_path):
     assert compare(ref, got)
 
 
-def _split(got, names):
-    return {n: {k: v for k, v in got.items() if k.split(".")[0].split("#")[0] == n} for n in names}
 
 
 def test_oracle_batch_matches_reference():
     """All scenarios stepped together (updateWorldsE/updateWorldsH over the batch ABI) give
     each scenario's own reference output."""
     names = sorted(SCENARIOS)
-    got = _split(run_scenario("oracle", ",".join(names), SCENARIOS[names[0]], "ref"), names)
+    got = split_run(run_scenario("oracle", ",".join(names), SCENARIOS[names[0]], "ref"), names)
     for n in names:
         bad = compare(golden(n), got[n])
         assert not bad, n + ": " + "\n".join(bad[:10])
