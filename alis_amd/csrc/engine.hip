@@ -875,6 +875,7 @@ struct alis_dev_world {
 	bool timing = false;
 	hipEvent_t halfDone = nullptr;  // the end of the last postHalf (bound to its last launch)
 	std::vector<hipEvent_t> evPairs[2];
+	std::vector<hipEvent_t> evFree;  // timing events already read, for reuse
 	double evMs[2] = {0, 0};
 	long evCount[2] = {0, 0};
 };
@@ -1431,13 +1432,22 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 	return a;
 }
 
+// A pair of timing events of D: reused ones where alis_dev_timing_read has read them.
+int timingEvents(alis_dev_world *D, hipEvent_t &e0, hipEvent_t &e1)
+{
+	for (hipEvent_t *e : {&e0, &e1}) {
+		if (!D->evFree.empty()) *e = D->evFree.back(), D->evFree.pop_back();
+		else HIPCHECK(hipEventCreate(e));
+	}
+	return 0;
+}
+
 template<class T>
 int coreHalf(alis_dev_world *D, int h)
 {
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (D->timing) {
-		HIPCHECK(hipEventCreate(&e0));
-		HIPCHECK(hipEventCreate(&e1));
+		if (timingEvents(D, e0, e1)) return -1;
 	}
 	dim3 grid((D->kNUM + 63) / 64, (D->jNUM + 3) / 4, D->iNUM), block(64, 4);
 	// timing events are bound to the launches themselves (their start and end), so timing
@@ -2087,8 +2097,7 @@ int batchCore(alis_dev_batch *B, int h)
 	alis_dev_world *D0 = B->W[0];
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (D0->timing) {
-		HIPCHECK(hipEventCreate(&e0));
-		HIPCHECK(hipEventCreate(&e1));
+		if (timingEvents(D0, e0, e1)) return -1;
 	}
 	const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h];
 	const int *start = (const int *)B->curlStart[h];
@@ -2226,7 +2235,8 @@ void alis_dev_release(alis_dev_world *D)
 			hipHostFree(P.exPinned), hipEventDestroy(P.exDone);
 		}
 	if (D->halfDone) hipEventDestroy(D->halfDone);
-	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
+	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) (void)hipEventDestroy(e);
+	for (hipEvent_t e : D->evFree) (void)hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
 	for (Staging *S : {&D->phsStage, &D->phfStage}) {
@@ -2272,8 +2282,8 @@ int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int
 	return D->fp64 ? syncHost<double>(D, b) : syncHost<float>(D, b);
 }
 
-// Probe copies (alis_dev_sync_boxes): the segments of E, H (every part) and of the currents
-// of dispersive objects inside each box, for every world, gathered by one launch into one
+// Probe copies (alis_dev_sync_boxes): the segments of the selected field arrays and of the
+// currents of dispersive objects inside each box, for every world, gathered by one launch into one
 // buffer and copied by one transfer into pinned memory; the host then scatters the values
 // into the descriptors' arrays.  The segment table is uploaded only when it changes (a
 // program reads the same probes every step).
@@ -2290,7 +2300,7 @@ struct GatherState {
 };
 GatherState g_gather;
 
-void gatherWorld(alis_dev_world *D, const Box &box, std::vector<GatherSeg> &seg, std::vector<GatherHost> &out, long &total)
+void gatherWorld(alis_dev_world *D, const Box &box, int mask, std::vector<GatherSeg> &seg, std::vector<GatherHost> &out, long &total)
 {
 	const alis_dev_world_desc *W = &D->W;
 	const Box fb = fieldBox(D);
@@ -2304,8 +2314,9 @@ void gatherWorld(alis_dev_world *D, const Box &box, std::vector<GatherSeg> &seg,
 	            std::max(box.k0, fb.k0), std::min(box.k1, fb.k1)};
 	for (int h = 0; h < 2; h++)
 		for (int m = 0; m <= D->cplx; m++)
-			for (int a = 0; a < 3; a++) add(D->F[h][m][a], D->L, b, W->field[h][m][a]);
-	for (size_t n = 0; n < D->obj.size(); n++) {
+			for (int a = 0; a < 3; a++)
+				if (mask >> (h * 6 + m * 3 + a) & 1) add(D->F[h][m][a], D->L, b, W->field[h][m][a]);
+	for (size_t n = 0; (mask & ALIS_DEV_PROBE_CURRENTS) && n < D->obj.size(); n++) {
 		const DevObj &O = D->obj[n];
 		const Box c{std::max(box.i0, O.box.i0), std::min(box.i1, O.box.i1), std::max(box.j0, O.box.j0), std::min(box.j1, O.box.j1),
 		            std::max(box.k0, O.box.k0), std::min(box.k1, O.box.k1)};
@@ -2324,8 +2335,8 @@ int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 	std::vector<GatherHost> out;
 	long total = 0;
 	for (int e = 0; e < n; e++) {
-		const int *b = box + 6 * e;
-		gatherWorld(D[e], Box{b[0], b[1], b[2], b[3], b[4], b[5]}, seg, out, total);
+		const int *b = box + 7 * e;
+		gatherWorld(D[e], Box{b[0], b[1], b[2], b[3], b[4], b[5]}, b[6], seg, out, total);
 	}
 	if (!total) return 0;
 	GatherState &G = g_gather;
@@ -3094,11 +3105,18 @@ int alis_dev_timing(alis_dev_world *D, int enable)
 {
 	HIPCHECK(hipStreamSynchronize(D->st));
 	for (int h = 0; h < 2; h++) {
-		for (hipEvent_t e : D->evPairs[h]) hipEventDestroy(e);
+		D->evFree.insert(D->evFree.end(), D->evPairs[h].begin(), D->evPairs[h].end());
 		D->evPairs[h].clear();
 		D->evMs[h] = 0, D->evCount[h] = 0;
 	}
 	D->timing = enable != 0;
+	// events for the launches to come, made before them (creating one costs a few
+	// microseconds of host time, which would otherwise fall between a small world's launches)
+	while (D->timing && D->evFree.size() < 4096) {
+		hipEvent_t e;
+		HIPCHECK(hipEventCreate(&e));
+		D->evFree.push_back(e);
+	}
 	return 0;
 }
 
@@ -3111,8 +3129,8 @@ int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches)
 		HIPCHECK(hipEventElapsedTime(&t, v[a], v[a + 1]));
 		D->evMs[h] += t;
 		D->evCount[h]++;
-		hipEventDestroy(v[a]);
-		hipEventDestroy(v[a + 1]);
+		D->evFree.push_back(v[a]);
+		D->evFree.push_back(v[a + 1]);
 	}
 	v.clear();
 	*ms = D->evMs[h];

@@ -19,6 +19,18 @@ int fieldComponent(field *F, int *isH, int *m, int *axis)
 	return 0;
 }
 
+/* The same for the single-sample accessors rawEx .. rawHz (real part). */
+int rawComponent(field *F, int *isH, int *m, int *axis)
+{
+	field *tab[6] = { rawEx, rawEy, rawEz, rawHx, rawHy, rawHz };
+	for (int a = 0; a < 6; a++)
+		if (F == tab[a]) {
+			*isH = a >= 3, *m = 0, *axis = a % 3;
+			return 1;
+		}
+	return 0;
+}
+
 /* Pole susceptibility and its frequency derivative, real parts (src/fields.c:17-35). */
 static float poleFF(matter M, float w, int p)
 {

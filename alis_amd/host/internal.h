@@ -24,7 +24,7 @@ void alisCommit(world W);          /* create the device world if absent         
 void alisInvalidate(world W);      /* setup changed: bring state home, drop the device */
 void alisDropDevice(world W);      /* host arrays are newer: drop the device copy      */
 void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1);
-void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1);   /* get(): box copy, batched per half step */
+void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask);   /* get(): box copy of the arrays in mask, batched per half step */
 void alisStepped(void);            /* device fields changed: probe copies are stale */
 void alisSyncPhaser(phaser P);
 void alisPhaserDevice(phaser B);   /* register the array B on the device if it is not */
@@ -50,5 +50,6 @@ void sourceHalfStep(world W, source S, float t, int h);  /* evaluate and inject 
 
 /* fields.c: field identity helpers used by sources and probes */
 int fieldComponent(field *F, int *isH, int *m, int *axis);
+int rawComponent(field *F, int *isH, int *m, int *axis);
 
 #endif

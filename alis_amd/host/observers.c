@@ -52,10 +52,14 @@ float get(world W, field F, float x, float y, float z)
 		int readsAux = F==SoEx || F==SoEy || F==SoEz || F==SoEE || F==ScEx || F==ScEy || F==ScEz || F==ScEE;
 		if (readsAux || W->isAux) syncWorld(W);
 		else {
-			/* the accessors read within two cells of the node, mirrored across a symmetry
-			 * plane when its index is negative */
+			/* the node, mirrored across a symmetry plane when its index is negative (as the
+			 * accessors do): the component accessors read one array at the node and its upper
+			 * neighbours; the others read within two cells of it, in any array */
 			int ci = i < 0 && W->xMinSur==SYM ? -i : i, cj = j < 0 && W->yMinSur==SYM ? -j : j, ck = k < 0 && W->zMinSur==SYM ? -k : k;
-			alisProbe(W, ci-2, ci+2, cj-2, cj+2, ck-2, ck+2);
+			int isH, m, axis;
+			if (fieldComponent(F, &isH, &m, &axis) || rawComponent(F, &isH, &m, &axis))
+				alisProbe(W, ci, ci+1, cj, cj+1, ck, ck+1, 1 << (isH * 6 + m * 3 + axis));
+			else alisProbe(W, ci-2, ci+2, cj-2, cj+2, ck-2, ck+2, ALIS_DEV_PROBE_ALL);
 		}
 	}
 	return F(W, i, j, k);

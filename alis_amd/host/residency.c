@@ -234,7 +234,7 @@ void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 typedef struct {
 	world W;
 	int n;
-	int (*box)[6];
+	int (*box)[7];                  /* i0 i1 j0 j1 k0 k1, arrays (alis_dev_sync_boxes) */
 	unsigned long *epoch;
 } probeSet;
 static probeSet *probeSets;
@@ -253,7 +253,7 @@ static void forgetProbes(world W)
 		}
 }
 
-void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1)
+void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask)
 {
 	if (!W->dev || !W->hostStale) return;
 	probeSet *P = 0;
@@ -263,7 +263,7 @@ void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 		P = &probeSets[nProbeSets++];
 		*P = (probeSet) {W, 0, malloc(MAXPROBES * sizeof *P->box), malloc(MAXPROBES * sizeof *P->epoch)};
 	}
-	const int b[6] = {i0, i1, j0, j1, k0, k1};
+	const int b[7] = {i0, i1, j0, j1, k0, k1, mask};
 	int at = -1;
 	for (int q = 0; q < P->n && at < 0; q++) if (!memcmp(P->box[q], b, sizeof b)) at = q;
 	if (at < 0) {
@@ -281,7 +281,7 @@ void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 	for (int a = 0; a < nProbeSets; a++)
 		if (probeSets[a].W->dev && probeSets[a].W->hostStale) m += probeSets[a].n;
 	alis_dev_world **D = malloc(m * sizeof *D);
-	int *boxes = malloc(6 * m * sizeof(int));
+	int *boxes = malloc(7 * m * sizeof(int));
 	m = 0;
 	for (int a = 0; a < nProbeSets; a++) {
 		probeSet *Q = &probeSets[a];
@@ -289,7 +289,7 @@ void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 		for (int q = 0; q < Q->n; q++) {
 			if (Q->epoch[q] == epochNow) continue;
 			D[m] = Q->W->dev;
-			memcpy(boxes + 6 * m, Q->box[q], sizeof b);
+			memcpy(boxes + 7 * m, Q->box[q], sizeof b);
 			Q->epoch[q] = epochNow;
 			m++;
 		}

@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--serve", type=int, default=24, help="requests of the serving run (0: skip)")
     ap.add_argument("--metasurface", type=int, default=640, help="x/y period of the metasurface run in cells (0: skip)")
     ap.add_argument("--ff-angles", type=int, default=181, help="far-field map of N x N directions")
-    ap.add_argument("--only", choices=["kpoints", "serving", "metasurface"],
+    ap.add_argument("--only", choices=["kpoints", "serving", "metasurface", "single"],
                     help="run this section alone and print its object (for profiling)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-seed", type=int, default=2024)
@@ -328,7 +328,8 @@ def main():
     if args.only:
         # one section alone (for profiling its kernels): its object as the JSON line
         out = {"kpoints": lambda: batched(alis, w, hip, args), "serving": lambda: serving(alis, args),
-               "metasurface": lambda: metasurface(alis, w, hip, args)}[args.only]()
+               "metasurface": lambda: metasurface(alis, w, hip, args),
+               "single": lambda: single_fp64(alis, w, hip, args)}[args.only]()
         print(json.dumps({"section": args.only, **out}), flush=True)
         return
     with quiet():

@@ -120,10 +120,13 @@ int alis_dev_ghosts(alis_dev_world *D, int h);
  * ADE currents) when i0 > i1, else E/H inside the box and the currents of objects meeting it. */
 int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1);
 
-/* Probe copies: E and H (every part) inside box[6e..6e+5] = {i0,i1,j0,j1,k0,k1} of world
- * D[e], and the currents of its dispersive objects inside that box, to the host arrays of
- * the descriptor, for e = 0..n-1: one gather launch and one copy per world, one wait per
- * stream.  Serves get() (src/output.c:50-53), which reads a few cells per call. */
+/* Probe copies: for e = 0..n-1, the cells of box[7e..7e+5] = {i0,i1,j0,j1,k0,k1} of world
+ * D[e] in the arrays that box[7e+6] selects -- bit h*6 + m*3 + axis for field array
+ * (E or H, part m, axis), ALIS_DEV_PROBE_CURRENTS for the currents of its dispersive
+ * objects -- to the host arrays of the descriptor: one gather launch and one copy for all
+ * of them, one wait.  Serves get() (src/output.c:50-53), which reads a few cells per call. */
+#define ALIS_DEV_PROBE_CURRENTS (1 << 12)
+#define ALIS_DEV_PROBE_ALL ((1 << 13) - 1)
 int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n);
 
 /* Read / overwrite one whole field array (h: 0 E, 1 H; m: real/imag; axis 0..2) as doubles

@@ -419,8 +419,9 @@ int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const doub
 
 int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 {
+	/* every array inside the box: a superset of the selected ones */
 	for (int e = 0; e < n; e++) {
-		const int *b = box + 6 * e;
+		const int *b = box + 7 * e;
 		if (alis_dev_sync(D[e], b[0], b[1], b[2], b[3], b[4], b[5])) return -1;
 	}
 	return 0;
