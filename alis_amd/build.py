@@ -61,7 +61,7 @@ def build_library():
     # a program built from the reference's sources with gcc links; the system libgcc_s may
     # be newer and round float complex quotients differently.  -Bsymbolic: the library's
     # own calls bind inside it, also when another alis.h implementation is loaded.
-    run(["g++", "-shared", "-static-libgcc", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-fopenmp", "-lz", f"-L{ROCM/'lib'}", "-lamdhip64", "-lrocblas",
+    run(["g++", "-shared", "-static-libgcc", "-Wl,-Bsymbolic", "-o", str(tmp), *objs, "-fopenmp", "-lz", f"-L{ROCM/'lib'}", "-lamdhip64",
          f"-Wl,-rpath,{ROCM/'lib'}", "-Wl,--no-undefined"])
     os.replace(tmp, out)
     clients(PKG, "libalis.so", "alis")

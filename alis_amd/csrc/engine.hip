@@ -24,7 +24,6 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
-#include <rocblas/rocblas.h>
 
 #include "alis_dev.h"
 
@@ -865,7 +864,6 @@ struct alis_dev_world {
 	hipStream_t st = nullptr;   // shared by a world and its TF/SF auxiliary worlds
 	int *streamRefs = nullptr;  // worlds using st; the last one destroys it
 	Staging phsStage, phfStage;      // staging of batched phaser samples and flushes led by this world
-	rocblas_handle blas = nullptr;   // far-field products, created on first use
 	void *ffWork = nullptr;          // far-field workspace
 	size_t ffWorkBytes = 0;
 	void *scratch = nullptr;    // pack/unpack buffer (double-sized)
@@ -2244,7 +2242,6 @@ void alis_dev_release(alis_dev_world *D)
 		if (S->dev) hipFree(S->dev);
 		if (S->done) hipEventDestroy(S->done);
 	}
-	if (D->blas) rocblas_destroy_handle(D->blas);
 	if (D->st && --*D->streamRefs == 0) {
 		hipStreamDestroy(D->st);
 		delete D->streamRefs;
@@ -2650,7 +2647,7 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 //
 // For a face F[u][w] (u its first axis, w its last, k-fastest like the accumulators) and a
 // batch of columns, S(col) = sum_v alpha_v(col) sum_p F[.][.] beta_p(col): beta runs along
-// the face's in-plane phase axis p and is contracted with F by a complex GEMM (rocBLAS),
+// the face's in-plane phase axis p and is contracted with F by a complex GEMM (k_ff_gemm),
 // alpha runs along the other axis v and is applied by a reduction kernel.  z faces: p = y,
 // v = x, both phase vectors.  x faces: p = y, v = z; y faces: p = x, v = z, where alpha is
 // the plane-wave profile T(z) and depends on the polarisation.  A beta matrix is shared by
@@ -2778,6 +2775,100 @@ __global__ void k_ff_finish(double2 *S, const double2 *part, FFReduce r)
 		}
 }
 
+// The face contraction C_m[v][col] = sum_p A_m[v][p] B[p][col] of complex doubles on FP64
+// MFMA (v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16], and slot r of
+// its result holds D[l/16 + 4r][l%16]).  A_m is a face component F stored with p fastest
+// (trans = 0: A[v][p] = F[v np + p]) or v fastest (trans = 1: A[v][p] = F[p nv + v]); B is
+// beta[p][col].  A workgroup of four wavefronts computes a 64 x 64 tile of one matrix m
+// (blockIdx.z); each wavefront a 32 x 32 quarter as 2 x 2 tiles of 16 x 16, real and
+// imaginary parts in their own accumulators: Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br.  The
+// k loop stages 16 values of p of A and B in LDS per round (zero beyond the matrix edges).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int FFG_T = 64, FFG_K = 16;
+
+struct FFGemm {
+	const double2 *A[4];       // the face components of the launch's matrices
+	double2 *C[4];             // their products [v][col]
+	const double2 *B;          // beta [p][col]
+	int nv, np, nc, trans;
+};
+
+__global__ __launch_bounds__(256) void k_ff_gemm(FFGemm g)
+{
+	__shared__ double Ar[FFG_T][FFG_K + 1], Ai[FFG_T][FFG_K + 1];
+	__shared__ double Br[FFG_K][FFG_T + 1], Bi[FFG_K][FFG_T + 1];
+	const int m = blockIdx.z, t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+	const int v0 = (int)blockIdx.y * FFG_T, c0 = (int)blockIdx.x * FFG_T;
+	const double2 *A = g.A[m];
+	const int wv = (wave >> 1) * 32, wc = (wave & 1) * 32;
+	v4d cr[2][2], ci[2][2];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++) cr[a][b] = v4d{0, 0, 0, 0}, ci[a][b] = v4d{0, 0, 0, 0};
+	// A (64 rows v x 16 p) and B (16 rows p x 64 cols) of the round at p0, one complex of each
+	// per thread and pass, fetched into registers one round ahead of their use
+	double2 xa[4], xb[4];
+	auto fetch = [&](int p0) {
+#pragma unroll
+		for (int pass = 0; pass < 4; pass++) {
+			int v, p;
+			if (!g.trans) v = pass * 16 + (t >> 4), p = t & 15;          // p fastest in memory
+			else v = t & 63, p = pass * 4 + (t >> 6);                     // v fastest in memory
+			const int gv = v0 + v, gp = p0 + p;
+			xa[pass] = make_double2(0.0, 0.0);
+			if (gv < g.nv && gp < g.np) xa[pass] = g.trans ? A[(long)gp * g.nv + gv] : A[(long)gv * g.np + gp];
+			const int bp = pass * 4 + (t >> 6), bc = t & 63;
+			xb[pass] = make_double2(0.0, 0.0);
+			if (p0 + bp < g.np && c0 + bc < g.nc) xb[pass] = g.B[(long)(p0 + bp) * g.nc + c0 + bc];
+		}
+	};
+	fetch(0);
+	for (int p0 = 0; p0 < g.np; p0 += FFG_K) {
+#pragma unroll
+		for (int pass = 0; pass < 4; pass++) {
+			int v, p;
+			if (!g.trans) v = pass * 16 + (t >> 4), p = t & 15;
+			else v = t & 63, p = pass * 4 + (t >> 6);
+			Ar[v][p] = xa[pass].x, Ai[v][p] = xa[pass].y;
+			const int bp = pass * 4 + (t >> 6), bc = t & 63;
+			Br[bp][bc] = xb[pass].x, Bi[bp][bc] = xb[pass].y;
+		}
+		__syncthreads();
+		if (p0 + FFG_K < g.np) fetch(p0 + FFG_K);
+#pragma unroll
+		for (int ks = 0; ks < FFG_K; ks += 4) {
+			const int k = ks + (lane >> 4);
+			double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+			for (int a = 0; a < 2; a++) {
+				ar[a] = Ar[wv + 16 * a + (lane & 15)][k], ai[a] = Ai[wv + 16 * a + (lane & 15)][k];
+				br[a] = Br[k][wc + 16 * a + (lane & 15)], bi[a] = Bi[k][wc + 16 * a + (lane & 15)];
+			}
+#pragma unroll
+			for (int a = 0; a < 2; a++)
+#pragma unroll
+				for (int b = 0; b < 2; b++) {
+					cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+					cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
+					ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+					ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+				}
+		}
+		__syncthreads();
+	}
+	double2 *C = g.C[m];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int v = v0 + wv + 16 * a + (lane >> 4) + 4 * r, c = c0 + wc + 16 * b + (lane & 15);
+				if (v < g.nv && c < g.nc) C[(long)v * g.nc + c] = make_double2(cr[a][b][r], ci[a][b][r]);
+			}
+}
+
 __global__ void k_ff_widen(const float *src, double *dst, long n)
 {
 	const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2799,7 +2890,6 @@ int ffWorkspace(alis_dev_world *D, size_t bytes)
 	return 0;
 }
 
-#define BLASCHECK(x) do { rocblas_status st_ = (x); if (st_ != rocblas_status_success) return fail(std::string("rocBLAS: ") + rocblas_status_to_string(st_)); } while (0)
 
 }  // namespace
 
@@ -2877,10 +2967,6 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 		F = (const double2 *)Fw;
 	}
 
-	if (!D->blas) BLASCHECK(rocblas_create_handle(&D->blas));
-	BLASCHECK(rocblas_set_stream(D->blas, D->st));
-	const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-
 	const FFPhase XN = {dxE, xN, ni, 0, g->symX, 0, g->coskx}, XH = {dxH, xH, ni, 1, g->symX, 0, g->coskx};
 	const FFPhase YN = {dyE, yN, nj, 0, g->symY, 1, g->cosky}, YH = {dyH, yH, nj, 1, g->symY, 1, g->cosky};
 	struct Block { int kinds[2]; int comps[2]; FFPhase beta; int profile, half; FFPhase alpha; int tcomp[2]; bool trans; int np, nv; };
@@ -2899,6 +2985,7 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 		const int nc = std::min(NC, n - c0);
 		for (const Block &b : blocks) {
 			FFReduce r{};
+			FFGemm gm{};
 			r.C = Cm, r.nv = b.nv, r.nc = nc;
 			for (int f = 0; f < Q.nface; f++) {
 				const PhsFaceGeom &fg = Q.face[f];
@@ -2907,21 +2994,17 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 					const int m = r.nm++;
 					r.out[m] = fg.kind * 4 + b.comps[cc];
 					r.tcomp[m] = b.tcomp[cc];
-					const double2 *Fm = F + (size_t)b.comps[cc] * Q.ncell + fg.off;
-					double2 *Cout = Cm + (size_t)m * b.nv * nc;
-					if (m == 0) {
-						dim3 grid(blocksFor(nc), b.np);
-						k_ff_beta<<<grid, 256, 0, D->st>>>(beta, b.beta, cdev + c0, nc);
-						HIPCHECK(hipGetLastError());
-					}
-					// C[v][col] = sum_p F[.][.] beta[p][col]  (column-major: C^T = beta^T op(F))
-					BLASCHECK(rocblas_zgemm(D->blas, rocblas_operation_none, b.trans ? rocblas_operation_transpose : rocblas_operation_none,
-					                        nc, b.nv, b.np, &one, (const rocblas_double_complex *)beta, nc,
-					                        (const rocblas_double_complex *)Fm, b.trans ? b.nv : b.np, &zero,
-					                        (rocblas_double_complex *)Cout, nc));
+					gm.A[m] = F + (size_t)b.comps[cc] * Q.ncell + fg.off;
+					gm.C[m] = Cm + (size_t)m * b.nv * nc;
 				}
 			}
 			if (!r.nm) continue;
+			// beta[p][col], then C_m[v][col] = sum_p F_m[.][.] beta[p][col] for the block's matrices
+			k_ff_beta<<<dim3(blocksFor(nc), b.np), 256, 0, D->st>>>(beta, b.beta, cdev + c0, nc);
+			HIPCHECK(hipGetLastError());
+			gm.B = beta, gm.nv = b.nv, gm.np = b.np, gm.nc = nc, gm.trans = b.trans;
+			k_ff_gemm<<<dim3((nc + FFG_T - 1) / FFG_T, (b.nv + FFG_T - 1) / FFG_T, r.nm), 256, 0, D->st>>>(gm);
+			HIPCHECK(hipGetLastError());
 			r.profile = b.profile, r.half = b.half, r.ph = b.alpha;
 			r.dz = b.half ? dzH : dzE;
 			r.T = Tdev, r.nz = g->nz, r.kLo = g->kLo, r.k0 = Q.kMin, r.symZ = g->symZ, r.coskz = g->coskz;

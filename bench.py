@@ -142,14 +142,17 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None, f
                        "finite": all(math.isfinite(vals[q]) for q in range(n))}
     if farfield:
         import tempfile
+        X, Y = farfield
+        w.wl_farfield(ctx, 1, 1, b"")                    # first call: far-field set-up
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        w.wl_farfield(ctx, X, Y, b"")                    # the map without its image files
+        hip.hipDeviceSynchronize()
+        out["farfield_sec"] = time.perf_counter() - t0
         with tempfile.TemporaryDirectory() as d:
-            X, Y = farfield
-            w.wl_farfield(ctx, 1, 1, d.encode())         # first call: GEMM library set-up
-            hip.hipDeviceSynchronize()
             t0 = time.perf_counter()
-            w.wl_farfield(ctx, X, Y, d.encode())
-            hip.hipDeviceSynchronize()
-            out["farfield_sec"] = time.perf_counter() - t0
+            w.wl_farfield(ctx, X, Y, d.encode())         # and with its seven text images
+            out["farfield_files_sec"] = time.perf_counter() - t0
     w.wl_free(ctx)
     return out
 
@@ -415,9 +418,9 @@ def metasurface(alis, w, hip, args):
         "value": round(vm, 1), "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(rm["sec"] / msteps * 1e3, 4),
         "steps": msteps, "roofline": {k: rlm[k] for k in ("bound", "achieved", "peak", "unit", "frac", "kernel",
                                                         "bytes_per_launch", "launch_ms")},
-        "farfield": {"directions": X * Y, "map": f"{X}x{Y} cylindrical (theta, phi)", "dtype": "f64",
-                     "sec": round(rm["farfield_sec"], 4),
-                     "directions_per_s": round(X * Y / rm["farfield_sec"], 1)},
+        "farfield": {"directions": X * Y, "map": f"{X}x{Y} cylindrical (theta, phi), P and S", "dtype": "f64",
+                     "sec": round(rm["farfield_sec"], 4), "directions_per_s": round(X * Y / rm["farfield_sec"], 1),
+                     "sec_with_text_images": round(rm["farfield_files_sec"], 4)},
     }
     fc = farfield_cpu_baseline(X, 8) if args.cpu_steps > 0 else None
     if fc:

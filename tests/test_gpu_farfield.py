@@ -1,4 +1,4 @@
-"""Near-to-far projection on the GPU (alis_dev_farfield: rocBLAS complex GEMMs over the
+"""Near-to-far projection on the GPU (alis_dev_farfield: the FP64 MFMA complex GEMM k_ff_gemm over the
 phaser faces plus the phase/profile reductions) against the CPU oracle's surface sums.
 
 The far field of every frequency of a scenario's phaser array, in double precision, at
