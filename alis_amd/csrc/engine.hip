@@ -696,6 +696,77 @@ __global__ __launch_bounds__(256) void k_spectrum_add(double2 *__restrict__ acc,
 		if (u < pn) acc[2 * ((long)(p0 + u) * nb + b)] = sum[u], acc[2 * ((long)(p0 + u) * nb + b) + 1] = cmp[u];
 }
 
+// Per-row sums over a plane of a world's fields for the instantaneous Poynting flux
+// (src/output.c:233-332).  Term t (PLANE_TERM ints: normal axis, plane index, E component,
+// H component, row axis, first row, last row, first and last inner index, weight offset,
+// output offset, first row of the launch) gives one sum per row r:
+//     sum over inner s, in order, of w[s] * E_e(cell) * (H_h(cell) + H_h(cell + 1 along the normal))
+// with the cell at (normal = plane index, row axis = r, inner axis = s), the real parts, the
+// H pair added in the field precision and the rest in double.  One thread per row.
+constexpr int PLANE_TERM = 12;
+
+template<class T> struct Arrays6 { const T *p[6]; };   // E x y z, H x y z (or E x y z, 1/eps x y z)
+
+template<class T>
+__global__ __launch_bounds__(256) void k_plane_sums(Arrays6<T> F, Lay L, const int *__restrict__ terms, int nterm,
+                                                    long rows, const double *__restrict__ w, double *__restrict__ out)
+{
+	const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= rows) return;
+	int t = 0;
+	while (t + 1 < nterm && g >= terms[PLANE_TERM * (t + 1) + 11]) t++;
+	const int *u = terms + PLANE_TERM * t;
+	const int nrm = u[0], row = u[4], inner = 3 - u[0] - u[4];
+	const int r = u[5] + (int)(g - u[11]);
+	const T *E = F.p[u[2]], *H = F.p[3 + u[3]];
+	int c[3];
+	c[nrm] = u[1], c[row] = r;
+	const long step = inner == 0 ? L.si : inner == 1 ? L.sj : 1, shift = nrm == 0 ? L.si : nrm == 1 ? L.sj : 1;
+	c[inner] = u[7];
+	long id = L.at(c[0], c[1], c[2]);
+	double sum = 0;
+	for (int q = 0; q <= u[8] - u[7]; q++, id += step)
+		sum += w[u[9] + q] * (double)E[id] * (double)(H[id] + H[id + shift]);
+	out[u[10] + (r - u[5])] = sum;
+}
+
+// Column sums of the power absorbed by a dispersive object (src/output.c:375-398): for each
+// column (i, j) of the box, over k in order, the three terms dz E_q current_q (q = x, y, z;
+// dz = E widths for x and y, H widths for z, passed as weights), skipped where 1/eps is 0;
+// current_q = (a_q ? 2 fill_q E_q / (1 + 1/a_q) : 0) + sum_p J_p,q, over 1/eps_q, in the field
+// precision (src/output.c:25-47).  out per column: the three sums, then the three terms of the
+// first cell and of the last cell (the host's boundary corrections).
+template<class T>
+__global__ __launch_bounds__(256) void k_absorption(Arrays6<T> EC, Lay L, const ObjDev<T> *__restrict__ obj, int n, Box b,
+                                                    const double *__restrict__ dzE, const double *__restrict__ dzH, double *__restrict__ out)
+{
+	const int nj = b.j1 - b.j0 + 1;
+	const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (col >= (long)(b.i1 - b.i0 + 1) * nj) return;
+	const int i = b.i0 + (int)(col / nj), j = b.j0 + (int)(col % nj);
+	const ObjDev<T> &O = obj[n];
+	double sum[3] = {0, 0, 0}, first[3] = {0, 0, 0}, last[3] = {0, 0, 0};
+	for (int k = b.k0; k <= b.k1; k++) {
+		const long id = L.at(i, j, k), o = O.LO.at(i, j, k);
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			const T c = EC.p[3 + q][id];
+			if (c == T(0)) continue;
+			const T e = EC.p[q][id];
+			T f = 0;
+			if (O.a[q] != T(0)) f += T(2) * O.fill[q][o] * e / (T(1) + T(1) / O.a[q]);
+			for (int p = 0; p < O.NK; p++) f += O.J[0][q][p][o];
+			const T d = (T)(q < 2 ? dzE[k - b.k0] : dzH[k - b.k0]);
+			const double term = (double)(d * e * (f / c));
+			sum[q] += term;
+			if (k == b.k0) first[q] = term;
+			if (k == b.k1) last[q] = term;
+		}
+	}
+	double *r = out + col * 9;
+	for (int q = 0; q < 3; q++) r[q] = sum[q], r[3 + q] = first[q], r[6 + q] = last[q];
+}
+
 // Inner sums of the net flux Re(E x H*) of one frequency over the phaser faces
 // (src/phaser.c:142-223).  Term t of the table (FLUX_TERM ints: kind, first cell, rows na, row
 // length nb, components x and y, weight offset, output offset, first workgroup) sums
@@ -3073,6 +3144,84 @@ void alis_dev_batch_release(alis_dev_batch *B)
 		if (B->postDone[h]) hipEventDestroy(B->postDone[h]);
 	}
 	delete B;
+}
+
+// ---- observers: Poynting flux through planes, absorbed power ----
+
+extern "C++" {
+namespace {
+
+template<class T>
+int planeSums(alis_dev_world *D, int nterm, const int *terms, const double *w, long nw, double *out, long nout)
+{
+	const Box fb = fieldBox(D);
+	const int lo[3] = {fb.i0, fb.j0, fb.k0}, hi[3] = {fb.i1, fb.j1, fb.k1};
+	std::vector<int> t((size_t)PLANE_TERM * nterm);
+	long rows = 0;
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms + 11 * n;
+		const int nrm = u[0], row = u[4], inner = 3 - u[0] - u[4];
+		const long nr = (long)u[6] - u[5] + 1, ns = (long)u[8] - u[7] + 1;
+		if (nrm < 0 || nrm > 2 || row < 0 || row > 2 || row == nrm || u[2] < 0 || u[2] > 2 || u[3] < 0 || u[3] > 2 || nr < 1 || ns < 1
+		    || u[1] < lo[nrm] || u[1] + 1 > hi[nrm] || u[5] < lo[row] || u[6] > hi[row] || u[7] < lo[inner] || u[8] > hi[inner]
+		    || u[9] < 0 || u[9] + ns > nw || u[10] < 0 || u[10] + nr > nout)
+			return fail("plane sums: term outside the grid or the buffers");
+		std::copy(u, u + 11, t.begin() + (size_t)PLANE_TERM * n);
+		t[(size_t)PLANE_TERM * n + 11] = (int)rows;
+		rows += nr;
+	}
+	const long tab = ((long)t.size() + 1) / 2;
+	if (ensureScratch(D, nout + nw + tab)) return -1;
+	double *dOut = (double *)D->scratch, *dW = dOut + nout;
+	int *dT = (int *)(dW + nw);
+	k_fill<double><<<blocksFor(nout), 256, 0, D->st>>>(dOut, nout, 0.0);
+	HIPCHECK(hipMemcpyAsync(dW, w, nw * sizeof(double), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemcpyAsync(dT, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
+	Arrays6<T> F;
+	for (int a = 0; a < 3; a++) F.p[a] = (const T *)D->F[0][0][a], F.p[3 + a] = (const T *)D->F[1][0][a];
+	k_plane_sums<T><<<blocksFor(rows), 256, 0, D->st>>>(F, D->L, dT, nterm, rows, dW, dOut);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, dOut, nout * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+template<class T>
+int absorption(alis_dev_world *D, int n, const Box &b, const double *dzE, const double *dzH, double *out)
+{
+	const long cols = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1), nk = b.k1 - b.k0 + 1;
+	if (ensureScratch(D, 9 * cols + 2 * nk)) return -1;
+	double *dOut = (double *)D->scratch, *dz = dOut + 9 * cols;
+	HIPCHECK(hipMemcpyAsync(dz, dzE, nk * sizeof(double), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipMemcpyAsync(dz + nk, dzH, nk * sizeof(double), hipMemcpyHostToDevice, D->st));
+	Arrays6<T> EC;
+	for (int a = 0; a < 3; a++) EC.p[a] = (const T *)D->F[0][0][a], EC.p[3 + a] = (const T *)D->C[a];
+	k_absorption<T><<<blocksFor(cols), 256, 0, D->st>>>(EC, D->L, (const ObjDev<T> *)D->objTab, n, b, dz, dz + nk, dOut);
+	HIPCHECK(hipGetLastError());
+	HIPCHECK(hipMemcpyAsync(out, dOut, 9 * cols * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int alis_dev_plane_sums(alis_dev_world *D, int nterm, const int *terms, const double *w, long nw, double *out, long nout)
+{
+	if (nterm < 1 || nw < 1 || nout < 1) return fail("plane sums: no terms or no buffers");
+	return D->fp64 ? planeSums<double>(D, nterm, terms, w, nw, out, nout) : planeSums<float>(D, nterm, terms, w, nw, out, nout);
+}
+
+int alis_dev_absorption(alis_dev_world *D, int n, int i0, int i1, int j0, int j1, int k0, int k1,
+                        const double *dzE, const double *dzH, double *out)
+{
+	if (n < 0 || n >= (int)D->obj.size()) return fail("absorption: no such dispersive object");
+	if (!D->C[0]) return fail("absorption: the world has no permittivity arrays");
+	const Box &o = D->obj[n].clip;
+	if (i0 > i1 || j0 > j1 || k0 > k1 || i0 < o.i0 || i1 > o.i1 || j0 < o.j0 || j1 > o.j1 || k0 < o.k0 || k1 > o.k1)
+		return fail("absorption: columns outside the object's box");
+	const Box b{i0, i1, j0, j1, k0, k1};
+	return D->fp64 ? absorption<double>(D, n, b, dzE, dzH, out) : absorption<float>(D, n, b, dzE, dzH, out);
 }
 
 // ---- probe spectra ----

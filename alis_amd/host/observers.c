@@ -343,164 +343,118 @@ void (writeSpectrum)(world W, int N, float LMin, float LMax, char *format, ...)
 }
 
 
-/* Instantaneous Poynting flux through a plane normal to x (y, z) at x, within the given
- * transverse bounds; H is averaged onto the E plane (src/output.c:233-332). */
-float (poyntingX)(world W, float x, float yMin, float yMax, float zMin, float zMax, ...)
+/*
+ * Instantaneous Poynting flux through a plane (poyntingX/Y/Z, src/output.c:233-332).  For the
+ * plane normal to axis a at index c, with in-plane axes b = a+1 and d = a+2 (cyclic), the flux
+ * is the trapezoid sum of E_b (H_d + H_d') over rows along b (from the second index on),
+ * weighted by the H widths of b, minus that of E_d (H_b + H_b') over rows along d; H' is the H
+ * value one cell further along a.  The row sums run on the device (alis_dev_plane_sums), so
+ * no copy of the world is needed; the rows are added here in order, in double.
+ */
+typedef struct {
+	int MIN, MAX, Min, Max, sur;
+	float domMin, domMax;
+	float *dE, *dH;             /* cell widths of the E and H grids */
+	int (*toIndex)(world, float);
+} axisView;
+
+static axisView viewOf(world W, int a)
 {
-	syncWorld(W);
-	int i = W->xtoi(W, x<0 && W->xMinSur==SYM ? -x : x);
-	int jMin = yMin==yMax ? MAX(W->jMin, W->ytoj(W, yMin)-1) : MAX(W->jMin, W->ytoj(W, yMin));
-	int jMax = yMin==yMax ? MIN(W->jMax, W->ytoj(W, yMax)+1) : MIN(W->jMax, W->ytoj(W, yMax));
-	int kMin = zMin==zMax ? MAX(W->kMin, W->ztok(W, zMin)-1) : MAX(W->kMin, W->ztok(W, zMin));
-	int kMax = zMin==zMax ? MIN(W->kMax, W->ztok(W, zMax)+1) : MIN(W->kMax, W->ztok(W, zMax));
-	double Sum = 0;
-	for (int j = jMin+1; j <= jMax; j++) {
-		double s = 0;
-		for (int k = kMin; k <= kMax; k++) {
-			double w = (k == kMin || k == kMax) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
-			s += w * W->E->y[i][j][k] * (W->H->z[i][j][k] + W->H->z[i+1][j][k]);
-		}
-		Sum += W->CH->dy[j] * s;
-	}
-	for (int k = kMin+1; k <= kMax; k++) {
-		double s = 0;
-		for (int j = jMin; j <= jMax; j++) {
-			double w = (j == jMin || j == jMax) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
-			s -= w * W->E->z[i][j][k] * (W->H->y[i][j][k] + W->H->y[i+1][j][k]);
-		}
-		Sum += W->CH->dz[k] * s;
-	}
-	if (yMin==yMax || W->Dom.y.Min==W->Dom.y.Max) Sum /= W->CH->dy[jMax]+W->CH->dy[jMax-1];
-	if (zMin==zMax || W->Dom.z.Min==W->Dom.z.Max) Sum /= W->CH->dz[kMax]+W->CH->dz[kMax-1];
-	if (W->yMinSur==SYM) Sum *= 2;
-	if (W->zMinSur==SYM) Sum *= 2;
-	return (x<0 && W->xMinSur==SYM ? -0.5 : 0.5) * Sum;
+	if (a == 0) return (axisView) {W->iMIN, W->iMAX, W->iMin, W->iMax, W->xMinSur, W->Dom.x.Min, W->Dom.x.Max, W->CE->dx, W->CH->dx, W->xtoi};
+	if (a == 1) return (axisView) {W->jMIN, W->jMAX, W->jMin, W->jMax, W->yMinSur, W->Dom.y.Min, W->Dom.y.Max, W->CE->dy, W->CH->dy, W->ytoj};
+	return (axisView) {W->kMIN, W->kMAX, W->kMin, W->kMax, W->zMinSur, W->Dom.z.Min, W->Dom.z.Max, W->CE->dz, W->CH->dz, W->ztok};
 }
 
-float (poyntingY)(world W, float y, float xMin, float xMax, float zMin, float zMax, ...)
+/* The index range of [lo, hi] on an axis: one cell wider on each side for a single point. */
+static void indexRange(world W, const axisView *v, float lo, float hi, int *from, int *to)
 {
-	syncWorld(W);
-	int j = W->ytoj(W, y<0 && W->yMinSur==SYM ? -y : y);
-	int iMin = xMin==xMax ? MAX(W->iMin, W->xtoi(W, xMin)-1) : MAX(W->iMin, W->xtoi(W, xMin));
-	int iMax = xMin==xMax ? MIN(W->iMax, W->xtoi(W, xMax)+1) : MIN(W->iMax, W->xtoi(W, xMax));
-	int kMin = zMin==zMax ? MAX(W->kMin, W->ztok(W, zMin)-1) : MAX(W->kMin, W->ztok(W, zMin));
-	int kMax = zMin==zMax ? MIN(W->kMax, W->ztok(W, zMax)+1) : MIN(W->kMax, W->ztok(W, zMax));
-	double Sum = 0;
-	for (int k = kMin+1; k <= kMax; k++) {
-		double s = 0;
-		for (int i = iMin; i <= iMax; i++) {
-			double w = (i == iMin || i == iMax) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
-			s += w * W->E->z[i][j][k] * (W->H->x[i][j][k] + W->H->x[i][j+1][k]);
-		}
-		Sum += W->CH->dz[k] * s;
-	}
-	for (int i = iMin+1; i <= iMax; i++) {
-		double s = 0;
-		for (int k = kMin; k <= kMax; k++) {
-			double w = (k == kMin || k == kMax) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
-			s -= w * W->E->x[i][j][k] * (W->H->z[i][j][k] + W->H->z[i][j+1][k]);
-		}
-		Sum += W->CH->dx[i] * s;
-	}
-	if (xMin==xMax || W->Dom.x.Min==W->Dom.x.Max) Sum /= W->CH->dx[iMax]+W->CH->dx[iMax-1];
-	if (zMin==zMax || W->Dom.z.Min==W->Dom.z.Max) Sum /= W->CH->dz[kMax]+W->CH->dz[kMax-1];
-	if (W->xMinSur==SYM) Sum *= 2;
-	if (W->zMinSur==SYM) Sum *= 2;
-	return (y<0 && W->yMinSur==SYM ? -0.5 : 0.5) * Sum;
+	*from = lo == hi ? MAX(v->Min, v->toIndex(W, lo) - 1) : MAX(v->Min, v->toIndex(W, lo));
+	*to = lo == hi ? MIN(v->Max, v->toIndex(W, hi) + 1) : MIN(v->Max, v->toIndex(W, hi));
 }
 
-float (poyntingZ)(world W, float z, float xMin, float xMax, float yMin, float yMax, ...)
+static float planeFlux(world W, int a, float pos, float lo1, float hi1, float lo2, float hi2)
 {
-	syncWorld(W);
-	int k = W->ztok(W, z<0 && W->zMinSur==SYM ? -z : z);
-	int iMin = xMin==xMax ? MAX(W->iMin, W->xtoi(W, xMin)-1) : MAX(W->iMin, W->xtoi(W, xMin));
-	int iMax = xMin==xMax ? MIN(W->iMax, W->xtoi(W, xMax)+1) : MIN(W->iMax, W->xtoi(W, xMax));
-	int jMin = yMin==yMax ? MAX(W->jMin, W->ytoj(W, yMin)-1) : MAX(W->jMin, W->ytoj(W, yMin));
-	int jMax = yMin==yMax ? MIN(W->jMax, W->ytoj(W, yMax)+1) : MIN(W->jMax, W->ytoj(W, yMax));
+	alisCommit(W);
+	const int b = (a + 1) % 3, d = (a + 2) % 3;
+	const axisView A = viewOf(W, a), B = viewOf(W, b), Dd = viewOf(W, d);
+	const int c = A.toIndex(W, pos < 0 && A.sur==SYM ? -pos : pos);
+	/* the transverse bounds arrive in (x, y, z) order of the two remaining axes */
+	const float loB = b < d ? lo1 : lo2, hiB = b < d ? hi1 : hi2, loD = b < d ? lo2 : lo1, hiD = b < d ? hi2 : hi1;
+	int b0, b1, d0, d1;
+	indexRange(W, &B, loB, hiB, &b0, &b1);
+	indexRange(W, &Dd, loD, hiD, &d0, &d1);
+	const int nb = b1 - b0 + 1, nd = d1 - d0 + 1;
 	double Sum = 0;
-	for (int i = iMin+1; i <= iMax; i++) {
-		double s = 0;
-		for (int j = jMin; j <= jMax; j++) {
-			double w = (j == jMin || j == jMax) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
-			s += w * W->E->x[i][j][k] * (W->H->y[i][j][k] + W->H->y[i][j][k+1]);
+	if (nb > 0 && nd > 0) {
+		/* inner weights: E widths, halved at both ends; term 1 runs inner over d, term 2 over b */
+		double *w = malloc((nb + nd) * sizeof(double)), *out = malloc((nb + nd) * sizeof(double));
+		for (int q = d0; q <= d1; q++) w[q - d0] = (q == d0 || q == d1) ? 0.5 * Dd.dE[q] : Dd.dE[q];
+		for (int q = b0; q <= b1; q++) w[nd + q - b0] = (q == b0 || q == b1) ? 0.5 * B.dE[q] : B.dE[q];
+		int terms[2 * 11], nt = 0;
+		if (nb > 1) {
+			const int t[11] = {a, c, b, d, b, b0 + 1, b1, d0, d1, 0, 0};
+			memcpy(terms + 11 * nt++, t, sizeof t);
 		}
-		Sum += W->CH->dx[i] * s;
-	}
-	for (int j = jMin+1; j <= jMax; j++) {
-		double s = 0;
-		for (int i = iMin; i <= iMax; i++) {
-			double w = (i == iMin || i == iMax) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
-			s -= w * W->E->y[i][j][k] * (W->H->x[i][j][k] + W->H->x[i][j][k+1]);
+		if (nd > 1) {
+			const int t[11] = {a, c, d, b, d, d0 + 1, d1, b0, b1, nd, nb};
+			memcpy(terms + 11 * nt++, t, sizeof t);
 		}
-		Sum += W->CH->dy[j] * s;
+		if (nt) alisFatal("plane sums", alis_dev_plane_sums(W->dev, nt, terms, w, nb + nd, out, nb + nd));
+		for (int r = b0 + 1; r <= b1 && nb > 1; r++) Sum += B.dH[r] * out[r - b0 - 1];
+		for (int r = d0 + 1; r <= d1 && nd > 1; r++) Sum += Dd.dH[r] * -out[nb + r - d0 - 1];
+		free(w), free(out);
 	}
-	if (xMin==xMax || W->Dom.x.Min==W->Dom.x.Max) Sum /= W->CH->dx[iMax]+W->CH->dx[iMax-1];
-	if (yMin==yMax || W->Dom.y.Min==W->Dom.y.Max) Sum /= W->CH->dy[jMax]+W->CH->dy[jMax-1];
-	if (W->xMinSur==SYM) Sum *= 2;
-	if (W->yMinSur==SYM) Sum *= 2;
-	return (z<0 && W->zMinSur==SYM ? -0.5 : 0.5) * Sum;
+	/* a single point or a flat domain on a transverse axis: per unit length (axes in x, y, z order) */
+	for (int q = 0; q < 2; q++) {
+		const int onB = (q == 0) == (b < d);
+		const axisView *V = onB ? &B : &Dd;
+		const int top = onB ? b1 : d1;
+		if ((onB ? loB == hiB : loD == hiD) || V->domMin == V->domMax) Sum /= V->dH[top] + V->dH[top-1];
+	}
+	if (B.sur==SYM) Sum *= 2;
+	if (Dd.sur==SYM) Sum *= 2;
+	return (pos < 0 && A.sur==SYM ? -0.5 : 0.5) * Sum;
 }
 
-/* Net outward flux through the faces of a box; periodic axes spanning the world are
- * skipped (src/output.c:335-356). */
+float (poyntingX)(world W, float x, float yMin, float yMax, float zMin, float zMax, ...) { return planeFlux(W, 0, x, yMin, yMax, zMin, zMax); }
+float (poyntingY)(world W, float y, float xMin, float xMax, float zMin, float zMax, ...) { return planeFlux(W, 1, y, xMin, xMax, zMin, zMax); }
+float (poyntingZ)(world W, float z, float xMin, float xMax, float yMin, float yMax, ...) { return planeFlux(W, 2, z, xMin, xMax, yMin, yMax); }
+
+/* Net outward flux through the faces of a box; an axis whose box spans a Bloch-periodic world
+ * is skipped, and a box reaching a symmetry plane counts its far face twice (src/output.c:335-356). */
 float poyntingOut(world W, float xMin, float xMax, float yMin, float yMax, float zMin, float zMax)
 {
-	float Sx = 0, Sy = 0, Sz = 0;
-	if (xMin > W->xMin || xMax < W->xMax || W->xMinSur != BBC) {
-		Sx = (poyntingX)(W, xMax, yMin, yMax, zMin, zMax);
-		if (xMin < 0 && W->xMinSur==SYM) Sx *= 2;
-		else Sx -= (poyntingX)(W, xMin, yMin, yMax, zMin, zMax);
+	const float lo[3] = {xMin, yMin, zMin}, hi[3] = {xMax, yMax, zMax};
+	const float wlo[3] = {W->xMin, W->yMin, W->zMin}, whi[3] = {W->xMax, W->yMax, W->zMax};
+	float S[3] = {0, 0, 0};
+	for (int a = 0; a < 3; a++) {
+		const int sur = viewOf(W, a).sur;
+		if (!(lo[a] > wlo[a] || hi[a] < whi[a] || sur != BBC)) continue;
+		const int b = (a + 1) % 3, d = (a + 2) % 3, u = b < d ? b : d, v = b < d ? d : b;
+		S[a] = planeFlux(W, a, hi[a], lo[u], hi[u], lo[v], hi[v]);
+		if (lo[a] < 0 && sur==SYM) S[a] *= 2;
+		else S[a] -= planeFlux(W, a, lo[a], lo[u], hi[u], lo[v], hi[v]);
 	}
-	if (yMin > W->yMin || yMax < W->yMax || W->yMinSur != BBC) {
-		Sy = (poyntingY)(W, yMax, xMin, xMax, zMin, zMax);
-		if (yMin < 0 && W->yMinSur==SYM) Sy *= 2;
-		else Sy -= (poyntingY)(W, yMin, xMin, xMax, zMin, zMax);
-	}
-	if (zMin > W->zMin || zMax < W->zMax || W->zMinSur != BBC) {
-		Sz = (poyntingZ)(W, zMax, xMin, xMax, yMin, yMax);
-		if (zMin < 0 && W->zMinSur==SYM) Sz *= 2;
-		else Sz -= (poyntingZ)(W, zMin, xMin, xMax, yMin, yMax);
-	}
-	return Sx + Sy + Sz;
+	return S[0] + S[1] + S[2];
 }
 
 
-/* Polarisation current of dispersive object n at an E position: conductive and pole
- * currents over 1/eps (src/output.c:25-47). */
-static float objCurrent(world W, int q, int n, int i, int j, int k)
+/*
+ * Power absorbed by a dispersive object (objectAbsorption, src/output.c:375-435): E . J over
+ * the object's box (J: the object's conductive and pole currents over 1/eps,
+ * src/output.c:25-47), with the trapezoid corrections where the box reaches the grid's outer
+ * planes.  The column sums over z run on the device (alis_dev_absorption); the columns are
+ * weighted and added here in order, in double.
+ */
+/* The weighted power of one column from its device sums s (three sums, the three terms of
+ * its first and of its last cell): the first cell's terms come off at the grid's bottom plane
+ * and the last cell's x and y terms at its top plane (src/output.c:387-397). */
+static double columnPower(world W, const double *s, int i, int j, int k0, int k1, double x, double y, double z)
 {
+	double xs = s[0], ys = s[1], zs = s[2];
+	if (k0 == W->kMIN) xs -= s[3] * 0.5, ys -= s[4] * 0.5, zs -= s[5];
+	if (k1 == W->kMAX) xs -= s[6] * 0.5, ys -= s[7] * 0.5;
 	coeffs *C = W->CE;
-	float a = q == 0 ? C->ax[n] : q == 1 ? C->ay[n] : C->az[n];
-	float ***fill = q == 0 ? C->Jx[n] : q == 1 ? C->Jy[n] : C->Jz[n];
-	float ***E = q == 0 ? W->E->x : q == 1 ? W->E->y : W->E->z;
-	float ****J = q == 0 ? W->E->Jx[n] : q == 1 ? W->E->Jy[n] : W->E->Jz[n];
-	float ***Cq = q == 0 ? C->x : q == 1 ? C->y : C->z;
-	float f = 0;
-	if (a) f += 2 * fill[i][j][k] * E[i][j][k] / (1 + 1 / a);
-	for (int p = 0; p < C->NK[n]; p++) if (J[p]) f += J[p][i][j][k];
-	return f / Cq[i][j][k];
-}
-
-/* E.J of object n on the column (i, j) with boundary weights x, y, z (src/output.c:375-398). */
-static double columnAbsorption(world W, int n, int i, int j, double x, double y, double z)
-{
-	coeffs *C = W->CE;
-	double xs = 0, ys = 0, zs = 0;
-	int kMinJ = MAX(W->kMIN, C->kMin[n]), kMaxJ = MIN(W->kMAX, C->kMax[n]);
-	for (int k = kMinJ; k <= kMaxJ; k++) {
-		if (C->x[i][j][k]) xs += C->dz[k] * W->E->x[i][j][k] * objCurrent(W, 0, n, i, j, k);
-		if (C->y[i][j][k]) ys += C->dz[k] * W->E->y[i][j][k] * objCurrent(W, 1, n, i, j, k);
-		if (C->z[i][j][k]) zs += W->CH->dz[k] * W->E->z[i][j][k] * objCurrent(W, 2, n, i, j, k);
-	}
-	if (kMinJ == W->kMIN) {
-		if (C->x[i][j][W->kMIN]) xs -= C->dz[W->kMIN] * W->E->x[i][j][W->kMIN] * objCurrent(W, 0, n, i, j, W->kMIN) * 0.5;
-		if (C->y[i][j][W->kMIN]) ys -= C->dz[W->kMIN] * W->E->y[i][j][W->kMIN] * objCurrent(W, 1, n, i, j, W->kMIN) * 0.5;
-		if (C->z[i][j][W->kMIN]) zs -= W->CH->dz[W->kMIN] * W->E->z[i][j][W->kMIN] * objCurrent(W, 2, n, i, j, W->kMIN);
-	}
-	if (kMaxJ == W->kMAX) {
-		if (C->x[i][j][W->kMAX]) xs -= C->dz[W->kMAX] * W->E->x[i][j][W->kMAX] * objCurrent(W, 0, n, i, j, W->kMAX) * 0.5;
-		if (C->y[i][j][W->kMAX]) ys -= C->dz[W->kMAX] * W->E->y[i][j][W->kMAX] * objCurrent(W, 1, n, i, j, W->kMAX) * 0.5;
-	}
 	return x * xs * W->CH->dx[i] * C->dy[j] + y * ys * C->dx[i] * W->CH->dy[j] + z * zs * C->dx[i] * C->dy[j];
 }
 
@@ -511,31 +465,40 @@ static int sameObject(object A, object B)
 	return 1;
 }
 
-/* Power absorbed by the dispersive object O (E.J over its box, src/output.c:401-435). */
 float objectAbsorption(world W, object O)
 {
-	syncWorld(W);
-	int n = 0;
-	for (int o = 0; !sameObject(O, W->CE->O[o]); o++) if (dispersiveMatter(W->CE->M[o])) n++;
+	alisCommit(W);
 	coeffs *C = W->CE;
-	int iMinJ = MAX(W->iMIN, C->iMin[n]), iMaxJ = MIN(W->iMAX, C->iMax[n]);
-	int jMinJ = MAX(W->jMIN, C->jMin[n]), jMaxJ = MIN(W->jMAX, C->jMax[n]);
+	int n = 0;
+	for (int o = 0; !sameObject(O, C->O[o]); o++) if (dispersiveMatter(C->M[o])) n++;
+	const int i0 = MAX(W->iMIN, C->iMin[n]), i1 = MIN(W->iMAX, C->iMax[n]);
+	const int j0 = MAX(W->jMIN, C->jMin[n]), j1 = MIN(W->jMAX, C->jMax[n]);
+	const int k0 = MAX(W->kMIN, C->kMin[n]), k1 = MIN(W->kMAX, C->kMax[n]);
 	double Sum = 0;
-	#pragma omp parallel for reduction(+:Sum)
-	for (int i = iMinJ; i <= iMaxJ; i++) {
-		for (int j = jMinJ; j <= jMaxJ; j++) Sum += columnAbsorption(W, n, i, j, 1, 1, 1);
-		if (jMinJ == W->jMIN) Sum += columnAbsorption(W, n, i, W->jMIN, -0.5, -1, -0.5);
-		if (jMaxJ == W->jMAX) Sum += columnAbsorption(W, n, i, W->jMAX, -0.5, 0, -0.5);
-	}
-	if (iMinJ == W->iMIN) {
-		for (int j = jMinJ; j <= jMaxJ; j++) Sum += columnAbsorption(W, n, W->iMIN, j, -1, -0.5, -0.5);
-		if (jMinJ == W->jMIN) Sum += columnAbsorption(W, n, W->iMIN, W->jMIN, 0.5, 0.5, 0.25);
-		if (jMaxJ == W->jMAX) Sum += columnAbsorption(W, n, W->iMIN, W->jMAX, 0.5, 0.0, 0.25);
-	}
-	if (iMaxJ == W->iMAX) {
-		for (int j = jMinJ; j <= jMaxJ; j++) Sum += columnAbsorption(W, n, W->iMAX, j, 0, -0.5, -0.5);
-		if (jMinJ == W->jMIN) Sum += columnAbsorption(W, n, W->iMAX, W->jMIN, 0.0, 0.5, 0.25);
-		if (jMaxJ == W->jMAX) Sum += columnAbsorption(W, n, W->iMAX, W->jMAX, 0.0, 0.0, 0.25);
+	if (i0 <= i1 && j0 <= j1 && k0 <= k1) {
+		const int nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+		double *dz = malloc(2 * nk * sizeof(double)), *col = malloc(9 * sizeof(double) * (i1 - i0 + 1) * nj);
+		for (int k = k0; k <= k1; k++) dz[k - k0] = C->dz[k], dz[nk + k - k0] = W->CH->dz[k];
+		alisFatal("absorption", alis_dev_absorption(W->dev, n, i0, i1, j0, j1, k0, k1, dz, dz + nk, col));
+		/* one column with the trapezoid weights x, y, z of its three components */
+		#define COLUMN(i, j, x, y, z) columnPower(W, col + 9 * ((long)((i) - i0) * nj + (j) - j0), i, j, k0, k1, x, y, z)
+		for (int i = i0; i <= i1; i++) {
+			for (int j = j0; j <= j1; j++) Sum += COLUMN(i, j, 1, 1, 1);
+			if (j0 == W->jMIN) Sum += COLUMN(i, W->jMIN, -0.5, -1, -0.5);
+			if (j1 == W->jMAX) Sum += COLUMN(i, W->jMAX, -0.5, 0, -0.5);
+		}
+		if (i0 == W->iMIN) {
+			for (int j = j0; j <= j1; j++) Sum += COLUMN(W->iMIN, j, -1, -0.5, -0.5);
+			if (j0 == W->jMIN) Sum += COLUMN(W->iMIN, W->jMIN, 0.5, 0.5, 0.25);
+			if (j1 == W->jMAX) Sum += COLUMN(W->iMIN, W->jMAX, 0.5, 0.0, 0.25);
+		}
+		if (i1 == W->iMAX) {
+			for (int j = j0; j <= j1; j++) Sum += COLUMN(W->iMAX, j, 0, -0.5, -0.5);
+			if (j0 == W->jMIN) Sum += COLUMN(W->iMAX, W->jMIN, 0.0, 0.5, 0.25);
+			if (j1 == W->jMAX) Sum += COLUMN(W->iMAX, W->jMAX, 0.0, 0.0, 0.25);
+		}
+		#undef COLUMN
+		free(dz), free(col);
 	}
 	if (W->Dom.x.Min==W->Dom.x.Max) Sum /= W->CH->dx[W->iMax]+W->CH->dx[W->iMax-1];
 	if (W->Dom.y.Min==W->Dom.y.Max) Sum /= W->CH->dy[W->jMax]+W->CH->dy[W->jMax-1];

@@ -224,6 +224,24 @@ int alis_dev_batch_core(alis_dev_batch *B, int h);
 int alis_dev_batch_post(alis_dev_batch *B, int h);
 void alis_dev_batch_release(alis_dev_batch *B);
 
+/* Per-row sums over a plane of the fields (real parts) for the instantaneous Poynting flux
+ * poyntingX/Y/Z (src/output.c:233-332), so the host needs no copy of the world: nterm terms of
+ * 11 ints -- normal axis, plane index, E component, H component, row axis, first and last row,
+ * first and last inner index (the inner axis is the remaining one), weight offset, output
+ * offset.  out[offset + r - first row] = sum over inner s in order of
+ * w[offset + s - first] * E(cell) * (H(cell) + H(cell one step along the normal)), the H pair
+ * added in the field precision, the rest in double.  Outputs no term writes are zero. */
+int alis_dev_plane_sums(alis_dev_world *D, int nterm, const int *terms, const double *w, long nw, double *out, long nout);
+
+/* Column sums of the power absorbed by the n-th dispersive object (objectAbsorption,
+ * src/output.c:375-435; the current of src/output.c:25-47): for each column (i, j) of the box
+ * (inside the object's box), over k = k0..k1 in order, the terms dz E_q current_q of q = x, y,
+ * z with dz from dzE (x, y) or dzH (z), indexed k - k0, skipped where 1/eps is 0, the terms in
+ * the field precision and the sums in double.  out[9 ((i-i0)(j1-j0+1) + j-j0) + ...]: the three
+ * sums, the three terms at k0, the three terms at k1. */
+int alis_dev_absorption(alis_dev_world *D, int n, int i0, int i1, int j0, int j1, int k0, int k1,
+                        const double *dzE, const double *dzH, double *out);
+
 /* Probe spectra: the running DFT of writeSpectrum (src/output.c:177-230, whose loop at
  * :212-216 adds one sample of every probe to every frequency bin per call) in FP64 with
  * compensated summation.  A spectrum has nprobe series and nbin bins; the sample of scaled

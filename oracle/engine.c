@@ -634,6 +634,43 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 	return 0;
 }
 
+int alis_dev_plane_sums(alis_dev_world *D, int nterm, const int *terms, const double *w, long nw, double *out, long nout)
+{
+	if (nterm < 1 || nw < 1 || nout < 1) return fail("plane sums: no terms or no buffers");
+	const alis_dev_world_desc *W = &D->W;
+	const int lo[3] = {W->iMIN - 1, W->jMIN - 1, W->kMIN - 1}, hi[3] = {W->iMAX + 1, W->jMAX + 1, W->kMAX + 1};
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms + 11 * n;
+		const int nrm = u[0], row = u[4], inner = 3 - u[0] - u[4];
+		if (nrm < 0 || nrm > 2 || row < 0 || row > 2 || row == nrm || u[2] < 0 || u[2] > 2 || u[3] < 0 || u[3] > 2 || u[6] < u[5] || u[8] < u[7]
+		    || u[1] < lo[nrm] || u[1] + 1 > hi[nrm] || u[5] < lo[row] || u[6] > hi[row] || u[7] < lo[inner] || u[8] > hi[inner]
+		    || u[9] < 0 || u[9] + (u[8] - u[7] + 1) > nw || u[10] < 0 || u[10] + (u[6] - u[5] + 1) > nout)
+			return fail("plane sums: term outside the grid or the buffers");
+	}
+	for (long c = 0; c < nout; c++) out[c] = 0;
+	for (int n = 0; n < nterm; n++) {
+		if (D->fp64) planeSums_d(D, terms + 11 * n, w, out);
+		else planeSums_f(D, terms + 11 * n, w, out);
+	}
+	return 0;
+}
+
+int alis_dev_absorption(alis_dev_world *D, int n, int i0, int i1, int j0, int j1, int k0, int k1,
+                        const double *dzE, const double *dzH, double *out)
+{
+	if (n < 0 || n >= D->W.nobj) return fail("absorption: no such dispersive object");
+	if (!D->hasEps) return fail("absorption: the world has no permittivity arrays");
+	const alis_dev_ade *o = &D->obj[n];
+	const int ci0 = MAX(D->W.iMIN, o->i0), ci1 = MIN(D->W.iMAX, o->i1), cj0 = MAX(D->W.jMIN, o->j0), cj1 = MIN(D->W.jMAX, o->j1);
+	const int ck0 = MAX(D->W.kMIN, o->k0), ck1 = MIN(D->W.kMAX, o->k1);
+	if (i0 > i1 || j0 > j1 || k0 > k1 || i0 < ci0 || i1 > ci1 || j0 < cj0 || j1 > cj1 || k0 < ck0 || k1 > ck1)
+		return fail("absorption: columns outside the object's box");
+	const int b[6] = {i0, i1, j0, j1, k0, k1};
+	if (D->fp64) absorption_d(D, n, b, dzE, dzH, out);
+	else absorption_f(D, n, b, dzE, dzH, out);
+	return 0;
+}
+
 /* ---- probe spectra (src/output.c:177-230) ----
  * The running DFT of writeSpectrum in double with compensated (Neumaier) summation: every
  * sample is added to every bin as it arrives, in call order (the device stages samples and

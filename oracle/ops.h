@@ -342,3 +342,54 @@ static void SUF(phaserStep)(world *D, phs *P, int q0, int nq, const double *eE, 
 			}
 	}
 }
+
+/* ---- observers: src/output.c:233-332 (Poynting flux through a plane) and :375-398
+ * (absorbed power); the device entries' sums, restated with the same loops ---- */
+
+/* Term (11 ints) of alis_dev_plane_sums: one row sum per row, inner cells in order. */
+static void SUF(planeSums)(world *D, const int *u, const double *w, double *out)
+{
+	arr *E = D->F[0][0], *H = D->F[1][0];
+	const int nrm = u[0], row = u[4], inner = 3 - u[0] - u[4];
+	for (int r = u[5]; r <= u[6]; r++) {
+		double sum = 0;
+		for (int s = u[7]; s <= u[8]; s++) {
+			int c[3], d[3];
+			c[nrm] = u[1], c[row] = r, c[inner] = s;
+			d[0] = c[0], d[1] = c[1], d[2] = c[2];
+			d[nrm]++;
+			const T e = V(&E[u[2]], c[0], c[1], c[2]);
+			const T hh = V(&H[u[3]], c[0], c[1], c[2]) + V(&H[u[3]], d[0], d[1], d[2]);
+			sum += w[u[9] + s - u[7]] * (double)e * (double)hh;
+		}
+		out[u[10] + r - u[5]] = sum;
+	}
+}
+
+/* Column sums of object n's absorbed power over the box b (i0 i1 j0 j1 k0 k1). */
+static void SUF(absorption)(world *D, int n, const int *b, const double *dzE, const double *dzH, double *out)
+{
+	const alis_dev_ade *o = &D->obj[n];
+	arr *E = D->F[0][0];
+	long col = 0;
+	for (int i = b[0]; i <= b[1]; i++)
+		for (int j = b[2]; j <= b[3]; j++, col++) {
+			double sum[3] = {0, 0, 0}, first[3] = {0, 0, 0}, last[3] = {0, 0, 0};
+			for (int k = b[4]; k <= b[5]; k++)
+				for (int q = 0; q < 3; q++) {
+					const T c = (T)EPS(D, q, i, j, k);
+					if (c == 0) continue;
+					const T e = V(&E[q], i, j, k), a = (T)o->a[q];
+					T f = 0;
+					if (a != 0) f += (T)2 * (T)o->fill[q][i][j][k] * e / ((T)1 + (T)1 / a);
+					for (int p = 0; p < o->nk; p++) f += V(&D->J[n][0][q][p], i, j, k);
+					const T d = (T)(q < 2 ? dzE[k - b[4]] : dzH[k - b[4]]);
+					const double term = (double)(d * e * (f / c));
+					sum[q] += term;
+					if (k == b[4]) first[q] = term;
+					if (k == b[5]) last[q] = term;
+				}
+			double *r = out + col * 9;
+			for (int q = 0; q < 3; q++) r[q] = sum[q], r[3 + q] = first[q], r[6 + q] = last[q];
+		}
+}

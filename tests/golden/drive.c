@@ -250,6 +250,9 @@ typedef struct {
 	trace T;
 	int kq;                     /* k-point scenarios: their index, whose spectrum they write */
 	float spec[3][3];           /* and the nodes of their spectrum's probes */
+	object absorbers[4];        /* dispersive objects whose absorbed power is recorded */
+	int nabs;
+	double poyntingRun;         /* poyntingOut of the observer box every 25 steps, summed */
 } scen;
 static void dipoleSetup(scen *c)
 {
@@ -292,6 +295,7 @@ static void nanoblockSetup(scen *c)
 	putObjects(W, Gold, Block, n(1.5), Substrate, Air);
 	planewave(W, DTF, Ppol, 0, 0, Pulse, 600, 400);
 	phaser P = createPhasers(W, 500, 700, 100, -42, 42, -42, 42, -42, 42);
+	c->absorbers[c->nabs++] = Block;
 	field *F[5] = { Ex, Ex, Hy, Ez, Ey };
 	float pts[5][3] = {{0,0,12}, {0,0,0}, {10,-6,20}, {-20,4,-20}, {40,40,40}};
 	c->W = W, c->P = P, c->np = 5;
@@ -394,6 +398,7 @@ static void multipoleSetup(scen *c)
 	object Dd = {Diamond, {{-40, -10}, {10, 40}, {0, 30}}};
 	matter Aniso = {{2, 3, 4}, {{5, 0.1, 1, 0}, {6, 0.2, 1.2, 0}, {4, 0.3, 0.8, 0}}};
 	putObjects(W, Au, A, Ag, B, Si, C, Aniso, Dd, Air);
+	c->absorbers[c->nabs++] = A, c->absorbers[c->nabs++] = B, c->absorbers[c->nabs++] = Dd;
 	pointDipole(W, Ey, 0, 0, 0, Pulse, 500, 300);
 	pointDipole(W, Ez, 10, -10, 5, Pulse, 550, 200, 2, 45);
 	field *F[4] = { Ey, Ez, Hx, Jy };
@@ -466,10 +471,17 @@ static int setupScenario(scen *c, const char *name)
 	return fprintf(stderr, "unknown scenario %s\n", name), 2;
 }
 
+/* The observer box: the middle half of the domain on every axis. */
+static float boxOut(world W)
+{
+	return poyntingOut(W, 0.5f * W->xMin, 0.5f * W->xMax, 0.5f * W->yMin, 0.5f * W->yMax, 0.5f * W->zMin, 0.5f * W->zMax);
+}
+
 static void stepScenario(scen *c, int phaserDone)
 {
 	if (c->P && !phaserDone) updatePhaser(c->W, c->P);
 	sample(&c->T, c->W, c->F, c->pts, c->np);
+	if (c->kq < 0 && c->W->n % 25 == 0) c->poyntingRun += boxOut(c->W);
 	if (c->kq >= 0) {
 		float (*p)[3] = c->spec;
 		writeSpectrum(c->W, RUNSTEPS, 100, 800, "%s/kp%d-spectrum", OUTDIR, c->kq, get(c->W, Hz, p[0][0], p[0][1], p[0][2]),
@@ -508,6 +520,17 @@ static void dumpScenario(scen *c, int stepped)
 	if (c->P) sync(c->W);
 	putTrace(c->name, &c->T);
 	if (c->kq >= 0 && stepped) putSpectrum(c);
+	if (c->kq < 0 && stepped) {
+		/* observers at the last step: planes through the observer box and the box's net flux;
+		 * the power absorbed by each recorded dispersive object */
+		world W = c->W;
+		fprintf(OUT, "%s.poynting %.9g %.9g %.9g %.9g %.9g\n", c->name,
+			poyntingX(W, 0.3f * W->xMax, 0.5f * W->yMin, 0.5f * W->yMax, 0.5f * W->zMin, 0.5f * W->zMax),
+			poyntingY(W, 0.3f * W->yMin, 0.5f * W->xMin, 0.5f * W->xMax, 0.5f * W->zMin, 0.5f * W->zMax),
+			poyntingZ(W, 0.3f * W->zMax, 0.5f * W->xMin, 0.5f * W->xMax, 0.5f * W->yMin, 0.5f * W->yMax),
+			boxOut(W), c->poyntingRun);
+		for (int a = 0; a < c->nabs; a++) fprintf(OUT, "%s.absorb%d %.9g\n", c->name, a, objectAbsorption(W, c->absorbers[a]));
+	}
 	dumpState(c->name, c->W);
 	if (c->P) dumpPhasers(c->name, c->P, stepped);
 }

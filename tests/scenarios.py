@@ -127,7 +127,7 @@ def golden(name):
     return parse((GOLDEN / f"{name}.txt").read_text())
 
 
-_FLOAT_POST = re.compile(r"\.(flux|ff\d+)$")          # double-precision post-processing
+_FLOAT_POST = re.compile(r"\.(flux|ff\d+|poynting|absorb\d+)$")   # double-precision post-processing
 _PHASER_DATA = re.compile(r"\.ph\d+\.(xMin|xMax|yMin|yMax|zMin|zMax)\.")
 
 

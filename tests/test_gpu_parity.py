@@ -158,6 +158,20 @@ def test_flux_device_sums_bitwise(prec):
         assert any(o[k] != 0 for k in keys)
 
 
+@pytest.mark.parametrize("prec", ["ref", "fp32"])
+def test_observers_device_sums_bitwise(prec):
+    """poyntingX/Y/Z/Out and objectAbsorption (src/output.c:233-435) take their sums from the
+    device (k_plane_sums, k_absorption) without copying the world to the host, added in the
+    order of the oracle's restatement: the values at the last step and the running sum of
+    poyntingOut every 25 steps equal the oracle's to the bit in every scenario."""
+    for name in sorted(n for n in SCENARIOS if n != "kpoints"):
+        print(f"[gpu] {name} {prec}: observers", flush=True)
+        g, o = run_scenario("hip", name, 60, prec), run_scenario("oracle", name, 60, prec)
+        keys = [k for k in o if k.endswith(".poynting") or ".absorb" in k]
+        assert keys and all(k in g for k in keys)
+        assert all(g[k] == o[k] for k in keys), [(k, g[k], o[k]) for k in keys if g[k] != o[k]][:4]
+
+
 @pytest.mark.parametrize("name", WITH_PHASERS)
 def test_phaser_read_between_flushes(name):
     """Phaser updates reach the accumulators eight samples at a time; a read after 13 steps
