@@ -62,15 +62,7 @@ static int nodeX(world W, float x) { return nearestNode(W, W->itox, x); }
 static int nodeY(world W, float y) { return nearestNode(W, W->jtoy, y); }
 static int nodeZ(world W, float z) { return nearestNode(W, W->ktoz, z); }
 
-/* One axis of a world: pointers to its members in the reference's struct. */
-typedef struct {
-	int *Min, *Max, *Num, *MIN, *MAX, *NUM, *minSur, *maxSur;
-	float *lo, *hi, *LO, *HI, *size, *step, *cos, *sin;
-	float (**coord)(world, float);
-	int (**node)(world, float);
-} axis;
-
-static axis axisOf(world W, int a)
+axis axisOf(world W, int a)
 {
 	if (a == 0) return (axis) {&W->iMin, &W->iMax, &W->iNum, &W->iMIN, &W->iMAX, &W->iNUM, &W->xMinSur, &W->xMaxSur,
 		&W->xMin, &W->xMax, &W->xMIN, &W->xMAX, &W->xSize, &W->dx, &W->coskx, &W->sinkx, &W->itox, &W->xtoi};

@@ -9,6 +9,14 @@
 #include "alis_dev.h"
 
 /* grid.c */
+/* One axis of a world (0 x, 1 y, 2 z): pointers to its members in the reference's struct. */
+typedef struct {
+	int *Min, *Max, *Num, *MIN, *MAX, *NUM, *minSur, *maxSur;
+	float *lo, *hi, *LO, *HI, *size, *step, *cos, *sin;
+	float (**coord)(world, float);
+	int (**node)(world, float);
+} axis;
+axis axisOf(world W, int a);
 float ***makeFieldFilled(int iMin, int iMax, int jMin, int jMax, int kMin, int kMax, float v);
 void resetField3(float ***f, int iMin, int iMax, int jMin, int jMax, int kMin, int kMax);
 void makeAuxFieldArrays(world W, vfield *F, coeffs *C);

@@ -247,22 +247,28 @@ tfsf (createTFSF)(world W, dom DOM, sur SUR)
 	while (*chain) chain = &(*chain)->S;
 	*chain = S;
 
+	/* the auxiliary world: an axis marked NIL shrinks to two cells across (x, y: its
+	 * boundary follows the main world's -- mirror, Bloch or periodic) or keeps the main
+	 * domain plus a cell each side between perfectly matched layers (z) */
 	dom Dom = W->Dom;
 	sur Sur = SUR;
-	if (SUR.x.MinSur==NIL) {
-		Dom.x.Min = (W->xMinSur==SYM) ? 0 : W->xMIN;
-		Dom.x.Max = (W->xMinSur==SYM) ? 0 : W->itox(W, 1+W->xtoi(W, W->xMIN));
-		Sur.x.MinSur = (W->xMinSur==SYM) ? SYM : (W->xMinSur==BBC) ? BBC : PBC;
-	}
-	if (SUR.y.MinSur==NIL) {
-		Dom.y.Min = (W->yMinSur==SYM) ? 0 : W->yMIN;
-		Dom.y.Max = (W->yMinSur==SYM) ? 0 : W->jtoy(W, 1+W->ytoj(W, W->yMIN));
-		Sur.y.MinSur = (W->yMinSur==SYM) ? SYM : (W->yMinSur==BBC) ? BBC : PBC;
-	}
-	if (SUR.z.MinSur==NIL) {
-		Dom.z.Min = W->ktoz(W, W->ztok(W, W->zMin)-1);
-		Dom.z.Max = W->ktoz(W, W->ztok(W, W->zMax)+1);
-		Sur.z.MinSur = Sur.z.MaxSur = PML;
+	float *domLo[3] = {&Dom.x.Min, &Dom.y.Min, &Dom.z.Min}, *domHi[3] = {&Dom.x.Max, &Dom.y.Max, &Dom.z.Max};
+	float *surLo[3] = {&Sur.x.MinSur, &Sur.y.MinSur, &Sur.z.MinSur};
+	const float given[3] = {SUR.x.MinSur, SUR.y.MinSur, SUR.z.MinSur};
+	for (int a = 0; a < 3; a++) {
+		if (given[a] != NIL) continue;
+		const axis X = axisOf(W, a);
+		if (a < 2) {
+			const int sym = *X.minSur == SYM;
+			*domLo[a] = sym ? 0 : *X.LO;
+			*domHi[a] = sym ? 0 : (*X.coord)(W, 1 + (*X.node)(W, *X.LO));
+			*surLo[a] = sym ? SYM : *X.minSur == BBC ? BBC : PBC;
+		}
+		else {
+			*domLo[a] = (*X.coord)(W, (*X.node)(W, *X.lo) - 1);
+			*domHi[a] = (*X.coord)(W, (*X.node)(W, *X.hi) + 1);
+			Sur.z.MinSur = Sur.z.MaxSur = PML;
+		}
 	}
 
 	S->W = createWorld(Dom, W->Res, Sur, "");
@@ -296,47 +302,74 @@ tfsf (createTFSF)(world W, dom DOM, sur SUR)
 		}
 	}
 
-	if (!DOM.x.Max && DOM.x.Min>0 || SUR.x.MinSur==SYM) DOM.x.Min = -(DOM.x.Max = DOM.x.Max ? DOM.x.Max : DOM.x.Min);
-	if (!DOM.y.Max && DOM.y.Min>0 || SUR.y.MinSur==SYM) DOM.y.Min = -(DOM.y.Max = DOM.y.Max ? DOM.y.Max : DOM.y.Min);
-	if (!DOM.z.Max && DOM.z.Min>0 || SUR.z.MinSur==SYM) DOM.z.Min = -(DOM.z.Max = DOM.z.Max ? DOM.z.Max : DOM.z.Min);
-	S->iMin = W->xMinSur==SYM ? 0 : DOM.x.Min == DOM.x.Max ? W->iMin : W->iMin > W->xtoi(W, DOM.x.Min) ? W->iMIN : W->xtoi(W, DOM.x.Min);
-	S->jMin = W->yMinSur==SYM ? 0 : DOM.y.Min == DOM.y.Max ? W->jMin : W->jMin > W->ytoj(W, DOM.y.Min) ? W->jMIN : W->ytoj(W, DOM.y.Min);
-	S->kMin = W->zMinSur==SYM ? 0 : DOM.z.Min == DOM.z.Max ? W->kMin : W->kMin > W->ztok(W, DOM.z.Min) ? W->kMIN : W->ztok(W, DOM.z.Min);
-	S->iMax = DOM.x.Min == DOM.x.Max ? W->iMax : W->iMax < W->xtoi(W, DOM.x.Max) ? W->iMAX : W->xtoi(W, DOM.x.Max);
-	S->jMax = DOM.y.Min == DOM.y.Max ? W->jMax : W->jMax < W->ytoj(W, DOM.y.Max) ? W->jMAX : W->ytoj(W, DOM.y.Max);
-	S->kMax = DOM.z.Min == DOM.z.Max ? W->kMax : W->kMax < W->ztok(W, DOM.z.Max) ? W->kMAX : W->ztok(W, DOM.z.Max);
-	S->iMIN = (S->iMin == W->iMIN) ? S->iMin : S->iMin + 1, S->iMAX = (S->iMax == W->iMAX) ? S->iMax + 1 : S->iMax;
-	S->jMIN = (S->jMin == W->jMIN) ? S->jMin : S->jMin + 1, S->jMAX = (S->jMax == W->jMAX) ? S->jMax + 1 : S->jMax;
-	S->kMIN = (S->kMin == W->kMIN) ? S->kMin : S->kMin + 1, S->kMAX = (S->kMax == W->kMAX) ? S->kMax + 1 : S->kMax;
+	/* the total-field box: per axis the nodes of DOM (a single extent is a half width),
+	 * clipped to the grid -- a box reaching past the domain takes the whole grid -- and the
+	 * first and last strip planes inside it */
+	float *boxLo[3] = {&DOM.x.Min, &DOM.y.Min, &DOM.z.Min}, *boxHi[3] = {&DOM.x.Max, &DOM.y.Max, &DOM.z.Max};
+	const float boxSur[3] = {SUR.x.MinSur, SUR.y.MinSur, SUR.z.MinSur};
+	int *tMin[3] = {&S->iMin, &S->jMin, &S->kMin}, *tMax[3] = {&S->iMax, &S->jMax, &S->kMax};
+	int *tMIN[3] = {&S->iMIN, &S->jMIN, &S->kMIN}, *tMAX[3] = {&S->iMAX, &S->jMAX, &S->kMAX};
+	for (int a = 0; a < 3; a++) {
+		const axis X = axisOf(W, a);
+		if ((!*boxHi[a] && *boxLo[a] > 0) || boxSur[a] == SYM) *boxLo[a] = -(*boxHi[a] = *boxHi[a] ? *boxHi[a] : *boxLo[a]);
+		const int flat = *boxLo[a] == *boxHi[a];
+		*tMin[a] = *X.minSur == SYM ? 0 : flat ? *X.Min : *X.Min > (*X.node)(W, *boxLo[a]) ? *X.MIN : (*X.node)(W, *boxLo[a]);
+		*tMax[a] = flat ? *X.Max : *X.Max < (*X.node)(W, *boxHi[a]) ? *X.MAX : (*X.node)(W, *boxHi[a]);
+		*tMIN[a] = *tMin[a] == *X.MIN ? *tMin[a] : *tMin[a] + 1;
+		*tMAX[a] = *tMax[a] == *X.MAX ? *tMax[a] + 1 : *tMax[a];
+	}
 
 	if (W->CE->M && W->CE->O) putBackground(W, S, W->CE->M, W->CE->O);
 	return S;
 }
 
 
-/* One injection plane at z index k; a Bloch world gets per-cell phases of the in-plane
- * wave vector (src/source.c:337-355). */
+/* One injection plane of component F at z index k: scalar amplitude and phase; on a Bloch
+ * world, per-cell phases phase + kx x + ky y at the component's staggered position
+ * (src/source.c:337-355). */
 static void sourcePlane(world W, field F, int k, float kx, float ky, waveform waveForm, float wavelength, float supplement, float resolution, float amplitude, float phase)
 {
 	source S = createSource(W, F, waveForm, wavelength, supplement, resolution);
 	S->amplitude = amplitude;
 	S->phase = phase;
 	S->iMin = W->iMIN, S->iMax = W->iMAX, S->jMin = W->jMIN, S->jMax = W->jMAX, S->kMin = k, S->kMax = k;
-	if (W->complexField) {
-		float di = (F==Ex||F==Hy||F==iEx||F==iHy) ? 0.5 : 0;
-		float dj = (F==Ey||F==Hx||F==iEy||F==iHx) ? 0.5 : 0;
-		S->Phase = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
-		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++)
-			S->Phase[i][j][k] = phase + kx * W->itox(W, i-di) + ky * W->jtoy(W, j-dj);
-	}
+	if (!W->complexField) return;
+	const int xStag = F==Ex || F==Hy || F==iEx || F==iHy, yStag = F==Ey || F==Hx || F==iEy || F==iHx;
+	const float di = xStag ? 0.5 : 0, dj = yStag ? 0.5 : 0;
+	S->Phase = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++)
+		S->Phase[i][j][k] = phase + kx * W->itox(W, i-di) + ky * W->jtoy(W, j-dj);
 }
 
-/* E and H injection planes of a plane wave of polarisation pol and direction
- * (theta, phi); sets the Bloch phases / symmetry parities of W (src/source.c:358-411). */
+/*
+ * The E and H injection planes of a plane wave of polarisation pol from direction (theta,
+ * phi) (src/source.c:358-411).  The wave enters through the z face it travels away from: E
+ * on plane kE, H on kH (half a cell further), in the medium of that face (index n).  The
+ * in-plane wave vector sets the Bloch phases; on a symmetry plane the polarisation sets the
+ * parity.  Each polarisation injects the tangential components below -- amplitude
+ * base * a * b, dropped where the azimuth makes it vanish -- into the real part and, on a
+ * Bloch world, into the imaginary part a quarter period later; the chain order is the one
+ * listed.
+ */
+typedef struct {
+	field *F, *iF;
+	int hPlane;                 /* 0: E plane kE, 1: H plane kH */
+	int needs;                  /* 1: skipped for phi = 90, 270;  2: skipped for phi = 0, 180 */
+	int sign;                   /* sign of the base amplitude */
+	int a, b;                   /* factors: 0 none, 1 cos(phi), 2 sin(phi), 3 cos(theta) */
+} planeTerm;
+
+static const planeTerm planeTerms[2][4] = {
+	/* P: E along the meridian, H across it */
+	{{Ex, iEx, 0, 1, +1, 1, 3}, {Ey, iEy, 0, 2, +1, 2, 3}, {Hx, iHx, 1, 2, +1, 2, 0}, {Hy, iHy, 1, 1, -1, 1, 0}},
+	/* S: E across the meridian, H along it */
+	{{Ex, iEx, 0, 2, -1, 2, 0}, {Ey, iEy, 0, 1, +1, 1, 0}, {Hx, iHx, 1, 1, +1, 1, 3}, {Hy, iHy, 1, 2, +1, 2, 3}},
+};
+
 static void sourcePlanes(world W, tfield pol(phaser,float,float), float theta, float phi, waveform waveForm, float wavelength, float supplement, float resolution, float amplitude, float phase)
 {
-	int kE = (90<theta && theta<270) ? W->kMin : (theta<90 || 270<theta) ? W->kMax : 0;
-	int kH = (90<theta && theta<270) ? W->kMin : (theta<90 || 270<theta) ? W->kMax+1 : 0;
+	const int up = 90 < theta && theta < 270, down = theta < 90 || 270 < theta;
+	const int kE = up ? W->kMin : down ? W->kMax : 0, kH = up ? W->kMin : down ? W->kMax+1 : 0;
 	float sinth = sinf(theta*PI/180) * sqrtf(W->CE->z?1/W->CE->z[W->iMin][W->jMin][kH]:1);
 	float costh = cosf(theta*PI/180) * sqrtf(W->CE->z?1/W->CE->z[W->iMin][W->jMin][kH]:1);
 	float sinph = sinf(phi*PI/180);
@@ -346,43 +379,32 @@ static void sourcePlanes(world W, tfield pol(phaser,float,float), float theta, f
 
 	if (W->xMinSur == BBC) W->coskx = cosf(kx*(W->xMax-W->xMin)), W->sinkx = sinf(kx*(W->xMax-W->xMin));
 	if (W->yMinSur == BBC) W->cosky = cosf(ky*(W->yMax-W->yMin)), W->sinky = sinf(ky*(W->yMax-W->yMin));
-	if ((W->xMinSur == SYM) && (phi==90 || phi==270)) W->coskx = pol==Ppol ? -1 : 1;
-	if ((W->xMinSur == SYM) && (phi==0  || phi==180) && (theta==0 || theta==180)) W->coskx = pol==Ppol ? 1 : -1;
-	if ((W->yMinSur == SYM) && (phi==0  || phi==180)) W->cosky = pol==Ppol ? -1 : 1;
-	if ((W->yMinSur == SYM) && (phi==90 || phi==270) && (theta==0 || theta==180)) W->cosky = pol==Ppol ? 1 : -1;
+	const int alongY = phi==90 || phi==270, alongX = phi==0 || phi==180, normal = theta==0 || theta==180;
+	if (W->xMinSur == SYM && alongY) W->coskx = pol==Ppol ? -1 : 1;
+	if (W->xMinSur == SYM && alongX && normal) W->coskx = pol==Ppol ? 1 : -1;
+	if (W->yMinSur == SYM && alongX) W->cosky = pol==Ppol ? -1 : 1;
+	if (W->yMinSur == SYM && alongY && normal) W->cosky = pol==Ppol ? 1 : -1;
 
-	if (waveForm) {
+	if (waveForm && (pol == Ppol || pol == Spol)) {
 		float n = (W->CE->z ? 1/sqrtf(W->CE->z[W->iMin][W->jMin][kH]) : 1);
 		float kzdz = W->CE->dz[kE] * PI * fabsf(costh) / wavelength;
 		float ampE = W->CE->dtdz[kE] * (amplitude ? amplitude : 1) * 2 / ((1+n) * sqrtf(n) * n);
 		float ampH = W->CH->dtdz[kH] * (amplitude ? amplitude : 1) * 2 / ((1+n) * sqrtf(n));
-		int notX = phi != 90 && phi != 270, notY = phi != 0 && phi != 180;
-		float pE = (phase+90)*PI/180, pH = (phase+90)*PI/180+kzdz, qE = phase*PI/180, qH = phase*PI/180+kzdz;
-
-		if (pol == Ppol) {
-			if (notX) sourcePlane(W, Ex, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*cosph*costh, pE);
-			if (notY) sourcePlane(W, Ey, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*sinph*costh, pE);
-			if (notY) sourcePlane(W, Hx, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*sinph, pH);
-			if (notX) sourcePlane(W, Hy, kH, kx, ky, waveForm, wavelength, supplement, resolution,-ampH*cosph, pH);
-			if (W->complexField) {
-				if (notX) sourcePlane(W, iEx, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*cosph*costh, qE);
-				if (notY) sourcePlane(W, iEy, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*sinph*costh, qE);
-				if (notY) sourcePlane(W, iHx, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*sinph, qH);
-				if (notX) sourcePlane(W, iHy, kH, kx, ky, waveForm, wavelength, supplement, resolution,-ampH*cosph, qH);
+		/* phases of the real parts (a quarter period ahead) and of the imaginary parts; H is
+		 * half a cell further along the propagation */
+		const float re[2] = {(phase+90)*PI/180, (phase+90)*PI/180+kzdz}, im[2] = {phase*PI/180, phase*PI/180+kzdz};
+		const float factor[4] = {1, cosph, sinph, costh};
+		const planeTerm *T = planeTerms[pol == Spol];
+		for (int part = 0; part <= W->complexField; part++)
+			for (int q = 0; q < 4; q++) {
+				if ((T[q].needs == 1 && alongY) || (T[q].needs == 2 && alongX)) continue;
+				float base = T[q].hPlane ? ampH : ampE;
+				if (T[q].sign < 0) base = -base;
+				float amp = base * factor[T[q].a];
+				if (T[q].b) amp = amp * factor[T[q].b];
+				sourcePlane(W, part ? T[q].iF : T[q].F, T[q].hPlane ? kH : kE, kx, ky, waveForm, wavelength, supplement, resolution,
+				            amp, part ? im[T[q].hPlane] : re[T[q].hPlane]);
 			}
-		}
-		if (pol == Spol) {
-			if (notY) sourcePlane(W, Ex, kE, kx, ky, waveForm, wavelength, supplement, resolution,-ampE*sinph, pE);
-			if (notX) sourcePlane(W, Ey, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*cosph, pE);
-			if (notX) sourcePlane(W, Hx, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*cosph*costh, pH);
-			if (notY) sourcePlane(W, Hy, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*sinph*costh, pH);
-			if (W->complexField) {
-				if (notY) sourcePlane(W, iEx, kE, kx, ky, waveForm, wavelength, supplement, resolution,-ampE*sinph, qE);
-				if (notX) sourcePlane(W, iEy, kE, kx, ky, waveForm, wavelength, supplement, resolution, ampE*cosph, qE);
-				if (notX) sourcePlane(W, iHx, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*cosph*costh, qH);
-				if (notY) sourcePlane(W, iHy, kH, kx, ky, waveForm, wavelength, supplement, resolution, ampH*sinph*costh, qH);
-			}
-		}
 	}
 	if (wavelength && W->T != wavelength/W->dt) W->f = 1/wavelength, W->T = wavelength/W->dt;
 }
@@ -409,44 +431,56 @@ void (planewave)(world W, dom Dom, tfield pol(phaser,float,float), float theta, 
 }
 
 
-/* Gaussian beam of spot size `spotsize` focused at (x,y,z) launched from a z plane
- * (src/source.c:435-477). */
+/* A Gaussian beam launched from the z face it travels away from (src/source.c:435-477): in
+ * the medium of index n at the launch plane, with waist w0 = lambda spot / (n sqrt(2 ln 2))
+ * (spot: the full width at half maximum in wavelengths) at the focus (x, y, z), the launch
+ * plane, a distance d from the focus, carries the field of the beam there -- amplitude
+ * w0/w(d) e^{-r^2/w(d)^2}, phase -k r^2 / (2 R(d)) plus the Gouy phase atan(d/z0) -- with
+ * the Rayleigh range z0 = pi n w0^2 / lambda, w(d) = w0 sqrt(1 + (d/z0)^2) and
+ * R(d) = d (1 + (z0/d)^2); r is measured at the component's staggered position, along the
+ * axes whose focus coordinate is finite.  Without a spot size the plane is uniform. */
+typedef struct { float w0, z0, w, R, gouy, k; } gaussian;
+
+static gaussian gaussianAt(float n, float wavelength, float spot, float d)
+{
+	gaussian g;
+	g.k = 2 * PI * n / wavelength;
+	g.w0 = wavelength * (spot ? spot : 1) / (n * sqrtf(-2.0 * logf(0.5)));
+	g.z0 = PI * n * sq(g.w0) / wavelength;
+	g.w = g.w0 * sqrtf(1 + sq(d/g.z0));
+	g.R = d * (1 + sq(g.z0/d));
+	g.gouy = atanf(d/g.z0);
+	return g;
+}
+
 static void beam(world W, field F, float x, float y, float z, float theta, waveform waveForm, float wavelength, float supplement, float spotsize, float amplitude, float phase)
 {
 	while (theta < 0) theta += 360;
 	while (theta >= 360) theta -= 360;
 	if (W->xMinSur==SYM) W->coskx = (F==Ex || F==iEx) ? 1 : -1;
 	if (W->yMinSur==SYM) W->cosky = (F==Ey || F==iEy) ? 1 : -1;
-
 	if (waveForm) {
-		int k = (90<theta && theta<270) ? W->kMin : (theta<90 || 270<theta) ? W->kMax : 0;
-		float n = W->CE->z ? 1/sqrtf(W->CE->z[W->xtoi(W,MIN(x,W->xMax))][W->ytoj(W,MIN(y,W->yMax))][k]) : 1;
+		const int k = (90<theta && theta<270) ? W->kMin : (theta<90 || 270<theta) ? W->kMax : 0;
+		const float n = W->CE->z ? 1/sqrtf(W->CE->z[W->xtoi(W,MIN(x,W->xMax))][W->ytoj(W,MIN(y,W->yMax))][k]) : 1;
 		source S = createSource(W, F, waveForm, wavelength, supplement, amplitude);
 		S->iMin = W->iMIN, S->iMax = W->iMAX, S->jMin = W->jMIN, S->jMax = W->jMAX, S->kMin = k, S->kMax = k;
-		if (!amplitude) amplitude = 1;
-		amplitude /= sqrtf(n*n*n);
-
-		if (spotsize) {
-			float kz = 2 * PI * n / wavelength;
-			float zz = fabsf(z - W->ktoz(W,k));
-			float w0 = wavelength * (spotsize ? spotsize : 1) / (n * sqrtf(-2.0 * logf(0.5)));
-			float z0 = PI * n * sq(w0) / wavelength;
-			float wz = w0 * sqrtf(1 + sq(zz/z0));
-			float Rz = zz * (1 + sq(z0/zz));
-			float nz = atanf(zz/z0);
-			S->Amplitude = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
-			S->Phase = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
-			for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) {
-				float r2 = 0;
-				if (x < INF) r2 += sq(x - W->itox(W,F==Ex?i-0.5:i));
-				if (y < INF) r2 += sq(y - W->jtoy(W,F==Ey?j-0.5:j));
-				S->Amplitude[i][j][k] = amplitude * w0/wz * expf(-r2/sq(wz));
-				S->Phase[i][j][k] = phase * PI / 180 - kz * (0.5 * r2 / Rz) + nz;
-			}
+		const float amp = (amplitude ? amplitude : 1) / sqrtf(n*n*n);
+		if (!spotsize) {
+			S->amplitude = amp;
+			S->phase = phase * PI / 180;
 		}
 		else {
-			S->amplitude = amplitude;
-			S->phase = phase * PI / 180;
+			const gaussian g = gaussianAt(n, wavelength, spotsize, fabsf(z - W->ktoz(W,k)));
+			S->Amplitude = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+			S->Phase = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+			for (int i = S->iMin; i <= S->iMax; i++)
+				for (int j = S->jMin; j <= S->jMax; j++) {
+					float r2 = 0;
+					if (x < INF) r2 += sq(x - W->itox(W, F==Ex ? i-0.5 : i));
+					if (y < INF) r2 += sq(y - W->jtoy(W, F==Ey ? j-0.5 : j));
+					S->Amplitude[i][j][k] = amp * g.w0/g.w * expf(-r2/sq(g.w));
+					S->Phase[i][j][k] = phase * PI / 180 - g.k * (0.5 * r2 / g.R) + g.gouy;
+				}
 		}
 	}
 	if (wavelength && W->T != wavelength/W->dt) W->f = 1/wavelength, W->T = wavelength/W->dt;
