@@ -32,6 +32,7 @@ void alisCommit(world W);          /* create the device world if absent         
 void alisInvalidate(world W);      /* setup changed: bring state home, drop the device */
 void alisDropDevice(world W);      /* host arrays are newer: drop the device copy      */
 void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1);
+void alisSyncArrays(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask);   /* the arrays in mask over a box */
 void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask);   /* get(): box copy of the arrays in mask, batched per half step */
 void alisStepped(void);            /* device fields changed: probe copies are stale */
 void alisSyncPhaser(phaser P);

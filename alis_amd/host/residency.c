@@ -217,6 +217,20 @@ void alisDropDevice(world W)
 	releaseTree(W);
 }
 
+/* The host mirrors of the arrays in mask (alis_dev_sync_boxes) over a box, without
+ * touching the rest of the world's mirrors. */
+void alisSyncArrays(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask)
+{
+	if (W->isAux && W->parent) {
+		syncWorld(W);
+		return;
+	}
+	if (!W->dev || !W->hostStale) return;
+	const int box[7] = {i0, i1, j0, j1, k0, k1, mask};
+	alis_dev_world *D = W->dev;
+	alisFatal("box sync", alis_dev_sync_boxes(&D, box, 1));
+}
+
 void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 {
 	if (!W->dev || !W->hostStale) return;

@@ -1,7 +1,9 @@
 /*
- * Slices of the world and their writers (src/slices.c:1-427): snapshots, time averages and
- * single-frequency phasors on 1-D/2-D cuts, written as text, PNG (own encoder on zlib) or
- * the array container of h5write.  Evaluated on the host arrays after a device sync.
+ * Slices (src/slices.c:1-427): cuts of a world -- a node, a line or a plane of nodes --
+ * sampled with a field accessor and written as text, PNG (an encoder of its own over zlib)
+ * or the array container of h5write.  A cut is sampled from host mirrors refreshed over its
+ * own box only, in the arrays the accessor reads, not from a copy of the whole world.  Worlds
+ * on graded grids are resampled onto the uniform grid before they are written.
  */
 #include <string.h>
 #include <limits.h>
@@ -9,92 +11,117 @@
 #include <zlib.h>
 #include "internal.h"
 
-typedef struct squeue *squeue;
-struct squeue {
-	world W;
-	char *ID;
-	float max, Max, ***F, ***G;
-	long int N;
-	method M;
-	squeue Q;
-};
+/* Accessors of the permittivity, which read no device state. */
+static int staticField(field F)
+{
+	return F==RI || F==LogRI || F==ContourRI || F==Contour || F==raweEffx || F==raweEffy || F==raweEffz || F==eEff;
+}
 
-/* Resample a non-uniform slice onto the uniform grid of spacing dx, dy, dz by piecewise
- * linear interpolation along each non-degenerate axis (src/slices.c:25-64). */
+/* Refresh the host mirrors that accessor F reads on cut S: the cut's box, mirrored across a
+ * symmetry plane and two cells wider (the accessors reach one cell beyond the node), in the
+ * component array of a component accessor or in every array otherwise. */
+static void refreshCut(world W, field F, slice S)
+{
+	if (staticField(F) || !W->dev || !W->hostStale) return;
+	const int from[3] = {S->iMin, S->jMin, S->kMin}, to[3] = {S->iMax, S->jMax, S->kMax};
+	int lo[3], hi[3];
+	for (int a = 0; a < 3; a++) {
+		const int sym = *axisOf(W, a).minSur == SYM && from[a] < 0;
+		lo[a] = (sym ? 0 : from[a]) - 2;
+		hi[a] = (sym ? MAX(-from[a], to[a]) : to[a]) + 2;
+	}
+	int isH, m, axis, mask = ALIS_DEV_PROBE_ALL;
+	if (fieldComponent(F, &isH, &m, &axis) || rawComponent(F, &isH, &m, &axis)) mask = 1 << (isH * 6 + m * 3 + axis);
+	alisSyncArrays(W, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], mask);
+}
+
+/* F over the cut into S->F; returns the largest magnitude. */
+static float sampleCut(world W, field F, slice S)
+{
+	refreshCut(W, F, S);
+	float max = 0;
+	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++) {
+		const float v = S->F[i][j][k] = F(W, i, j, k);
+		if (max < fabsf(v)) max = fabsf(v);
+	}
+	return max;
+}
+
+
+/* ---- geometry ---- */
+
+/* Linear resampling along axis a of a cut on a graded grid: the value at the uniform node n
+ * (position n d) from the two graded nodes around it (src/slices.c:25-64).  Axes before a
+ * are already resampled (their uniform range); axes after a keep the cut's node range. */
+static void resampleAxis(world W, slice S, float ***src, float ***dst, int a)
+{
+	const axis X = axisOf(W, a);
+	const int cut[3][2] = {{S->iMin, S->iMax}, {S->jMin, S->jMax}, {S->kMin, S->kMax}};
+	const int all[3][2] = {{S->iMIN, S->iMAX}, {S->jMIN, S->jMAX}, {S->kMIN, S->kMAX}};
+	const int p = a == 0 ? 1 : 0, q = a == 2 ? 1 : 2;
+	const int *rp = p < a ? all[p] : cut[p], *rq = q < a ? all[q] : cut[q];
+	const float d = *X.step;
+	for (int u = cut[a][0]; u < cut[a][1]; u++) {
+		const float c0 = (*X.coord)(W, u), c1 = (*X.coord)(W, u+1);
+		const int n0 = c0/d + (u > 0 ? 1 : 0), n1 = c1/d - (u < 0 ? 1 : 0);
+		for (int s = rp[0]; s <= rp[1]; s++)
+			for (int t = rq[0]; t <= rq[1]; t++)
+				for (int n = n0; n <= n1; n++) {
+					int g0[3], g1[3], o[3];
+					g0[a] = u, g1[a] = u + 1, o[a] = n;
+					g0[p] = g1[p] = o[p] = s;
+					g0[q] = g1[q] = o[q] = t;
+					dst[o[0]][o[1]][o[2]] = (src[g0[0]][g0[1]][g0[2]] * (c1 - n*d) + src[g1[0]][g1[1]][g1[2]] * (n*d - c0)) / (c1 - c0);
+				}
+	}
+}
+
 static float ***resampleUniform(world W, slice S, float ***F)
 {
-	float ***T, ***G = makeField(S->iMIN, S->iMAX, S->jMIN, S->jMAX, S->kMIN, S->kMAX);
-	if (S->iMin != S->iMIN || S->iMax != S->iMAX) {
-		T = F, F = G, G = T;
-		for (int i = S->iMin; i < S->iMax; i++) {
-			float x0 = W->itox(W, i), x1 = W->itox(W, i+1);
-			int n0 = x0/W->dx+(i>0?1:0), n1 = x1/W->dx-(i<0?1:0);
-			for (int j = S->jMin; j <= S->jMax; j++) for (int n = n0; n <= n1; n++) for (int k = S->kMin; k <= S->kMax; k++)
-				F[n][j][k] = (G[i][j][k] * (x1 - n*W->dx) + G[i+1][j][k] * (n*W->dx - x0)) / (x1 - x0);
-		}
-	}
-	if (S->jMin != S->jMIN || S->jMax != S->jMAX) {
-		T = F, F = G, G = T;
-		for (int j = S->jMin; j < S->jMax; j++) {
-			float y0 = W->jtoy(W, j), y1 = W->jtoy(W, j+1);
-			int n0 = y0/W->dy+(j>0?1:0), n1 = y1/W->dy-(j<0?1:0);
-			for (int i = S->iMIN; i <= S->iMAX; i++) for (int n = n0; n <= n1; n++) for (int k = S->kMin; k <= S->kMax; k++)
-				F[i][n][k] = (G[i][j][k] * (y1 - n*W->dy) + G[i][j+1][k] * (n*W->dy - y0)) / (y1 - y0);
-		}
-	}
-	if (S->kMin != S->kMIN || S->kMax != S->kMAX) {
-		T = F, F = G, G = T;
-		for (int k = S->kMin; k < S->kMax; k++) {
-			float z0 = W->ktoz(W, k), z1 = W->ktoz(W, k+1);
-			int n0 = z0/W->dz+(k>0?1:0), n1 = z1/W->dz-(k<0?1:0);
-			for (int i = S->iMIN; i <= S->iMAX; i++) for (int j = S->jMIN; j <= S->jMAX; j++) for (int n = n0; n <= n1; n++)
-				F[i][j][n] = (G[i][j][k] * (z1 - n*W->dz) + G[i][j][k+1] * (n*W->dz - z0)) / (z1 - z0);
-		}
+	float ***G = makeField(S->iMIN, S->iMAX, S->jMIN, S->jMAX, S->kMIN, S->kMAX);
+	const int graded[3] = {S->iMin != S->iMIN || S->iMax != S->iMAX, S->jMin != S->jMIN || S->jMax != S->jMAX, S->kMin != S->kMIN || S->kMax != S->kMAX};
+	for (int a = 0; a < 3; a++) {
+		if (!graded[a]) continue;
+		float ***t = F;
+		F = G, G = t;
+		resampleAxis(W, S, G, F, a);
 	}
 	removeField(G, S->iMIN, S->jMIN, S->kMIN);
 	return F;
 }
 
-/* Cut of the world between the given coordinates; 1-D and 2-D cuts of worlds with objects
- * carry a contour overlay of the refractive index (src/slices.c:67-117). */
+/* Cut between the given coordinates, clipped to the grid (mirrored across a symmetry plane);
+ * on a graded grid its storage spans the uniform nodes too.  Lines and planes of worlds with
+ * objects carry a contour of the refractive index, scaled to 0..0.5 (src/slices.c:67-117). */
 slice createSlice(world W, float xMin, float xMax, float yMin, float yMax, float zMin, float zMax)
 {
 	slice S;
 	MALLOC(S, 0, 1);
-	S->iMin = W->xtoi(W, xMin), S->iMax = W->xtoi(W, xMax);
-	S->jMin = W->ytoj(W, yMin), S->jMax = W->ytoj(W, yMax);
-	S->kMin = W->ztok(W, zMin), S->kMax = W->ztok(W, zMax);
-	if (W->Res.nug.x.m>1 || W->Res.nug.y.m>1 || W->Res.nug.z.m>1) S->interpolate = resampleUniform;
-
-	if (S->iMax > W->iMAX) S->iMax = W->iMAX;
-	if (S->jMax > W->jMAX) S->jMax = W->jMAX;
-	if (S->kMax > W->kMAX) S->kMax = W->kMAX;
-	if (S->iMin < W->iMIN && W->xMinSur!=SYM) S->iMin = W->iMIN;
-	if (S->jMin < W->jMIN && W->yMinSur!=SYM) S->jMin = W->jMIN;
-	if (S->kMin < W->kMIN && W->zMinSur!=SYM) S->kMin = W->kMIN;
-	if (S->iMin <-W->iMAX && W->xMinSur==SYM) S->iMin =-W->iMAX;
-	if (S->jMin <-W->jMAX && W->yMinSur==SYM) S->jMin =-W->jMAX;
-	if (S->kMin <-W->kMAX && W->zMinSur==SYM) S->kMin =-W->kMAX;
-	S->iMIN = (S->iMin==S->iMax || !S->interpolate) ? S->iMin : MIN(S->iMin, W->itox(W, S->iMin)/W->dx);
-	S->jMIN = (S->jMin==S->jMax || !S->interpolate) ? S->jMin : MIN(S->jMin, W->jtoy(W, S->jMin)/W->dy);
-	S->kMIN = (S->kMin==S->kMax || !S->interpolate) ? S->kMin : MIN(S->kMin, W->ktoz(W, S->kMin)/W->dz);
-	S->iMAX = (S->iMin==S->iMax || !S->interpolate) ? S->iMax : MAX(S->iMax, W->itox(W, S->iMax)/W->dx);
-	S->jMAX = (S->jMin==S->jMax || !S->interpolate) ? S->jMax : MAX(S->jMax, W->jtoy(W, S->jMax)/W->dy);
-	S->kMAX = (S->kMin==S->kMax || !S->interpolate) ? S->kMax : MAX(S->kMax, W->ktoz(W, S->kMax)/W->dz);
-	S->iNUM = 1-S->iMIN+S->iMAX;
-	S->jNUM = 1-S->jMIN+S->jMAX;
-	S->kNUM = 1-S->kMIN+S->kMAX;
+	if (W->Res.nug.x.m > 1 || W->Res.nug.y.m > 1 || W->Res.nug.z.m > 1) S->interpolate = resampleUniform;
+	const float lo[3] = {xMin, yMin, zMin}, hi[3] = {xMax, yMax, zMax};
+	int *cMin[3] = {&S->iMin, &S->jMin, &S->kMin}, *cMax[3] = {&S->iMax, &S->jMax, &S->kMax};
+	int *sMIN[3] = {&S->iMIN, &S->jMIN, &S->kMIN}, *sMAX[3] = {&S->iMAX, &S->jMAX, &S->kMAX}, *sNUM[3] = {&S->iNUM, &S->jNUM, &S->kNUM};
+	for (int a = 0; a < 3; a++) {
+		const axis X = axisOf(W, a);
+		*cMin[a] = (*X.node)(W, lo[a]), *cMax[a] = (*X.node)(W, hi[a]);
+		if (*cMax[a] > *X.MAX) *cMax[a] = *X.MAX;
+		if (*X.minSur != SYM && *cMin[a] < *X.MIN) *cMin[a] = *X.MIN;
+		if (*X.minSur == SYM && *cMin[a] < -*X.MAX) *cMin[a] = -*X.MAX;
+		const int plain = *cMin[a] == *cMax[a] || !S->interpolate;
+		*sMIN[a] = plain ? *cMin[a] : MIN(*cMin[a], (*X.coord)(W, *cMin[a]) / *X.step);
+		*sMAX[a] = plain ? *cMax[a] : MAX(*cMax[a], (*X.coord)(W, *cMax[a]) / *X.step);
+		*sNUM[a] = 1 - *sMIN[a] + *sMAX[a];
+	}
 	S->F = makeField(S->iMIN, S->iMAX, S->jMIN, S->jMAX, S->kMIN, S->kMAX);
-
-	if (!W->CE->x || !W->CE->y || !W->CE->z || (S->iMin!=S->iMax && S->jMin!=S->jMax && S->kMin!=S->kMax)) return S;
-
-	float Max = 0;
+	const int solid = S->iMin != S->iMax && S->jMin != S->jMax && S->kMin != S->kMax;
+	if (!W->CE->x || !W->CE->y || !W->CE->z || solid) return S;
 	S->C = makeField(S->iMIN, S->iMAX, S->jMIN, S->jMAX, S->kMIN, S->kMAX);
-	for (int i = S->iMIN; i <= S->iMAX; i++) for (int j = S->jMIN; j <= S->jMAX; j++) for (int k = S->kMIN; k <= S->kMAX; k++)
-		if (Max < (S->C[i][j][k] = ContourRI(W, i, j, k))) Max = S->C[i][j][k];
-	if (Max)
-		for (int i = S->iMIN; i <= S->iMAX; i++) for (int j = S->jMIN; j <= S->jMAX; j++) for (int k = S->kMIN; k <= S->kMAX; k++)
-			S->C[i][j][k] = 0.5*S->C[i][j][k]/Max;
+	float *c = &S->C[S->iMIN][S->jMIN][S->kMIN], Max = 0;
+	const long n = (long)S->iNUM * S->jNUM * S->kNUM;
+	long e = 0;
+	for (int i = S->iMIN; i <= S->iMAX; i++) for (int j = S->jMIN; j <= S->jMAX; j++) for (int k = S->kMIN; k <= S->kMAX; k++, e++)
+		if (Max < (c[e] = ContourRI(W, i, j, k))) Max = c[e];
+	for (e = 0; Max && e < n; e++) c[e] = 0.5*c[e]/Max;
 	return S;
 }
 
@@ -106,53 +133,67 @@ void deleteSlice(slice S)
 }
 
 
+/* ---- writers ---- */
+
+/* The extended axes of a cut: their count and node counts, in x, y, z order. */
+static int cutAxes(slice S, int axes[3], int n[3])
+{
+	const int ext[3] = {S->iMin != S->iMax, S->jMin != S->jMax, S->kMin != S->kMax}, num[3] = {S->iNUM, S->jNUM, S->kNUM};
+	int d = 0;
+	for (int a = 0; a < 3; a++) if (ext[a]) axes[d] = a, n[d++] = num[a];
+	return d;
+}
+
 void (h5)(world W, slice S, float Max, char *F, char *filename, unsigned long int *cmap)
 {
 	(void)Max, (void)F, (void)cmap;
-	int dim[3] = {S->iMin==S->iMax?0:S->iNUM, S->jMin==S->jMax?0:S->jNUM, S->kMin==S->kMax?0:S->kNUM};
 	if (S->interpolate) S->F = S->interpolate(W, S, S->F);
+	int dim[3] = {S->iMin == S->iMax ? 0 : S->iNUM, S->jMin == S->jMax ? 0 : S->jNUM, S->kMin == S->kMax ? 0 : S->kNUM};
 	h5write(dim, &S->F[S->iMIN][S->jMIN][S->kMIN], "%s", filename);
 }
 
+/* A line as "coordinate<TAB>value" rows; a plane as rows of the second axis from its last
+ * node down, each a tab-separated run along the first axis (src/slices.c:128-152). */
 void (txt)(world W, slice S, float Max, char *F, char *filename, unsigned long int *cmap)
 {
 	(void)Max, (void)F, (void)cmap;
-	int d = 0, dim[3] = {0};
+	int axes[3], n[3];
+	const int d = cutAxes(S, axes, n);
+	if (d == 3) printf("Can't write 3D slice to txt file.\n"), exit(0);
 	char name[1100];
 	snprintf(name, sizeof name, "%s.txt", filename);
-	if (S->iMin != S->iMax) dim[d++] = S->iNUM;
-	if (S->jMin != S->jMax) dim[d++] = S->jNUM;
-	if (S->kMin != S->kMax) dim[d++] = S->kNUM;
-	if (d==3) printf("Can't write 3D slice to txt file.\n"), exit(0);
 	FILE *fp = fopen(name, "w");
 	if (!fp) return;
-	if (d==1) {
-		if (S->iMin != S->iMax) for (int i = S->iMin; i <= S->iMax; i++) fprintf(fp, "%f\t%e\n", W->itox(W, i), S->F[i][S->jMin][S->kMin]);
-		if (S->jMin != S->jMax) for (int j = S->jMin; j <= S->jMax; j++) fprintf(fp, "%f\t%e\n", W->jtoy(W, j), S->F[S->iMin][j][S->kMin]);
-		if (S->kMin != S->kMax) for (int k = S->kMin; k <= S->kMax; k++) fprintf(fp, "%f\t%e\n", W->ktoz(W, k), S->F[S->iMin][S->jMin][k]);
+	if (d == 1) {
+		const axis X = axisOf(W, axes[0]);
+		const int first[3] = {S->iMin, S->jMin, S->kMin}, last[3] = {S->iMax, S->jMax, S->kMax};
+		for (int u = first[axes[0]]; u <= last[axes[0]]; u++) {
+			int c[3] = {S->iMin, S->jMin, S->kMin};
+			c[axes[0]] = u;
+			fprintf(fp, "%f\t%e\n", (*X.coord)(W, u), S->F[c[0]][c[1]][c[2]]);
+		}
 	}
-	else {
+	else if (d == 2) {
 		if (S->interpolate) S->F = S->interpolate(W, S, S->F);
-		float *v = &S->F[S->iMIN][S->jMIN][S->kMIN];
-		for (int j = dim[1]-1; j >= 0; j--) {
-			fprintf(fp, "%e", v[j]);
-			for (int i = 1; i < dim[0]; i++) fprintf(fp, "\t%e", v[(long)dim[1]*i+j]);
-			fprintf(fp, "\n");
+		const float *v = &S->F[S->iMIN][S->jMIN][S->kMIN];
+		for (int r = n[1] - 1; r >= 0; r--) {
+			for (int u = 0; u < n[0]; u++) fprintf(fp, u ? "\t%e" : "%e", v[(long)n[1] * u + r]);
+			fputc('\n', fp);
 		}
 	}
 	fclose(fp);
 }
 
-/* Minimal PNG (8-bit RGB, one IDAT, filter 0) written with zlib. */
+/* PNG (8-bit RGB, one IDAT, filter 0) over zlib. */
 static void pngChunk(FILE *fp, const char *type, const unsigned char *data, unsigned len)
 {
-	unsigned char b[4] = { len >> 24, len >> 16, len >> 8, len };
+	const unsigned char b[4] = { len >> 24, len >> 16, len >> 8, len };
 	fwrite(b, 1, 4, fp);
 	fwrite(type, 1, 4, fp);
 	if (len) fwrite(data, 1, len, fp);
 	uLong crc = crc32(0L, (const Bytef *)type, 4);
 	if (len) crc = crc32(crc, data, len);
-	unsigned char c[4] = { crc >> 24, crc >> 16, crc >> 8, crc };
+	const unsigned char c[4] = { crc >> 24, crc >> 16, crc >> 8, crc };
 	fwrite(c, 1, 4, fp);
 }
 
@@ -162,14 +203,11 @@ static void writePng(const char *name, int width, int height, const unsigned cha
 	if (!fp) return;
 	static const unsigned char sig[8] = { 137, 80, 78, 71, 13, 10, 26, 10 };
 	fwrite(sig, 1, 8, fp);
-	unsigned char ihdr[13] = { width >> 24, width >> 16, width >> 8, width, height >> 24, height >> 16, height >> 8, height, 8, 2, 0, 0, 0 };
+	const unsigned char ihdr[13] = { width >> 24, width >> 16, width >> 8, width, height >> 24, height >> 16, height >> 8, height, 8, 2, 0, 0, 0 };
 	pngChunk(fp, "IHDR", ihdr, 13);
-	uLong rawLen = (uLong)height * (3*(uLong)width + 1);
+	const uLong stride = 3*(uLong)width + 1, rawLen = (uLong)height * stride;
 	unsigned char *raw = malloc(rawLen);
-	for (long r = 0; r < height; r++) {
-		raw[r*(3*(long)width+1)] = 0;
-		memcpy(raw + r*(3*(long)width+1) + 1, rgb + r*3*(long)width, 3*(long)width);
-	}
+	for (long r = 0; r < height; r++) raw[r*stride] = 0, memcpy(raw + r*stride + 1, rgb + r*3*(long)width, 3*(long)width);
 	uLongf zLen = compressBound(rawLen);
 	unsigned char *z = malloc(zLen);
 	compress2(z, &zLen, raw, rawLen, Z_DEFAULT_COMPRESSION);
@@ -179,238 +217,239 @@ static void writePng(const char *name, int width, int height, const unsigned cha
 	free(raw), free(z);
 }
 
-/* Colour-mapped image of a 1-D (64 rows tall) or 2-D slice; fields whose name starts with
- * E/H/J/U/S/i are darkened by the contour overlay; "Log" maps span 3 decades
- * (src/slices.c:155-211). */
+/* Colour of value v on colour map cmap (cmap[0] colours, linear between neighbours) over
+ * [lo, hi], dimmed by the factor shade (src/slices.c:185-202). */
+static void colourOf(const unsigned long int *cmap, float lo, float hi, float v, float shade, unsigned char *rgb)
+{
+	const int colours = labs((long)cmap[0]);
+	const float f = (hi-lo) ? MAX(0, MIN(colours-1, (colours-1)*(v-lo)/(hi-lo))) : 0;
+	int n = f;
+	if (n > colours-2) n = colours-2;
+	for (int c = 0; c < 3; c++) {
+		const int sh = 16 - 8*c;
+		rgb[c] = shade * ((1-f+n) * (cmap[n+1]>>sh & 0xFF) + (f-n) * (cmap[n+2]>>sh & 0xFF));
+	}
+}
+
+/* A plane as an image (rows of the second axis from its last node down), a line as a band
+ * 64 rows tall; a symmetric colour map spans [-Max, Max], "Log" fields three decades below
+ * Max; fields of E, H, J, U, S and i names are dimmed by the cut's contour. */
 void (png)(world W, slice S, float Max, char *F, char *filename, unsigned long int *cmap)
 {
-	(void)W;
-	int d = 0, dim[3] = {1,1,1};
+	int axes[3], n[3] = {1, 1, 1};
+	const int d = cutAxes(S, axes, n);
+	if (d == 3) printf("Can't write 3D slice to png file.\n"), exit(0);
+	if (d < 2) n[1] = 1;
+	const int rows = d == 1 ? 64 : 1;
+	if (S->interpolate) S->F = S->interpolate(W, S, S->F);
+	float lo = 0;
+	if (cmap[0] > LONG_MAX) lo = -Max;
+	if (F[0]=='L' && F[1]=='o' && F[2]=='g' && F[3]!='R' && F[4]!='I') lo = Max-3;
+	const int dim = F[0]=='E' || F[0]=='H' || F[0]=='J' || F[0]=='U' || F[0]=='S' || F[0]=='i';
+	const float *v = &S->F[S->iMIN][S->jMIN][S->kMIN], *c = S->C ? &S->C[S->iMIN][S->jMIN][S->kMIN] : 0;
+	unsigned char *img = malloc((size_t)3*n[0]*n[1]*rows);
+	long out = 0;
+	for (int r = n[1]-1; r >= 0; r--) {
+		unsigned char *row = img + out;
+		for (int u = 0; u < n[0]; u++) {
+			const long at = (long)n[1]*u + r;
+			colourOf(cmap, lo, Max, v[at], dim && c ? 1-c[at] : 1, row + 3*u);
+		}
+		out += 3*n[0];
+		for (int t = 1; t < rows; t++, out += 3*n[0]) memcpy(img + out, row, 3*n[0]);
+	}
 	char name[1100];
 	snprintf(name, sizeof name, "%s.png", filename);
-	float Min = 0;
-	if (cmap[0] > LONG_MAX) Min = -Max;
-	if (F[0]=='L' && F[1]=='o' && F[2]=='g' && F[3]!='R' && F[4]!='I') Min = Max-3;
-	if (S->iMin != S->iMax) dim[d++] = S->iNUM;
-	if (S->jMin != S->jMax) dim[d++] = S->jNUM;
-	if (S->kMin != S->kMax) dim[d++] = S->kNUM;
-	if (d==1) dim[2] = 64;
-	if (d==3) printf("Can't write 3D slice to png file.\n"), exit(0);
-	if (S->interpolate) S->F = S->interpolate(W, S, S->F);
-
-	int colours = abs((long)cmap[0]);
-	unsigned char *img = malloc((size_t)3*dim[0]*dim[1]*dim[2]), *row = malloc((size_t)3*dim[0]);
-	float *v = &S->F[S->iMIN][S->jMIN][S->kMIN], *c = S->C ? &S->C[S->iMIN][S->jMIN][S->kMIN] : 0;
-	int shade = F[0]=='E' || F[0]=='H' || F[0]=='J' || F[0]=='U' || F[0]=='S' || F[0]=='i';
-	long out = 0;
-	for (int j = dim[1]-1; j >= 0; j--) {
-		for (int i = 0; i < dim[0]; i++) {
-			long at = (long)dim[1]*i+j;
-			float C = !c ? 1 : 1-c[at];
-			float f = (Max-Min) ? MAX(0, MIN(colours-1, (colours-1)*(v[at]-Min)/(Max-Min))) : 0;
-			int n = f;
-			if (!shade) C = 1;
-			if (n > colours-2) n = colours-2;
-			row[3*i]   = C * ((1-f+n) * (cmap[n+1]>>0x10&0xFF) + (f-n) * (cmap[n+2]>>0x10&0xFF));
-			row[3*i+1] = C * ((1-f+n) * (cmap[n+1]>>0x08&0xFF) + (f-n) * (cmap[n+2]>>0x08&0xFF));
-			row[3*i+2] = C * ((1-f+n) * (cmap[n+1]>>0x00&0xFF) + (f-n) * (cmap[n+2]>>0x00&0xFF));
-		}
-		for (int r = 0; r < dim[2]; r++, out += 3*dim[0]) memcpy(img + out, row, 3*dim[0]);
-	}
-	writePng(name, dim[0], dim[1]*dim[2], img);
-	free(img), free(row);
+	writePng(name, n[0], n[1]*rows, img);
+	free(img);
 }
 
 
-static squeue findSliceQueue(world W, char *name, method M, squeue *head)
+/* ---- slice observers ---- */
+
+/* A record per (world, method, file name), counting its calls. */
+typedef struct record {
+	world W;
+	method M;
+	char *name;
+	long calls;
+	float runMax, shownMax;     /* sliceSnap: running maximum and the one in use */
+	float ***A, ***B;           /* sliceTimeAvg: sum; sliceFreqDom: cosine and sine sums */
+	struct record *next;
+} record;
+
+/* The record of (W, M, file) with the first '%' of the file name replaced by
+ * "<field>-<slice>", counted; the output directories are made on first use. */
+static record *recordOf(record **list, world W, method M, char *file, const char *f, const char *s)
 {
-	if (!*head) {
-		MALLOC(*head, 0, 1);
+	char *p = strchr(file, '%');
+	if (p) {
+		char rest[1024];
+		snprintf(rest, sizeof rest, "%s", p + 1);
+		snprintf(p, 1024 - (p - file), "%s-%s%s", f, s, rest);
 	}
-	squeue Q = *head;
-	while (Q->Q && (W != Q->W || M != Q->M || strcmp(name, Q->ID))) Q = Q->Q;
-	int fresh = !Q->Q;
-	if (fresh) {
-		Q->W = W, Q->M = M;
-		MALLOC(Q->ID, 0, 1+strlen(name));
-		strcpy(Q->ID, name);
-		MALLOC(Q->Q, 0, 1);
+	record *q = *list;
+	while (q && (q->W != W || q->M != M || strcmp(q->name, file))) q = q->next;
+	if (!q) {
+		q = calloc(1, sizeof *q);
+		q->W = W, q->M = M, q->name = strdup(file);
+		q->next = *list, *list = q;
+		if (file[0]=='/') mkdir(W->ID, 0755);
+		char full[1100];
+		snprintf(full, sizeof full, "%s%s", W->ID, file);
+		if (full[0] && full[strlen(full)-1]=='/') mkdir(full, 0755);
 	}
-	char full[1024];
-	snprintf(full, sizeof full, "%s%s", W->ID, Q->ID);
-	strcpy(name, full);
-	if (fresh) {
-		if (Q->ID[0]=='/') mkdir(W->ID, 0755);
-		if (name[0] && name[strlen(name)-1]=='/') mkdir(name, 0755);
-	}
-	Q->N++;
-	return Q;
+	char full[1100];
+	snprintf(full, sizeof full, "%s%s", W->ID, q->name);
+	snprintf(file, 1024, "%s", full);
+	q->calls++;
+	return q;
 }
 
-/* Replace the first '%' of name with "<field>-<slice>". */
-static void tagName(char *name, const char *f, const char *s)
-{
-	char *p = strchr(name, '%'), rest[1024];
-	if (!p) return;
-	strcpy(rest, p+1);
-	sprintf(p, "%s-%s%s", f, s, rest);
-}
+/* The method and scale arguments of the slice observers: method, then for png the colour
+ * map and the scale, then the file name format and its arguments. */
+#define METHOD_ARGS(ap, M, cmap, Max, file) do { \
+	M = va_arg(ap, method); \
+	cmap = M == png ? va_arg(ap, unsigned long int *) : 0; \
+	Max = M == png ? va_arg(ap, double) : 1; \
+	const char *fmt_ = va_arg(ap, char*); \
+	vsnprintf(file, 1024, fmt_, ap); } while (0)
 
-static int staticField(field F)
-{
-	return F==RI || F==LogRI || F==ContourRI || F==Contour || F==raweEffx || F==raweEffy || F==raweEffz || F==eEff;
-}
-
-/* Snapshot of F on S every T/N steps (N per period; 0 writes once); Max < 0 rescales to the
- * running maximum over -Max periods (src/slices.c:242-279). */
+/* Snapshots of F on S, N per period (0: one per call); Max 0 scales each frame to its own
+ * maximum, Max < 0 to the largest magnitude of the previous -Max periods, whose frames are
+ * not written (src/slices.c:242-279). */
 void (sliceSnap)(world W, field F, slice S, char *f, char *s, ...)
 {
-	static squeue head;
-	char filename[1024], num[32];
+	static record *list;
+	char file[1024];
+	method M;
+	unsigned long int *cmap;
+	float Max;
 	va_list ap;
 	va_start(ap, s);
-	int N = staticField(F) ? 0 : va_arg(ap, int);
-	method M = va_arg(ap, method);
-	unsigned long int *cmap = (M == png) ? va_arg(ap, unsigned long int *) : 0;
-	float Max = (M == png) ? va_arg(ap, double) : 1;
-	int step = (!N || N>W->T) ? 1 : W->T/N;
-	int skip = Max<0 ? ((int)(-Max*W->T/step))*step : 0;
-	char *format = va_arg(ap, char*);
-	vsnprintf(filename, sizeof filename, format, ap);
+	const int N = staticField(F) ? 0 : va_arg(ap, int);
+	METHOD_ARGS(ap, M, cmap, Max, file);
 	va_end(ap);
-	tagName(filename, f, s);
-	squeue Q = findSliceQueue(W, filename, M, &head);
-	if (N) snprintf(num, sizeof num, "%05ld", (Q->N-skip)/step), strcat(filename, num);
-
-	if (Q->N % step == 0) {
-		if (!staticField(F)) syncWorld(W);
-		float max = 0;
-		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-			if (max < fabsf(S->F[i][j][k] = F(W, i, j, k))) max = fabsf(S->F[i][j][k]);
-		if (!Max) Max = max;
-		else if (Max<0) {
-			Max = Q->Max;
-			if (Q->max < max) Q->max = max;
-			if (Q->N % skip == 0) Q->Max = Q->max, Q->max = 0;
-		}
-		if (Q->N > skip) M(W, S, Max?Max:1, f, filename, cmap);
+	const int step = (!N || N > W->T) ? 1 : W->T/N;
+	const int skip = Max < 0 ? ((int)(-Max*W->T/step))*step : 0;
+	record *q = recordOf(&list, W, M, file, f, s);
+	if (N) snprintf(file + strlen(file), 32, "%05ld", (q->calls-skip)/step);
+	if (q->calls % step) return;
+	const float max = sampleCut(W, F, S);
+	if (!Max) Max = max;
+	else if (Max < 0) {
+		Max = q->shownMax;
+		if (q->runMax < max) q->runMax = max;
+		if (skip && q->calls % skip == 0) q->shownMax = q->runMax, q->runMax = 0;
 	}
+	if (q->calls > skip) M(W, S, Max ? Max : 1, f, file, cmap);
 }
 
-/* Average of F over every N calls (src/slices.c:282-320). */
+/* The average of F on S over every N calls (src/slices.c:282-320). */
 void (sliceTimeAvg)(world W, field F, slice S, char *f, char *s, ...)
 {
-	static squeue head;
-	char filename[1024], num[32];
+	static record *list;
+	char file[1024];
+	method M;
+	unsigned long int *cmap;
+	float Max;
 	va_list ap;
 	va_start(ap, s);
-	int N = staticField(F) ? 0 : va_arg(ap, int);
-	method M = va_arg(ap, method);
-	unsigned long int *cmap = (M == png) ? va_arg(ap, unsigned long int *) : 0;
-	float Max = (M == png) ? va_arg(ap, double) : 1;
-	char *format = va_arg(ap, char*);
-	vsnprintf(filename, sizeof filename, format, ap);
+	const int N = staticField(F) ? 0 : va_arg(ap, int);
+	METHOD_ARGS(ap, M, cmap, Max, file);
 	va_end(ap);
-	tagName(filename, f, s);
-	squeue Q = findSliceQueue(W, filename, M, &head);
-	if (N) snprintf(num, sizeof num, "%05ld", Q->N / N), strcat(filename, num);
-	if (Q->N == 1) Q->F = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
-	syncWorld(W);
+	record *q = recordOf(&list, W, M, file, f, s);
+	if (N) snprintf(file + strlen(file), 32, "%05ld", q->calls / N);
+	if (!q->A) q->A = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+	refreshCut(W, F, S);
 	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-		Q->F[i][j][k] += F(W, i, j, k);
-	if (N && Q->N % N == 0) {
-		float max = 0;
-		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-			if (max < fabsf(S->F[i][j][k] = Q->F[i][j][k] / N)) max = fabsf(S->F[i][j][k]);
-		if (Max<=0) Max = max;
-		M(W, S, Max?Max:1, f, filename, cmap);
-		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-			Q->F[i][j][k] = 0;
+		q->A[i][j][k] += F(W, i, j, k);
+	if (!N || q->calls % N) return;
+	float max = 0;
+	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++) {
+		S->F[i][j][k] = q->A[i][j][k] / N;
+		if (max < fabsf(S->F[i][j][k])) max = fabsf(S->F[i][j][k]);
+		q->A[i][j][k] = 0;
 	}
+	if (Max <= 0) Max = max;
+	M(W, S, Max ? Max : 1, f, file, cmap);
 }
 
-/* Single-frequency phasor of F at wavelength L accumulated over N calls, written as 12
- * phase frames (src/slices.c:323-379). */
+/* The phasor of F on S at wavelength L accumulated over every N calls, written as 12 frames
+ * of one period (src/slices.c:323-379); component, current and source/scattered field
+ * accessors only. */
 void (sliceFreqDom)(world W, field F, slice S, char *f, char *s, int N, float L, ...)
 {
 	field *allowed[30] = {Ex, Ey, Ez, Hx, Hy, Hz, iEx, iEy, iEz, iHx, iHy, iHz, rawEx, rawEy, rawEz, rawHx, rawHy, rawHz,
 	                      Jx, Jy, Jz, rawJx, rawJy, rawJz, ScEx, ScEy, ScEz, SoEx, SoEy, SoEz};
-	int n = 0;
-	while (n < 30 && F != allowed[n]) n++;
-	if (n == 30) printf("\nThe %s field is not supported in sliceFreqDom().\n", f), exit(0);
-
-	static squeue head;
-	char filename[1024], frame[1100];
+	int known = 0;
+	for (int a = 0; a < 30; a++) known |= F == allowed[a];
+	if (!known) printf("\nThe %s field is not supported in sliceFreqDom().\n", f), exit(0);
+	static record *list;
+	char file[1024], frame[1100];
+	method M;
+	unsigned long int *cmap;
+	float Max;
 	va_list ap;
 	va_start(ap, L);
-	method M = va_arg(ap, method);
-	unsigned long int *cmap = (M == png) ? va_arg(ap, unsigned long int *) : 0;
-	float Max = (M == png) ? va_arg(ap, double) : 1;
-	char *format = va_arg(ap, char*);
-	vsnprintf(filename, sizeof filename, format, ap);
+	METHOD_ARGS(ap, M, cmap, Max, file);
 	va_end(ap);
-	tagName(filename, f, s);
-	squeue Q = findSliceQueue(W, filename, M, &head);
-	if (Q->N == 1) {
-		Q->F = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
-		Q->G = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+	record *q = recordOf(&list, W, M, file, f, s);
+	if (!q->A) {
+		q->A = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
+		q->B = makeField(S->iMin, S->iMax, S->jMin, S->jMax, S->kMin, S->kMax);
 	}
-	syncWorld(W);
-	float re = cosf(2 * PI * Q->N * W->dt / L);
-	float im = sinf(2 * PI * Q->N * W->dt / L);
+	if (F==ScEx || F==ScEy || F==ScEz || F==SoEx || F==SoEy || F==SoEz) syncWorld(W);
+	else refreshCut(W, F, S);
+	const float re = cosf(2 * PI * q->calls * W->dt / L), im = sinf(2 * PI * q->calls * W->dt / L);
 	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++) {
-		Q->F[i][j][k] += F(W, i, j, k) * re;
-		Q->G[i][j][k] += F(W, i, j, k) * im;
+		q->A[i][j][k] += F(W, i, j, k) * re;
+		q->B[i][j][k] += F(W, i, j, k) * im;
 	}
-	if (N && Q->N % N == 0) {
-		if (Max <= 0) {
-			float max = 0;
-			for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-				if (max < sq(Q->F[i][j][k])+sq(Q->G[i][j][k])) max = sq(Q->F[i][j][k])+sq(Q->G[i][j][k]);
-			if (Max < sqrtf(max) * 2 / Q->N) Max = sqrtf(max) * 2 / Q->N;
-		}
-		for (int p = 0; p < 12; p++) {
-			for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
-				S->F[i][j][k] = (Q->F[i][j][k] * cosf(p * PI / 6) + Q->G[i][j][k] * sinf(p * PI / 6)) * 2 / Q->N;
-			snprintf(frame, sizeof frame, "%s%02d", filename, p+1);
-			M(W, S, Max, f, frame, cmap);
-		}
+	if (!N || q->calls % N) return;
+	if (Max <= 0) {
+		float max = 0;
+		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
+			max = MAX(max, sq(q->A[i][j][k]) + sq(q->B[i][j][k]));
+		if (Max < sqrtf(max) * 2 / q->calls) Max = sqrtf(max) * 2 / q->calls;
+	}
+	for (int p = 0; p < 12; p++) {
+		for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
+			S->F[i][j][k] = (q->A[i][j][k] * cosf(p * PI / 6) + q->B[i][j][k] * sinf(p * PI / 6)) * 2 / q->calls;
+		snprintf(frame, sizeof frame, "%s%02d", file, p+1);
+		M(W, S, Max, f, frame, cmap);
 	}
 }
 
 float sliceMax(world W, field F, slice S)
 {
-	if (!staticField(F)) syncWorld(W);
+	refreshCut(W, F, S);
 	float Max = 0;
-	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++) {
-		float f = fabsf(F(W, i, j, k));
-		if (Max < f) Max = f;
-	}
+	for (int i = S->iMin; i <= S->iMax; i++) for (int j = S->jMin; j <= S->jMax; j++) for (int k = S->kMin; k <= S->kMax; k++)
+		Max = MAX(Max, fabsf(F(W, i, j, k)));
 	return Max;
 }
 
-/* Trapezoid integral of F over the slice, per unit length along degenerate axes
+/* Trapezoid integral of F over the cut, in double, per unit length along its flat axes
  * (src/slices.c:382-426). */
 float sliceSum(world W, field F, slice S)
 {
-	if (!staticField(F)) syncWorld(W);
+	refreshCut(W, F, S);
+	const int first[3] = {S->iMin, S->jMin, S->kMin}, last[3] = {S->iMax, S->jMax, S->kMax};
+	float *width[3] = {W->CE->dx, W->CE->dy, W->CE->dz};
 	double Sum = 0;
 	for (int i = S->iMin; i <= S->iMax; i++) {
-		double wi = (S->iMin == S->iMax) ? W->CE->dx[i] : (i == S->iMin || i == S->iMax) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
 		double sy = 0;
 		for (int j = S->jMin; j <= S->jMax; j++) {
-			double wj = (S->jMin == S->jMax) ? W->CE->dy[j] : (j == S->jMin || j == S->jMax) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
 			double sz = 0;
 			for (int k = S->kMin; k <= S->kMax; k++) {
-				double wk = (S->kMin == S->kMax) ? W->CE->dz[k] : (k == S->kMin || k == S->kMax) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
+				const double wk = (first[2] != last[2] && (k == first[2] || k == last[2])) ? 0.5 * width[2][k] : width[2][k];
 				sz += wk * F(W, i, j, k);
 			}
-			sy += wj * sz;
+			sy += ((first[1] != last[1] && (j == first[1] || j == last[1])) ? 0.5 * width[1][j] : width[1][j]) * sz;
 		}
-		Sum += wi * sy;
+		Sum += ((first[0] != last[0] && (i == first[0] || i == last[0])) ? 0.5 * width[0][i] : width[0][i]) * sy;
 	}
-	if (S->iMin==S->iMax) Sum /= W->CE->dx[S->iMin];
-	if (S->jMin==S->jMax) Sum /= W->CE->dy[S->jMin];
-	if (S->kMin==S->kMax) Sum /= W->CE->dz[S->kMin];
+	for (int a = 0; a < 3; a++) if (first[a] == last[a]) Sum /= width[a][first[a]];
 	return Sum;
 }

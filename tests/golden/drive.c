@@ -751,6 +751,34 @@ void alis_drive_release(void)
 	releaseRun();
 }
 
+/* Run a scenario and record slices of its first world into dir at its last steps: snapshots
+ * of Ez and Hy on the plane z = 0 (text) and of Ex on the line through the centre along x
+ * (text), an Ez image (PNG), the average of Hy over the last 10 steps and the 500 nm phasor of
+ * Ez over the last 20 steps (text, 12 frames); returns 0. */
+int alis_drive_slices(const char *name, int steps, const char *dir)
+{
+	OUT = fopen("/dev/null", "w");
+	int rc = run(name, steps > 20 ? steps - 20 : 0);
+	fclose(OUT);
+	OUT = 0;
+	if (rc || !LAST) return rc ? rc : 3;
+	world W = LAST;
+	slice XY = createSliceXY(W, 10), X = createSliceX(W, 0, 0);
+	for (int s = 0; s < 20; s++) {
+		updateE(W);
+		updateH(W);
+		sliceTimeAvg(W, Hy, XY, 10, txt, "%s/avg-%%", dir);
+		sliceFreqDom(W, Ez, XY, 20, 500, txt, "%s/phasor-%%", dir);
+	}
+	sliceSnap(W, Ez, XY, 0, txt, "%s/snap-%%", dir);
+	sliceSnap(W, Hy, XY, 0, txt, "%s/snap-%%", dir);
+	sliceSnap(W, Ex, X, 0, txt, "%s/line-%%", dir);
+	sliceSnap(W, Ez, XY, 0, png(dkbr, 0), "%s/image-%%", dir);
+	deleteSlice(XY), deleteSlice(X);
+	releaseRun();
+	return 0;
+}
+
 /* The host library's double-precision far field (farFieldD, alis_amd/host/farfield.c). */
 double complex farFieldD(phaser P, tfield pol(phaser,float,float), double theta, double phi);
 
