@@ -47,6 +47,9 @@ void alisFatal(const char *what, int rc);
 phaser phaserBase(phaser P);     /* first element of the array holding P, or 0 */
 
 /* observers.c */
+/* format a name from a copy of *ap and advance *ap past the arguments its format names */
+void alisFormatName(char *out, size_t size, const char *format, va_list *ap);
+void alisForgetTexts(world W);    /* close the text outputs of a world */
 void alisForgetSpectra(world W);  /* drop the writeSpectrum records of a world */
 
 /* describe.c */

@@ -1,48 +1,12 @@
 /*
- * Observers and text output (src/output.c:13-445): probes, world maxima/integrals, per-file
- * text queues, probe spectra, Poynting fluxes through planes and absorbed power.  They read
- * the host arrays, which are refreshed from the device first (a small box for get()).
+ * Observers (src/output.c:13-101, 177-445): probes, world maxima and integrals, probe
+ * spectra, Poynting fluxes through planes and absorbed power.  Probes read host mirrors
+ * refreshed over a small box; fluxes and absorption are summed on the device.
  */
 #include <string.h>
 #include <sys/stat.h>
-#include <sys/time.h>
 #include "internal.h"
 #include "phaserview.h"
-
-/* Output file queue keyed by (world, name) (src/output.c:104-128). */
-typedef struct queue *queue;
-struct queue {
-	world W;
-	char *ID;
-	FILE *File;
-	unsigned long int N;
-	unsigned long long int timeStamp;
-	queue Q;
-};
-
-/* Find or create the queue of (W, name); on return `name` holds W->ID + name.  Creates the
- * output directory when the name starts with '/' or ends with '/'. */
-static queue findQueue(world W, char *name, queue *head)
-{
-	if (!*head) {
-		MALLOC(*head, 0, 1);
-	}
-	queue Q = *head;
-	while (Q->Q && (W != Q->W || strcmp(name, Q->ID))) Q = Q->Q;
-	if (!Q->Q) {
-		Q->W = W;
-		MALLOC(Q->ID, 0, 1+strlen(name));
-		strcpy(Q->ID, name);
-		MALLOC(Q->Q, 0, 1);
-		if (Q->ID[0]=='/') mkdir(W->ID, 0755);
-	}
-	char full[1024];
-	snprintf(full, sizeof full, "%s%s", W->ID, Q->ID);
-	strcpy(name, full);
-	if (Q->N == 0 && name[0] && name[strlen(name)-1]=='/') mkdir(name, 0755);
-	Q->N++;
-	return Q;
-}
 
 /* Value of field F at the grid node nearest to (x, y, z) (src/output.c:50-53). */
 float get(world W, field F, float x, float y, float z)
@@ -103,55 +67,6 @@ float worldSum(world W, field F)
 	return Sum;
 }
 
-
-/* Append text to file W->ID + name.txt; the file is flushed every 5 minutes and at exit
- * (src/output.c:131-154).  The first format names the file, the second the text. */
-void (writeTxt)(world W, char *format, ...)
-{
-	static queue head;
-	char str[4096], filename[1024];
-	va_list ap;
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	format = va_arg(ap, char*);
-	vsnprintf(str, sizeof str, format, ap);
-	va_end(ap);
-	queue Q = findQueue(W, filename, &head);
-	struct timeval now;
-	gettimeofday(&now, 0);
-	unsigned long long int time = now.tv_sec * 1000000LL + now.tv_usec;
-	if (!Q->timeStamp) {
-		strcat(filename, ".txt");
-		Q->File = fopen(filename, "w");
-		Q->timeStamp = time;
-	}
-	if (Q->File) fputs(str, Q->File);
-	if (time - Q->timeStamp >= 300*1000000LL) {
-		if (Q->File) fflush(Q->File);
-		Q->timeStamp = time;
-	}
-}
-
-/* One tab-separated row of doubles terminated by the INT2NAN(1) sentinel (src/output.c:157-174). */
-void (writeRow)(world W, char *format, ...)
-{
-	char str[4096], filename[1024];
-	va_list ap;
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	double f = va_arg(ap, double);
-	if (!NAN2INT(f)) {
-		int n = snprintf(str, sizeof str, "%g", f);
-		f = va_arg(ap, double);
-		while (!NAN2INT(f)) {
-			n += snprintf(str+n, sizeof str - n, "\t%g", f);
-			f = va_arg(ap, double);
-		}
-		snprintf(str+n, sizeof str - n, "\n");
-		(writeTxt)(W, filename, "%s", str, "");
-	}
-	va_end(ap);
-}
 
 /*
  * Probe spectra (writeSpectrum, src/output.c:177-230): the running DFT of the values a
@@ -304,7 +219,7 @@ void (writeSpectrum)(world W, int N, float LMin, float LMax, char *format, ...)
 	char name[1024];
 	va_list ap;
 	va_start(ap, format);
-	vsnprintf(name, sizeof name, format, ap);
+	alisFormatName(name, sizeof name, format, &ap);
 	int M = 0;
 	double v[256];
 	for (double d = va_arg(ap, double); !NAN2INT(d); d = va_arg(ap, double)) {

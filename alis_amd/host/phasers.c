@@ -435,52 +435,3 @@ float phaserPoynting(phaser P) { return fabsf(phaserPoyntingOut(P)); }
 float phaserPoyntingIn(phaser P) { return -phaserPoyntingOut(P); }
 
 
-/* Text spectra over the frequencies of the first array: wavelength, then one column per
- * phaser array (src/phaser.c:248-295). */
-void (writePoyntingSpectrum)(world W, char *format, ...)
-{
-	char str[4096], filename[1024];
-	va_list ap;
-	int M = 0;
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	for (phaser p = va_arg(ap, phaser); p; M++) p = va_arg(ap, phaser);
-	va_end(ap);
-	phaser *P;
-	MALLOC(P, 0, M ? M : 1);
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	for (int m = 0; m < M; m++) P[m] = va_arg(ap, phaser);
-	va_end(ap);
-	for (int n = 0; M && P[0][n].f; n++) {
-		int s = sprintf(str, "%g", 1/P[0][n].f);
-		for (int m = 0; m < M; m++) s += sprintf(str+s, "\t%g", phaserPoynting(P[m]+n));
-		sprintf(str+s, "\n");
-		(writeTxt)(W, filename, "%s", str, "");
-	}
-	free(P);
-}
-
-void (writeFarFieldSpectrum)(world W, tfield pol(phaser,float,float), float theta, float phi, char *format, ...)
-{
-	char str[4096], filename[1024];
-	va_list ap;
-	int M = 0;
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	for (phaser p = va_arg(ap, phaser); p; M++) p = va_arg(ap, phaser);
-	va_end(ap);
-	phaser *P;
-	MALLOC(P, 0, M ? M : 1);
-	va_start(ap, format);
-	vsnprintf(filename, sizeof filename, format, ap);
-	for (int m = 0; m < M; m++) P[m] = va_arg(ap, phaser);
-	va_end(ap);
-	for (int n = 0; M && P[0][n].f; n++) {
-		int s = sprintf(str, "%g", 1/P[0][n].f);
-		for (int m = 0; m < M; m++) s += sprintf(str+s, "\t%g", farFieldI(P[m]+n, pol, theta, phi));
-		sprintf(str+s, "\n");
-		(writeTxt)(W, filename, "%s", str, "");
-	}
-	free(P);
-}

@@ -95,6 +95,7 @@ void alisForgetWorld(world W)
 {
 	forgetProbes(W);
 	alisForgetSpectra(W);
+	alisForgetTexts(W);
 	if (W->dev) {
 		dropBatch();
 		alis_dev_release(W->dev);

@@ -5,7 +5,7 @@
  * it, and prints "key value" lines: FNV-1a hashes and sums of squares of setup arrays and of
  * the final state, probe traces, phaser accumulators, fluxes and far-field samples.
  *
- * usage (program): drive SCENARIO [STEPS [OUTPATH]]
+ * usage (program): drive SCENARIO [STEPS [OUTPATH]]   or   drive --text SCENARIO STEPS DIR
  * usage (library): alis_drive(SCENARIO, STEPS, OUTPATH) writes the same lines to OUTPATH.
  */
 #include "alis.h"
@@ -615,6 +615,41 @@ int alis_drive(const char *name, int steps, const char *path)
 	return rc;
 }
 
+/* Text outputs of a scenario with a phaser array after `steps` steps, written into dir:
+ * writeRow and writeTxt lines, a row call with no numbers (which writes no file), the
+ * Poynting and far-field spectra of the array (writePoyntingSpectrum with the array twice,
+ * writeFarFieldSpectrum at theta 30, phi 45) and the far-field cuts of its first frequency
+ * (farFieldTheta at phi 0, farFieldPhi at theta 60).  Over this library it also writes a
+ * text with a '%' and one longer than 1 KiB, which the reference does not write as given. */
+int alis_drive_text(const char *name, int steps, const char *dir)
+{
+	OUT = fopen("/dev/null", "w");
+	int rc = run(name, steps);
+	fclose(OUT);
+	OUT = 0;
+	if (rc || !LAST || !LASTP) return rc ? rc : 3;
+	world W = LAST;
+	phaser P = LASTP;
+	writeRow(W, "%s/rows", dir, 1.5, 2.0, -3.25e-7);
+	writeRow(W, "%s/rows", dir, (double)steps, 1e30);
+	writeRow(W, "%s/empty", dir);
+	writeTxt(W, "%s/notes-%d", dir, 3, "%s %d %g\n", "steps", steps, 0.125);
+	writePoyntingSpectrum(W, "%s/poynting", dir, P, P);
+	writeFarFieldSpectrum(W, Ppol, 30, 45, "%s/farfield", dir, P);
+	farFieldTheta(W, P, 0, "%s/theta", dir);
+	farFieldPhi(W, P, 60, "%s/phi", dir);
+#ifndef ALIS_REF
+	char longText[3001];
+	memset(longText, 'x', 3000);
+	longText[3000] = 0;
+	writeTxt(W, "%s/percent", dir, "%d%%\n", 100);
+	writeTxt(W, "%s/long", dir, "%s\n", longText);
+#endif
+	fflush(0);
+	releaseRun();
+	return 0;
+}
+
 #ifndef ALIS_REF
 #include "alis_dev.h"
 /* Raw sums of a writeSpectrum record (alis_amd/host/observers.c). */
@@ -808,7 +843,8 @@ int alis_drive_farfield(const char *name, int steps, int n, const double *theta,
 #ifdef DRIVE_MAIN
 int main(int argc, char **argv)
 {
-	if (argc < 2) return fprintf(stderr, "usage: %s SCENARIO [STEPS [OUTPATH]]\n", argv[0]), 2;
+	if (argc < 2) return fprintf(stderr, "usage: %s SCENARIO [STEPS [OUTPATH]] | --text SCENARIO STEPS DIR\n", argv[0]), 2;
+	if (argc > 4 && !strcmp(argv[1], "--text")) return alis_drive_text(argv[2], atoi(argv[3]), argv[4]);
 	if (argc > 3) return alis_drive(argv[1], atoi(argv[2]), argv[3]);
 	OUT = stdout;
 	setvbuf(stdout, 0, _IOLBF, 0);

@@ -3,8 +3,10 @@
 Runs only where the reference sources are present: oracle/Makefile compiles them (with the
 reference's own flags) into oracle/_ref/libref.a and links tests/golden/drive.c against it
 (oracle/_ref/drive_ref).  Each scenario's output lines are written to
-tests/golden/<scenario>.txt, and manifest.json records the step counts and a SHA-256 of the
-reference sources the fixtures came from.
+tests/golden/<scenario>.txt, the text files of the scenarios in TEXT (alis_drive_text: rows,
+Poynting and far-field spectra, far-field cuts) to tests/golden/text/<scenario>/, and
+manifest.json records the step counts and a SHA-256 of the reference sources the fixtures
+came from.
 
     python tests/golden/make_golden.py [SCENARIO ...]     (default: all)
 """
@@ -35,6 +37,9 @@ SCENARIOS = {
     "kpoints": 200,
 }
 
+# scenario -> steps for the text-output fixtures (tests/test_text.py)
+TEXT = {"symband": 200, "nanoblock": 200}
+
 
 def main():
     if not (REF / "update.c").exists():
@@ -53,11 +58,20 @@ def main():
                            env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"}, cwd=d)
             shutil.copyfile(tmp, out)
         print(f"{name}: {sum(1 for _ in out.open())} lines")
+    for name, steps in TEXT.items():
+        if only and name not in only:
+            continue
+        out = HERE / "text" / name
+        shutil.rmtree(out, ignore_errors=True)
+        out.mkdir(parents=True)
+        subprocess.run([str(drive), "--text", name, str(steps), str(out)], check=True, stdout=subprocess.DEVNULL,
+                       env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"}, cwd=out)
+        print(f"{name} text: {sorted(p.name for p in out.iterdir())}")
     h = hashlib.sha256()
     for f in sorted(REF.glob("*.[ch]")):
         h.update(f.name.encode())
         h.update(f.read_bytes())
-    manifest = {"scenarios": SCENARIOS, "reference_sources_sha256": h.hexdigest(),
+    manifest = {"scenarios": SCENARIOS, "text": TEXT, "reference_sources_sha256": h.hexdigest(),
                 "generator": "oracle/_ref/drive_ref (tests/golden/drive.c over the reference library)"}
     (HERE / "manifest.json").write_text(json.dumps(manifest, indent=1) + "\n")
 
