@@ -423,6 +423,7 @@ template<class T> struct PhsArgs {
 	PhsFaceGeom face[6];
 	long ncell;
 	void *acc;                  // [freq][comp 4][cell] complex (TA pairs)
+	void *cmp;                  // FP64 accumulators: their compensations, laid out as acc
 	int q0, nq;                 // the launch's frequencies (of the accumulators)
 };
 
@@ -480,22 +481,36 @@ __global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict_
 	slot[t] = e1, slot[a.ncell + t] = e2, slot[2 * a.ncell + t] = h1, slot[3 * a.ncell + t] = h2;
 }
 
+// s += y with Neumaier's compensation (Neumaier 1974, the improved Kahan-Babuska sum): c
+// collects the low-order part each addition drops; the sum's value is s + c.
+__device__ __forceinline__ void neumaierAdd(double &s, double &c, double y)
+{
+	const double t = s + y;
+	c += fabs(s) >= fabs(y) ? (s - t) + y : (y - t) + s;
+	s = t;
+}
+
 // ring: ns samples [s][4][cell]; ex: per sample `stride` doubles, frequency q of the launch at
 // ex[s * stride + 4 q] = (eE re, eE im, eH re, eH im).  Face cell t, frequencies qa ...
+// Float accumulators add plainly (the reference's sums); FP64 ones with compensation.
 template<class T, class TA>
 __device__ __forceinline__ void phaserFlushCell(const PhsArgs<T> &a, long t, int qa, const T *__restrict__ ring, int ns,
                                                 const double *__restrict__ ex, int stride)
 {
 	typedef typename Pair<TA>::V V;
-	V *acc = (V *)a.acc;
+	constexpr bool compensated = sizeof(TA) == 8;
+	V *acc = (V *)a.acc, *cmp = (V *)a.cmp;
 	const TA half = TA(0.5);
 	const int nq = min(PHS_QPT, a.nq - qa);
-	V x[PHS_QPT][4];
+	V x[PHS_QPT][4], c[PHS_QPT][4];
 #pragma unroll
 	for (int u = 0; u < PHS_QPT; u++)
 		if (u < nq) {
-			const V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
-			for (int c = 0; c < 4; c++) x[u][c] = __builtin_nontemporal_load(p + c * a.ncell);
+			const long o = (long)(a.q0 + qa + u) * 4 * a.ncell + t;
+			for (int m = 0; m < 4; m++) {
+				x[u][m] = __builtin_nontemporal_load(acc + o + m * a.ncell);
+				if constexpr (compensated) c[u][m] = __builtin_nontemporal_load(cmp + o + m * a.ncell);
+			}
 		}
 	for (int s = 0; s < ns; s++) {
 		const T *r = ring + (long)s * 4 * a.ncell + t;
@@ -505,18 +520,38 @@ __device__ __forceinline__ void phaserFlushCell(const PhsArgs<T> &a, long t, int
 			if (u < nq) {
 				const double *f = ex + (long)s * stride + 4 * (qa + u);
 				const TA er = TA(f[0]), ei = TA(f[1]), hr = TA(f[2]), hi = TA(f[3]);
-				x[u][0][0] = x[u][0][0] + er * TA(e1), x[u][0][1] = x[u][0][1] + ei * TA(e1);
-				x[u][1][0] = x[u][1][0] + er * TA(e2), x[u][1][1] = x[u][1][1] + ei * TA(e2);
-				x[u][2][0] = x[u][2][0] + hr * TA(h1) * half, x[u][2][1] = x[u][2][1] + hi * TA(h1) * half;
-				x[u][3][0] = x[u][3][0] + hr * TA(h2) * half, x[u][3][1] = x[u][3][1] + hi * TA(h2) * half;
+				const TA y[4][2] = { { er * TA(e1), ei * TA(e1) }, { er * TA(e2), ei * TA(e2) },
+				                     { hr * TA(h1) * half, hi * TA(h1) * half }, { hr * TA(h2) * half, hi * TA(h2) * half } };
+				for (int m = 0; m < 4; m++)
+					for (int p = 0; p < 2; p++) {
+						if constexpr (compensated) {
+							double sm = x[u][m][p], cm = c[u][m][p];
+							neumaierAdd(sm, cm, y[m][p]);
+							x[u][m][p] = sm, c[u][m][p] = cm;
+						}
+						else x[u][m][p] = x[u][m][p] + y[m][p];
+					}
 			}
 	}
 #pragma unroll
 	for (int u = 0; u < PHS_QPT; u++)
 		if (u < nq) {
-			V *p = acc + (long)(a.q0 + qa + u) * 4 * a.ncell + t;
-			for (int c = 0; c < 4; c++) __builtin_nontemporal_store(x[u][c], p + c * a.ncell);
+			const long o = (long)(a.q0 + qa + u) * 4 * a.ncell + t;
+			for (int m = 0; m < 4; m++) {
+				__builtin_nontemporal_store(x[u][m], acc + o + m * a.ncell);
+				if constexpr (compensated) __builtin_nontemporal_store(c[u][m], cmp + o + m * a.ncell);
+			}
 		}
+}
+
+// Fold the compensations of FP64 accumulators into their sums before anything reads them:
+// acc += cmp and cmp = 0 where cmp is not zero (a zero leaves the sum's bits alone).
+__global__ __launch_bounds__(256) void k_phs_fold(double *__restrict__ acc, double *__restrict__ cmp, long n)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const double c = cmp[t];
+	if (c != 0) acc[t] += c, cmp[t] = 0;
 }
 
 template<class T, class TA>
@@ -889,6 +924,8 @@ struct DevPhaser {
 	PhsFaceGeom face[6]{};
 	long ncell = 0;
 	void *acc = nullptr;        // TA
+	void *cmp = nullptr;        // FP64 accumulators: compensations (same allocation, after acc)
+	bool dirty = false;         // cmp may hold nonzero terms (folded before reads)
 	float *stage = nullptr;     // device float buffer for host syncs
 	size_t bytes = 0;           // device bytes of acc and stage (and the ring)
 	// deferred updates: samples of the face fields not yet added to acc
@@ -1876,14 +1913,25 @@ int phaserFlush(alis_dev_world *D, DevPhaser &P)
 	k_phs_flush<T, TA><<<nb * ny, 256, 0, D->st>>>(a, (const T *)P.ring, P.pend, P.exDev, 4 * P.pnq, ny);
 	HIPCHECK(hipGetLastError());
 	P.pend = 0;
+	P.dirty = sizeof(TA) == 8;
 	return 0;
 }
 
+// Make P's accumulators readable: add the pending samples, then fold the compensations of
+// FP64 accumulators into their sums.
 int phaserFlushAny(alis_dev_world *D, DevPhaser &P)
 {
-	if (!P.pend) return 0;
-	if (D->fp64) return D->dft64 ? phaserFlush<double, double>(D, P) : phaserFlush<double, float>(D, P);
-	return D->dft64 ? phaserFlush<float, double>(D, P) : phaserFlush<float, float>(D, P);
+	int rc = 0;
+	if (P.pend) {
+		if (D->fp64) rc = D->dft64 ? phaserFlush<double, double>(D, P) : phaserFlush<double, float>(D, P);
+		else rc = D->dft64 ? phaserFlush<float, double>(D, P) : phaserFlush<float, float>(D, P);
+	}
+	if (rc || !P.dirty) return rc;
+	const long n = (long)P.nf * 4 * P.ncell * 2;
+	k_phs_fold<<<blocksFor(n), 256, 0, D->st>>>((double *)P.acc, (double *)P.cmp, n);
+	HIPCHECK(hipGetLastError());
+	P.dirty = false;
+	return 0;
 }
 
 // One sample of P (src/phaser.c:83-110): the face fields go to the ring and the factors to
@@ -2002,6 +2050,7 @@ int phaserBatchUpdate(alis_dev_world *L, alis_dev_world *const *D, const int *id
 		fblocks += (int)blocksFor(P.ncell);
 		ny = std::max(ny, (P.pnq + PHS_QPT - 1) / PHS_QPT);
 		P.pend = 0;
+		P.dirty = sizeof(TA) == 8;
 	}
 	memcpy(L->phfStage.pinned, fa.data(), fa.size() * sizeof(PhsFlush<T>));
 	memcpy((char *)L->phfStage.pinned + ab, fstart.data(), fstart.size() * sizeof(int));
@@ -2025,6 +2074,7 @@ PhsArgs<T> phaserArgs(alis_dev_world *D, DevPhaser &P, int qa, int qn)
 	for (int f = 0; f < P.nface; f++) a.face[f] = P.face[f];
 	a.ncell = P.ncell;
 	a.acc = P.acc;
+	a.cmp = P.cmp;
 	a.q0 = qa, a.nq = qn;
 	return a;
 }
@@ -2490,7 +2540,12 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 	Q.ncell = off;
 	const long n = (long)Q.nf * 4 * Q.ncell * 2;
 	const size_t accBytes = (size_t)std::max(n, 1L) * (D->dft64 ? 8 : 4);
-	HIPCHECK(hipMalloc(&Q.acc, accBytes));
+	HIPCHECK(hipMalloc(&Q.acc, D->dft64 ? 2 * accBytes : accBytes));
+	if (D->dft64) {
+		Q.cmp = (char *)Q.acc + accBytes;
+		k_fill<double><<<blocksFor(std::max(n, 1L)), 256, 0, D->st>>>((double *)Q.cmp, std::max(n, 1L), 0.0);
+		HIPCHECK(hipGetLastError());
+	}
 	HIPCHECK(hipMalloc((void **)&Q.stage, (size_t)std::max(n, 1L) * sizeof(float)));
 	const size_t ringBytes = (size_t)PHS_RING * 4 * std::max(Q.ncell, 1L) * D->esz;
 	const size_t exBytes = (size_t)PHS_RING * 4 * std::max(Q.nf, 1) * sizeof(double);
@@ -2498,7 +2553,7 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 	HIPCHECK(hipMalloc((void **)&Q.exDev, exBytes));
 	HIPCHECK(hipHostMalloc((void **)&Q.exPinned, exBytes, hipHostMallocDefault));
 	HIPCHECK(hipEventCreateWithFlags(&Q.exDone, hipEventDisableTiming));
-	Q.bytes = accBytes + (size_t)std::max(n, 1L) * sizeof(float) + ringBytes + exBytes;
+	Q.bytes = (D->dft64 ? 2 : 1) * accBytes + (size_t)std::max(n, 1L) * sizeof(float) + ringBytes + exBytes;
 	D->bytes += Q.bytes;
 	// start from the host accumulators (normally zero)
 	std::vector<float> host(std::max(n, 1L));

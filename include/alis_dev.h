@@ -139,7 +139,10 @@ int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double
  * factors e^{i 2 pi f t} at the E and H times as re/im pairs), copy accumulators to the
  * host cfields, read them as doubles, remove.  Read layout: [frequency][component E1 E2 H1
  * H2][cell][re,im] with the cells of the present faces concatenated in the order xMin, xMax,
- * yMin, yMax, zMin, zMax (x faces (j,k), y faces (i,k), z faces (i,j), last index fastest). */
+ * yMin, yMax, zMin, zMax (x faces (j,k), y faces (i,k), z faces (i,j), last index fastest).
+ * Float accumulators add each sample plainly, as the reference does; FP64 accumulators add
+ * with Neumaier compensation, and every entry that reads them (sync, read, flux sums, far
+ * field) first folds the compensations into the sums. */
 int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *id);
 int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH);
 /* One sample into n phasers at once (entry e: world D[e], phaser id[e], frequencies

@@ -18,6 +18,10 @@
  * properties in tests/test_oracle_physics.py (discrete dispersion of a periodic box mode,
  * energy conservation in a closed PEC cavity, convergence of the float path towards it).
  *
+ * FP64 phaser accumulators add with Neumaier compensation, folded into the sums before every
+ * read, as the engine does; tests/test_phaser_compensation.py pins it with a sample stream
+ * whose plain double sum loses every term but the first.
+ *
  * Every access goes through a bounds-checked index map that aborts on a miss.
  */
 #include <complex.h>
@@ -51,7 +55,7 @@ typedef struct {
 	int iMin, iMax, jMin, jMax, kMin, kMax;
 	int nface, kind[6];
 	long off[6], ncell;
-	double *accD;
+	double *accD, *cmpD;      /* FP64 accumulators and their Neumaier compensations */
 	float *accF;
 	float **cf;
 } phs;
@@ -248,7 +252,7 @@ void alis_dev_release(alis_dev_world *D)
 		for (int m = 0; m < 2; m++)
 			for (int c = 0; c < 3; c++)
 				for (int p = 0; p < ALIS_DEV_MAXPOLES; p++) freeArr(&D->J[n][m][c][p]), freeArr(&D->K[n][m][c][p]);
-	for (int q = 0; q < D->nph; q++) free(D->ph[q].accD), free(D->ph[q].accF), free(D->ph[q].cf);
+	for (int q = 0; q < D->nph; q++) free(D->ph[q].accD), free(D->ph[q].cmpD), free(D->ph[q].accF), free(D->ph[q].cf);
 	free(D->ph);
 	free(D->J), free(D->K), free(D->obj), free(D->src[0]), free(D->src[1]), free(D->tf), free(D->aux);
 	free(D);
@@ -383,7 +387,7 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 		Q->nface++;
 	}
 	const long n = (long)Q->nf * 4 * Q->ncell * 2;
-	if (Q->dft64) Q->accD = calloc(n ? n : 1, sizeof(double));
+	if (Q->dft64) Q->accD = calloc(n ? n : 1, sizeof(double)), Q->cmpD = calloc(n ? n : 1, sizeof(double));
 	else Q->accF = calloc(n ? n : 1, sizeof(float));
 	for (int q = 0; q < Q->nf; q++)
 		for (int f = 0; f < Q->nface; f++) {
@@ -405,6 +409,16 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 static phs *getPhaser(alis_dev_world *D, int id)
 {
 	return id >= 0 && id < D->nph && D->ph[id].live ? &D->ph[id] : 0;
+}
+
+/* Fold the compensations of FP64 accumulators into their sums (before every read): where a
+ * compensation is not zero, acc += cmp and cmp = 0. */
+static void settle(phs *Q)
+{
+	if (!Q->dft64) return;
+	const long n = (long)Q->nf * 4 * Q->ncell * 2;
+	for (long c = 0; c < n; c++)
+		if (Q->cmpD[c] != 0) Q->accD[c] += Q->cmpD[c], Q->cmpD[c] = 0;
 }
 
 int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH)
@@ -438,6 +452,7 @@ int alis_dev_phaser_sync(alis_dev_world *D, int id)
 {
 	phs *Q = getPhaser(D, id);
 	if (!Q) return fail("no such phaser");
+	settle(Q);
 	for (int q = 0; q < Q->nf; q++)
 		for (int f = 0; f < Q->nface; f++) {
 			const long cells = (f + 1 < Q->nface ? Q->off[f+1] : Q->ncell) - Q->off[f];
@@ -461,6 +476,7 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 	phs *Q = getPhaser(D, id);
 	const long need = alis_dev_phaser_size(D, id);
 	if (!Q || n < need) return fail("phaser read buffer too small");
+	settle(Q);
 	for (long c = 0; c < need; c++) out[c] = Q->dft64 ? Q->accD[c] : Q->accF[c];
 	return 0;
 }
@@ -480,6 +496,7 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
 		    || u[5] < 0 || u[5] > 3 || u[6] < 0 || u[6] + wl > nw || u[7] < 0 || u[7] + len > nout)
 			return fail("flux sums: term outside the phaser faces or the buffers");
 	}
+	settle(Q);
 	for (long c = 0; c < nout; c++) out[c] = 0;
 	for (int n = 0; n < nterm; n++) {
 		const int *u = terms + 8 * n;
@@ -513,8 +530,8 @@ int alis_dev_phaser_remove(alis_dev_world *D, int id)
 {
 	phs *Q = getPhaser(D, id);
 	if (!Q) return fail("no such phaser");
-	free(Q->accD), free(Q->accF), free(Q->cf);
-	Q->accD = 0, Q->accF = 0, Q->cf = 0, Q->live = 0;
+	free(Q->accD), free(Q->cmpD), free(Q->accF), free(Q->cf);
+	Q->accD = 0, Q->cmpD = 0, Q->accF = 0, Q->cf = 0, Q->live = 0;
 	return 0;
 }
 
@@ -581,6 +598,7 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 				for (int t = 0; t < 6; t++) if (cols[c].src[p][t] >= 6) return fail("far field: no such profile component");
 			}
 	}
+	settle(Q);
 	memset(S, 0, (size_t)n * 96 * sizeof(double));
 	/* T components paired with the face components (E1 E2 H1 H2) of x and y faces */
 	static const int tX[4] = {5, 4, 2, 1}, tY[4] = {3, 5, 0, 2};

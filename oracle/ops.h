@@ -321,12 +321,16 @@ static void SUF(phaserStep)(world *D, phs *P, int q0, int nq, const double *eE, 
 				for (int q = q0; q < q0 + nq; q++) {
 					const double *xe = eE + 2*(q-q0), *xh = eH + 2*(q-q0);
 					if (P->dft64) {
-						double *acc = P->accD + 2*((long)q * 4 * P->ncell);
-						double *p0 = acc + 2*cell, *p1 = acc + 2*(P->ncell + cell), *p2 = acc + 2*(2*P->ncell + cell), *p3 = acc + 2*(3*P->ncell + cell);
-						p0[0] = p0[0] + xe[0] * (double)e1, p0[1] = p0[1] + xe[1] * (double)e1;
-						p1[0] = p1[0] + xe[0] * (double)e2, p1[1] = p1[1] + xe[1] * (double)e2;
-						p2[0] = p2[0] + xh[0] * (double)h1 * 0.5, p2[1] = p2[1] + xh[1] * (double)h1 * 0.5;
-						p3[0] = p3[0] + xh[0] * (double)h2 * 0.5, p3[1] = p3[1] + xh[1] * (double)h2 * 0.5;
+						/* double accumulation with Neumaier compensation */
+						const long o = 2*((long)q * 4 * P->ncell + cell), m = 2*P->ncell;
+						const double y[8] = { xe[0] * (double)e1, xe[1] * (double)e1, xe[0] * (double)e2, xe[1] * (double)e2,
+						                      xh[0] * (double)h1 * 0.5, xh[1] * (double)h1 * 0.5, xh[0] * (double)h2 * 0.5, xh[1] * (double)h2 * 0.5 };
+						for (int c = 0; c < 8; c++) {
+							double *s = P->accD + o + (c / 2) * m + c % 2, *r = P->cmpD + o + (c / 2) * m + c % 2;
+							const double t = *s + y[c];
+							*r += fabs(*s) >= fabs(y[c]) ? (*s - t) + y[c] : (y[c] - t) + *s;
+							*s = t;
+						}
 					}
 					else {
 						/* float complex accumulation as in the reference (exp factor rounded to float) */

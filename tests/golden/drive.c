@@ -787,7 +787,7 @@ void alis_drive_release(void)
 }
 
 /* Run a scenario and record slices of its first world into dir at its last steps: snapshots
- * of Ez and Hy on the plane z = 0 (text) and of Ex on the line through the centre along x
+ * of Ez and Hy on the plane z = 10 (text) and of Ex on the line through the centre along x
  * (text), an Ez image (PNG), the average of Hy over the last 10 steps and the 500 nm phasor of
  * Ez over the last 20 steps (text, 12 frames); returns 0. */
 int alis_drive_slices(const char *name, int steps, const char *dir)
