@@ -202,9 +202,11 @@ __device__ __forceinline__ int batchEntry(const int *start, int n, int b)
 // Several worlds (and real/imaginary parts) in one launch: entry e owns workgroups
 // [start[e], start[e+1]) of a one-dimensional grid; each workgroup updates TPB consecutive
 // tiles of its entry (tiles as in k_curl).
-// FLAT: 1 or 0 when every entry of the launch has packed rows or none has; CM: 1 or 0 when
-// every entry reads 1/eps arrays or none does; -1 to decide per entry.  Specialised launches compile to leaner code (64 k-point worlds: E curl
-// 380 -> 332 us).  H launches (alt < 0), which read the next plane of E, run each entry's
+// FLAT: 1 or 0 when every entry of the launch has packed rows or none has; CM: 2 when every
+// entry takes 1/eps from its per-segment table (and arrays where a segment is mixed), 1 when
+// every entry reads 1/eps arrays, 0 when none does; -1 to decide per entry.  Specialised
+// launches compile to leaner code (64 k-point worlds: E curl 380 -> 332 us); the host runs
+// one launch per class of entries.  H launches (alt < 0), which read the next plane of E, run each entry's
 // tiles in descending order, so that plane was read as a centre just before (64 k-point
 // worlds: H curl 331 -> 306 us; the 256^3 k_curl shows no difference).
 template<class T, int TPB, int FLAT = -1, int CM = -1>
@@ -225,7 +227,8 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 			const int off = (t % per) * 256 + lid, jj = off / pitch - 1, kk = off - (jj + 1) * pitch;
 			if (jj < 0 || jj >= a.nj || kk < 1 || kk > a.nk) continue;
 			const int i = a.iMIN + t / per, j = a.jMIN + jj, k = a.kMIN - 1 + kk;
-			if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true>(a, i, j, k);
+			if (CM == 2 || (CM < 0 && a.seg)) curlAt<T, 2, true, false, true>(a, i, j, k);
+			else if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true>(a, i, j, k);
 			else curlAt<T, 0, true, false, true>(a, i, j, k);
 		}
 		return;
@@ -234,7 +237,8 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 	for (int t0 = (b - start[e]) * TPB, t1 = min(t0 + TPB, tiles); t0 < t1; t0++) {
 		const int t = a.alt < 0 ? tiles - 1 - t0 : t0;
 		const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
-		if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1>(a, bx, by, bz);
+		if (CM == 2 || (CM < 0 && a.seg)) curlCell<T, 2>(a, bx, by, bz);
+		else if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1>(a, bx, by, bz);
 		else curlCell<T, 0>(a, bx, by, bz);
 	}
 }
@@ -1326,12 +1330,15 @@ int rowSegments(alis_dev_world *D)
 				}
 				else mixedCells += k1 - k0 + 1;
 			}
+	// a table that spares the arrays in under a quarter of the cells (rows that all cross
+	// material boundaries) only adds its own reads: the curl then reads the arrays alone
+	const double cells = (double)D->iNUM * D->jNUM * D->kNUM;
+	(void)uniform;
+	if (mixedCells > 0.75 * cells) return 0;
 	if (devAlloc(D, &D->seg, tab.size() * sizeof(T))) return -1;
 	HIPCHECK(hipMemcpyAsync(D->seg, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice, D->st));
 	HIPCHECK(hipStreamSynchronize(D->st));
-	const double cells = (double)D->iNUM * D->jNUM * D->kNUM;
 	D->epsBytes = (mixedCells * 3.0 * D->esz + nseg * 4.0 * D->esz) / cells;
-	(void)uniform;
 	return 0;
 }
 
@@ -2103,9 +2110,11 @@ struct alis_dev_batch {
 	bool fp64 = false;
 	hipStream_t st = nullptr;
 	void *curl[2] = {};         // CurlArgs<T>[] per half step
-	void *curlStart[2] = {};    // int[] workgroup starts
-	int curlN[2] = {}, curlBlocks[2] = {};
-	int curlFlat[2] = {-1, -1}, curlCM[2] = {-1, -1};  // shared by every entry, or -1: mixed
+	void *curlStart[2] = {};    // int[] workgroup starts (from 0 within each class)
+	// the entries in classes of one tile kind and one 1/eps source, each class contiguous in
+	// the tables and run by its own specialised launch
+	struct CurlClass { int first, n, blocks, flat, cm; };
+	std::vector<CurlClass> curlClass[2];
 	bool postReady = false;
 	struct Round {
 		std::vector<int> world;     // index into W
@@ -2136,9 +2145,8 @@ template<class T>
 int batchBuildCurl(alis_dev_batch *B)
 {
 	for (int h = 0; h < 2; h++) {
-		std::vector<CurlArgs<T>> args;
-		std::vector<int> start;
-		int blocks = 0;
+		struct Entry { CurlArgs<T> a; long tiles; int cls; };
+		std::vector<Entry> all;
 		for (alis_dev_world *D : B->W)
 			for (int m = 0; m <= D->cplx; m++) {
 				CurlArgs<T> a = curlArgs<T>(D, h, m);
@@ -2148,16 +2156,23 @@ int batchBuildCurl(alis_dev_batch *B)
 				a.flatPitch = flat ? D->kNUM + 2 : 0;
 				const long tiles = flat ? (D->L.si + 255) / 256 * D->iNUM
 				                        : (long)((D->kNUM + 63) / 64) * ((D->jNUM + 3) / 4) * D->iNUM;
-				args.push_back(a);
-				start.push_back(blocks);
-				blocks += (int)((tiles + CURL_TPB - 1) / CURL_TPB);
+				// 1/eps: 2 per-segment table (and arrays where a segment is mixed), 1 arrays, 0 none
+				const int cm = a.seg ? 2 : a.C0 ? 1 : 0;
+				all.push_back(Entry{a, tiles, (flat ? 3 : 0) + cm});
 			}
-		B->curlN[h] = (int)args.size(), B->curlBlocks[h] = blocks;
-		int nflat = 0, nc = 0;
-		for (const CurlArgs<T> &a : args) nflat += a.flatPitch != 0, nc += a.C0 != nullptr;
-		const int n = (int)args.size();
-		B->curlFlat[h] = nflat == n ? 1 : nflat == 0 ? 0 : -1;
-		B->curlCM[h] = nc == n ? 1 : nc == 0 ? 0 : -1;
+		std::stable_sort(all.begin(), all.end(), [](const Entry &x, const Entry &y) { return x.cls < y.cls; });
+		std::vector<CurlArgs<T>> args;
+		std::vector<int> start;
+		B->curlClass[h].clear();
+		for (const Entry &e : all) {
+			if (B->curlClass[h].empty() || B->curlClass[h].back().flat * 3 + B->curlClass[h].back().cm != e.cls)
+				B->curlClass[h].push_back({(int)args.size(), 0, 0, e.cls / 3, e.cls % 3});
+			alis_dev_batch::CurlClass &c = B->curlClass[h].back();
+			args.push_back(e.a);
+			start.push_back(c.blocks);
+			c.n++;
+			c.blocks += (int)((e.tiles + CURL_TPB - 1) / CURL_TPB);
+		}
 		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
 		if (batchAlloc(B, &B->curlStart[h], start.size() * sizeof(int))) return -1;
 		HIPCHECK(hipMemcpyAsync(B->curl[h], args.data(), args.size() * sizeof(CurlArgs<T>), hipMemcpyHostToDevice, B->st));
@@ -2218,18 +2233,27 @@ int batchCore(alis_dev_batch *B, int h)
 	if (D0->timing) {
 		if (timingEvents(D0, e0, e1)) return -1;
 	}
-	const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h];
-	const int *start = (const int *)B->curlStart[h];
-	const dim3 grid(B->curlBlocks[h]), block(64, 4);
-	const int f = B->curlFlat[h], c = B->curlCM[h];
-#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, F, C>, grid, block, 0, B->st, e0, e1, 0, args, start, B->curlN[h])
-	if (f == 1 && c == 1) CURL_BATCH(1, 1);
-	else if (f == 1 && c == 0) CURL_BATCH(1, 0);
-	else if (f == 0 && c == 1) CURL_BATCH(0, 1);
-	else if (f == 0 && c == 0) CURL_BATCH(0, 0);
-	else CURL_BATCH(-1, -1);
+	const std::vector<alis_dev_batch::CurlClass> &cls = B->curlClass[h];
+	for (size_t q = 0; q < cls.size(); q++) {
+		const alis_dev_batch::CurlClass &c = cls[q];
+		const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h] + c.first;
+		const int *start = (const int *)B->curlStart[h] + c.first;
+		const dim3 grid(c.blocks), block(64, 4);
+		hipEvent_t s0 = q == 0 ? e0 : nullptr, s1 = q + 1 == cls.size() ? e1 : nullptr;
+#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, F, C>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
+		if (c.flat) {
+			if (c.cm == 2) CURL_BATCH(1, 2);
+			else if (c.cm == 1) CURL_BATCH(1, 1);
+			else CURL_BATCH(1, 0);
+		}
+		else {
+			if (c.cm == 2) CURL_BATCH(0, 2);
+			else if (c.cm == 1) CURL_BATCH(0, 1);
+			else CURL_BATCH(0, 0);
+		}
 #undef CURL_BATCH
-	HIPCHECK(hipGetLastError());
+		HIPCHECK(hipGetLastError());
+	}
 	if (e1) {
 		D0->evPairs[h].push_back(e0);
 		D0->evPairs[h].push_back(e1);

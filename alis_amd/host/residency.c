@@ -5,8 +5,17 @@
  * mirrors and refreshed on demand: a small box for get(), everything for the host
  * observers, and everything when timer() reports the end of a loop.
  */
+#define _GNU_SOURCE             /* PTHREAD_RECURSIVE_MUTEX_INITIALIZER_NP */
+#include <pthread.h>
 #include <string.h>
 #include "internal.h"
+
+/* The registries of live worlds and phasers below are shared with the serving loop's setup
+ * thread, which creates worlds and phasers while the stepping thread reads and releases
+ * others: every access holds this (recursive) lock. */
+static pthread_mutex_t registry = PTHREAD_RECURSIVE_MUTEX_INITIALIZER_NP;
+#define LOCK() pthread_mutex_lock(&registry)
+#define UNLOCK() pthread_mutex_unlock(&registry)
 
 static world *liveWorlds;
 /* Batches of updateWorldsE/H (and of updateE/H on worlds with TF/SF boxes), most recently
@@ -85,8 +94,10 @@ void worldPrecision(world W, int fieldBits, int dftBits)
 
 void alisRegisterWorld(world W)
 {
+	LOCK();
 	if (nWorlds == capWorlds) capWorlds = capWorlds ? 2*capWorlds : 16, liveWorlds = realloc(liveWorlds, capWorlds*sizeof(world));
 	liveWorlds[nWorlds++] = W;
+	UNLOCK();
 }
 
 static void forgetProbes(world W);
@@ -101,29 +112,38 @@ void alisForgetWorld(world W)
 		alis_dev_release(W->dev);
 		W->dev = 0;
 	}
+	LOCK();
 	for (int a = 0; a < nWorlds; a++) if (liveWorlds[a] == W) { liveWorlds[a] = liveWorlds[--nWorlds]; break; }
 	for (int a = 0; a < nPhasers; a++) if (livePhasers[a]->W == W) livePhasers[a]->devId = -1;
+	UNLOCK();
 }
 
 void alisRegisterPhaser(phaser P)
 {
+	LOCK();
 	if (nPhasers == capPhasers) capPhasers = capPhasers ? 2*capPhasers : 16, livePhasers = realloc(livePhasers, capPhasers*sizeof(phaser));
 	livePhasers[nPhasers++] = P;
+	UNLOCK();
 }
 
 void alisForgetPhaser(phaser P)
 {
+	LOCK();
 	for (int a = 0; a < nPhasers; a++) if (livePhasers[a] == P) { livePhasers[a] = livePhasers[--nPhasers]; break; }
+	UNLOCK();
 }
 
 /* First element of the phaser array that contains P (arrays end with an f == 0 entry). */
 phaser phaserBase(phaser P)
 {
-	for (int a = 0; a < nPhasers; a++) {
+	phaser found = 0;
+	LOCK();
+	for (int a = 0; a < nPhasers && !found; a++) {
 		phaser B = livePhasers[a];
-		if (P >= B && P < B + B->count) return B;
+		if (P >= B && P < B + B->count) found = B;
 	}
-	return 0;
+	UNLOCK();
+	return found;
 }
 
 static void commitTree(world W)
@@ -168,7 +188,9 @@ void alisSyncPhaser(phaser P);
 void syncWorld(world W)
 {
 	syncTree(W);
+	LOCK();
 	for (int a = 0; a < nPhasers; a++) if (livePhasers[a]->W == W) alisSyncPhaser(livePhasers[a]);
+	UNLOCK();
 }
 
 void alisSyncPhaser(phaser P)
@@ -197,12 +219,14 @@ void alisInvalidate(world W)
 		return;
 	}
 	if (!W->dev) return;
+	LOCK();
 	for (int a = 0; a < nPhasers; a++) {
 		phaser B = livePhasers[a];
 		if (B->W != W) continue;
 		alisSyncPhaser(B);
 		B->devId = -1;
 	}
+	UNLOCK();
 	syncTree(W);
 	releaseTree(W);
 }
@@ -211,10 +235,12 @@ void alisInvalidate(world W)
 void alisDropDevice(world W)
 {
 	if (W->isAux && W->parent && W->parent->dev) W = W->parent;
+	LOCK();
 	for (int a = 0; a < nPhasers; a++) {
 		phaser B = livePhasers[a];
 		if (B->W == W && B->devId >= 0) alisSyncPhaser(B), B->devId = -1;
 	}
+	UNLOCK();
 	releaseTree(W);
 }
 
@@ -315,6 +341,8 @@ void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask
 
 void alisSyncAll(void)
 {
+	LOCK();
 	for (int a = 0; a < nWorlds; a++) if (!liveWorlds[a]->isAux) syncWorld(liveWorlds[a]);
 	for (int a = 0; a < nPhasers; a++) alisSyncPhaser(livePhasers[a]);
+	UNLOCK();
 }

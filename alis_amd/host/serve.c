@@ -1,14 +1,21 @@
 /*
  * Serving scattering solves with continuous batching (an addition to the reference, which
  * runs one world per program).  Requests arrive over time; each is a TF/SF scattering run
- * of one Drude-Lorentz sphere with a 16-frequency phaser box.  A request is admitted as
- * soon as it has arrived and its device memory fits the budget; every tick steps all
- * admitted worlds together (updateWorldsE/updateWorldsH: shared launches, worlds of
- * different sizes in one batch), so requests that arrive while others run join the batch
- * at the next tick and finished ones leave it.  The fields of an admitted request stay on
- * the device until it finishes.  Each result is the same as that request served alone:
- * the shared launches compute every cell of every world exactly as its own launch does.
+ * of one Drude-Lorentz sphere with a 16-frequency phaser box.
+ *
+ * Two threads.  The setup thread builds each request's world once it has arrived (grid,
+ * voxelised sphere, TF/SF box, phasers: host work), in arrival order, at most `slots`
+ * ahead of admission.  The serving thread steps all admitted worlds together
+ * (updateWorldsE/updateWorldsH/updatePhaserArrays: shared launches over worlds of
+ * different sizes) and, between any two steps, admits built requests in arrival order
+ * while a slot is free and their device bytes fit the budget; a request's first step
+ * commits it to the device, where its fields stay until it finishes and is retired.  So
+ * the GPU keeps stepping the admitted worlds while new ones are built, and a request joins
+ * the batch at the first step after it is built.  Each result is the same as that request
+ * served alone: the shared launches compute every cell of every world exactly as its own
+ * launch does.
  */
+#include <pthread.h>
 #include <time.h>
 #include <string.h>
 #include "internal.h"
@@ -19,14 +26,6 @@ static double nowSec(void)
 	struct timespec t;
 	clock_gettime(CLOCK_MONOTONIC, &t);
 	return t.tv_sec + 1e-9 * t.tv_nsec;
-}
-
-static void sleepUntil(double t0, double at)
-{
-	double d = at - (nowSec() - t0);
-	if (d <= 0) return;
-	struct timespec t = {(time_t)d, (long)((d - (time_t)d) * 1e9)};
-	nanosleep(&t, 0);
 }
 
 #define SERVE_DX 2.0f           /* nm */
@@ -81,14 +80,6 @@ static long deviceBytes(world W)
 	return b;
 }
 
-typedef struct {
-	int id;
-	world W;
-	phaser P;
-	int done, steps;
-	double bytes;
-} active;
-
 /* splitmix64 (Steele, Lea, Flood 2014): the state advances by the golden-ratio increment
  * and each output is the state through two xor-shift-multiply rounds. */
 static unsigned long long splitmix(unsigned long long *x)
@@ -120,6 +111,73 @@ void alisServeRequests(unsigned long long seed, int n, double rate, alisRequest 
 	}
 }
 
+typedef struct {
+	int id;
+	world W;
+	phaser P;
+	int done, steps;
+	double bytes;
+} active;
+
+/* The requests built by the setup thread and not yet admitted: request id sits in slot
+ * id % depth while taken <= id < built. */
+typedef struct {
+	pthread_mutex_t m;
+	pthread_cond_t cv;          /* a request was built, taken, or the run stops */
+	const alisRequest *R;
+	alisResult *out;
+	int n, depth, built, taken, stop;
+	active *slot;
+	double t0, setup;
+} setupQueue;
+
+static void waitFor(setupQueue *q, double seconds)
+{
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	const double at = t.tv_sec + 1e-9 * t.tv_nsec + seconds;
+	t.tv_sec = (time_t)at, t.tv_nsec = (long)((at - (double)t.tv_sec) * 1e9);
+	pthread_cond_timedwait(&q->cv, &q->m, &t);
+}
+
+static void *setupThread(void *arg)
+{
+	setupQueue *q = arg;
+	pthread_mutex_lock(&q->m);
+	while (!q->stop && q->built < q->n) {
+		if (q->built - q->taken >= q->depth) {
+			pthread_cond_wait(&q->cv, &q->m);
+			continue;
+		}
+		const double early = q->R[q->built].arrival - (nowSec() - q->t0);
+		if (early > 0) {
+			waitFor(q, early);
+			continue;
+		}
+		const int id = q->built;
+		pthread_mutex_unlock(&q->m);
+		const double s = nowSec();
+		active a = {id, 0, 0, 0, q->R[id].steps, 0};
+		a.W = requestWorld(&q->R[id], &a.P);
+		const double e = nowSec();
+		pthread_mutex_lock(&q->m);
+		q->setup += e - s;
+		q->out[id].ready = e - q->t0;
+		q->slot[id % q->depth] = a;
+		__atomic_store_n(&q->built, id + 1, __ATOMIC_RELEASE);
+		pthread_cond_broadcast(&q->cv);
+	}
+	pthread_mutex_unlock(&q->m);
+	return 0;
+}
+
+static alisServeStats lastStats;
+
+void alisServeLastStats(alisServeStats *s)
+{
+	*s = lastStats;
+}
+
 int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult *out)
 {
 	if (n < 0 || !C || C->stepsPerTick < 1 || C->slots < 1) return -1;
@@ -130,49 +188,69 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 	}
 	int fbits, dbits;
 	alisPrecisionDefault(&fbits, &dbits);
-	int next = 0, finished = 0, na = 0;
-	double used = 0;
+	memset(out, 0, n * sizeof *out);
+	memset(&lastStats, 0, sizeof lastStats);
+	setupQueue q = {.R = R, .out = out, .n = n, .depth = C->slots};
+	pthread_mutex_init(&q.m, 0);
+	pthread_condattr_t ca;
+	pthread_condattr_init(&ca);
+	pthread_condattr_setclock(&ca, CLOCK_MONOTONIC);
+	pthread_cond_init(&q.cv, &ca);
+	pthread_condattr_destroy(&ca);
+	q.slot = calloc(q.depth, sizeof *q.slot);
 	active *A = calloc(n ? n : 1, sizeof *A);
 	world *Ws = calloc(n ? n : 1, sizeof *Ws);
 	phaser *Ps = calloc(n ? n : 1, sizeof *Ps);
-	memset(out, 0, n * sizeof *out);
-	const double t0 = nowSec();
+	int finished = 0, na = 0;
+	double used = 0, stepping = 0;
+	long steps = 0;
+	q.t0 = nowSec();
+	pthread_t setup;
+	if (pthread_create(&setup, 0, setupThread, &q)) return free(q.slot), free(A), free(Ws), free(Ps), -1;
 	while (finished < n) {
-		/* admit, in arrival order, what has arrived and fits (always one if none runs) */
+		/* admit built requests in arrival order while a slot is free and they fit (always
+		 * one on an idle device); wait for one when nothing runs */
 		int joined = 0;
-		while (next < n && na < C->slots && R[next].arrival <= nowSec() - t0) {
-			const double need = requestBytes(&R[next], fbits);
-			if (na && used + need > C->budget) break;
-			active *a = &A[na++];
-			a->id = next;
-			a->W = requestWorld(&R[next], &a->P);
-			a->steps = R[next].steps;
-			a->done = 0;
-			a->bytes = need;
-			used += need;
-			out[next].admitted = nowSec() - t0;
-			next++;
-			joined = 1;
+		pthread_mutex_lock(&q.m);
+		for (;;) {
+			while (q.taken < q.built && na < C->slots) {
+				const double need = requestBytes(&R[q.taken], fbits);
+				if (na && used + need > C->budget) break;
+				active *a = &A[na++];
+				*a = q.slot[q.taken % q.depth];
+				a->bytes = need;
+				used += need;
+				out[a->id].admitted = nowSec() - q.t0;
+				q.taken++;
+				joined = 1;
+				pthread_cond_broadcast(&q.cv);
+			}
+			if (na) break;
+			pthread_cond_wait(&q.cv, &q.m);
 		}
-		if (!na) {
-			sleepUntil(t0, R[next].arrival);
-			continue;
-		}
+		pthread_mutex_unlock(&q.m);
 		int k = C->stepsPerTick;
 		for (int a = 0; a < na; a++) if (A[a].steps - A[a].done < k) k = A[a].steps - A[a].done;
 		for (int a = 0; a < na; a++) Ws[a] = A[a].W, Ps[a] = A[a].P;
-		for (int s = 0; s < k; s++) {
+		const double s0 = nowSec();
+		int s = 0;
+		while (s < k) {
 			updateWorldsE(Ws, na);
 			updateWorldsH(Ws, na);
 			updatePhaserArrays(Ws, Ps, na);
+			s++;
+			/* a built request and a free slot: admit it before the next step */
+			if (na < C->slots && __atomic_load_n(&q.built, __ATOMIC_ACQUIRE) > q.taken) break;
 		}
+		stepping += nowSec() - s0;
+		steps += s;
 		if (joined) {
 			/* the new worlds are committed now: replace estimates by the real footprint */
 			used = 0;
 			for (int a = 0; a < na; a++) used += A[a].bytes = deviceBytes(A[a].W);
 		}
 		for (int a = 0; a < na; a++) {
-			A[a].done += k;
+			A[a].done += s;
 			if (out[A[a].id].batchMax < na) out[A[a].id].batchMax = na;
 			out[A[a].id].bytes = A[a].bytes;
 		}
@@ -184,7 +262,7 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 				r->spectrum[f] = phaserPoyntingOut(&A[a].P[f]);
 				r->wavelength[f] = 1 / A[a].P[f].f;
 			}
-			r->finished = nowSec() - t0;
+			r->finished = nowSec() - q.t0;
 			r->latency = r->finished - R[A[a].id].arrival;
 			r->cellSteps = (double)A[a].W->iNUM * A[a].W->jNUM * A[a].W->kNUM * A[a].steps;
 			deletePhaser(A[a].P);
@@ -194,6 +272,18 @@ int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult 
 			finished++;
 		}
 	}
+	pthread_mutex_lock(&q.m);
+	q.stop = 1;
+	pthread_cond_broadcast(&q.cv);
+	pthread_mutex_unlock(&q.m);
+	pthread_join(setup, 0);
+	lastStats.makespan = nowSec() - q.t0;
+	lastStats.stepping = stepping;
+	lastStats.setup = q.setup;
+	lastStats.steps = steps;
+	pthread_cond_destroy(&q.cv);
+	pthread_mutex_destroy(&q.m);
+	free(q.slot);
 	free(A);
 	free(Ws);
 	free(Ps);

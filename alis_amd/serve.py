@@ -19,7 +19,8 @@ class Request(ctypes.Structure):
 
 
 class Result(ctypes.Structure):
-    _fields_ = [("admitted", ctypes.c_double), ("finished", ctypes.c_double), ("latency", ctypes.c_double),
+    _fields_ = [("ready", ctypes.c_double), ("admitted", ctypes.c_double), ("finished", ctypes.c_double),
+                ("latency", ctypes.c_double),
                 ("cellSteps", ctypes.c_double), ("bytes", ctypes.c_double), ("batchMax", ctypes.c_int),
                 ("wavelength", ctypes.c_float * 16), ("spectrum", ctypes.c_float * 16)]
 
@@ -28,11 +29,18 @@ class Config(ctypes.Structure):
     _fields_ = [("budget", ctypes.c_double), ("slots", ctypes.c_int), ("stepsPerTick", ctypes.c_int)]
 
 
+class Stats(ctypes.Structure):
+    _fields_ = [("makespan", ctypes.c_double), ("stepping", ctypes.c_double), ("setup", ctypes.c_double),
+                ("steps", ctypes.c_long)]
+
+
 def _bind(lib):
     lib.alisServe.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.POINTER(Config), ctypes.POINTER(Result)]
     lib.alisServe.restype = ctypes.c_int
     lib.alisServeRequests.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Request)]
     lib.alisServeMaxRadius.argtypes = [ctypes.c_int]
+    lib.alisServeLastStats.argtypes = [ctypes.POINTER(Stats)]
+    lib.alisServeLastStats.restype = None
     return lib
 
 
@@ -63,12 +71,29 @@ def run(reqs, budget, slots, steps_per_tick, lib=None):
     return out
 
 
-def summary(reqs, res):
+def last_stats(lib=None):
+    """Statistics of the last run: makespan, seconds stepping and building, steps."""
+    lib = _bind(lib or library())
+    s = Stats()
+    lib.alisServeLastStats(ctypes.byref(s))
+    return {"makespan_s": s.makespan, "stepping_s": s.stepping, "setup_s": s.setup, "steps": s.steps}
+
+
+def summary(reqs, res, stats=None):
     lat = np.array([r.latency for r in res])
     wait = np.array([r.admitted - q.arrival for q, r in zip(reqs, res)])
+    build = np.array([r.ready - q.arrival for q, r in zip(reqs, res)])
     span = max(r.finished for r in res) - min(q.arrival for q in reqs)
     cells = sum(r.cellSteps for r in res)
+    extra = {}
+    if stats:
+        # share of the run the serving thread spent in shared steps, whose launches keep the
+        # GPU busy (the rest: waiting for a request to be built, admissions, retirements)
+        extra = {"stepping_fraction": stats["stepping_s"] / stats["makespan_s"], "setup_s": stats["setup_s"],
+                 "shared_steps": stats["steps"]}
     return {
+        **extra,
+        "p50_build_s": float(np.percentile(build, 50)),
         "requests": len(res),
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),

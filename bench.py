@@ -284,14 +284,29 @@ def serving(alis, args):
     reqs = serve.requests(args.serve_seed, args.serve, args.serve_rate, alis)
     with quiet():
         res = serve.run(reqs, args.serve_budget_gib * 2**30, args.serve_slots, args.serve_tick, alis)
-    s = serve.summary(reqs, res)
+    s = serve.summary(reqs, res, serve.last_stats(alis))
+    # the accuracy the latencies are quoted at: the request with the fewest cell-steps served
+    # again alone in FP64 (the FP64 path agrees with the FP64 oracle to 1e-12), against its
+    # spectrum from the mixed FP32-field run
+    q = min(range(len(reqs)), key=lambda a: res[a].cellSteps)
+    os.environ["ALIS_PRECISION"] = "fp64"
+    with quiet():
+        ref = serve.run(serve.make([(0.0, reqs[q].cells, reqs[q].steps, reqs[q].radius, reqs[q].plasma)]),
+                        args.serve_budget_gib * 2**30, 1, args.serve_tick, alis)[0]
+    os.environ["ALIS_PRECISION"] = "fp32"
+    a, b = list(res[q].spectrum), list(ref.spectrum)
+    spec_err = max(abs(x - y) for x, y in zip(a, b)) / max(abs(y) for y in b)
     return {
         "workload": f"{args.serve} requests, Poisson arrivals {args.serve_rate:g}/s (seed {args.serve_seed}), grid edge "
                     "{64,96,128,192} incl. 8-cell CPML, Drude-Lorentz sphere r 10-40 cells, TF/SF plane wave "
                     "400-900 nm, 1000-6000 steps, 16-wavelength phaser; FP32 fields / FP64 DFT",
         "slots": args.serve_slots, "steps_per_tick": args.serve_tick, "budget_gib": args.serve_budget_gib,
         "p50_latency_s": round(s["p50_latency_s"], 3), "p99_latency_s": round(s["p99_latency_s"], 3),
-        "p50_queue_s": round(s["p50_queue_s"], 3), "makespan_s": round(s["makespan_s"], 3),
+        "p50_queue_s": round(s["p50_queue_s"], 3), "p50_build_s": round(s["p50_build_s"], 3),
+        "makespan_s": round(s["makespan_s"], 3),
+        "stepping_fraction": round(s["stepping_fraction"], 3), "setup_thread_s": round(s["setup_s"], 2),
+        "spectrum_rel_err_vs_fp64": spec_err,
+        "accuracy_request": f"{reqs[q].cells}^3, {reqs[q].steps} steps, r {reqs[q].radius:g} cells",
         "value": round(s["mcell_updates_per_s"], 1), "unit": "Mcell-updates/s",
         "max_batch": s["max_batch"], "max_request_gib": round(s["peak_request_bytes"] / 2**30, 3),
         "offered_gcell_updates": round(sum(r.cellSteps for r in res) / 1e9, 1),

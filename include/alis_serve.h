@@ -11,13 +11,15 @@
  * that, stepped `steps` times.  FP32 fields with FP64 DFT accumulators unless ALIS_PRECISION
  * says otherwise.
  *
- * alisServe runs a list of requests sorted by arrival time (seconds after the call): a
- * request is admitted once it has arrived, a slot is free and the device bytes of all
- * admitted requests stay within the budget (the first request on an idle device is always
- * admitted); all admitted requests are stepped together in shared launches
- * (updateWorldsE/updateWorldsH), with admissions and retirements every `stepsPerTick` steps
- * or sooner when a request finishes.  Fields stay on the device from admission to
- * retirement.  Each result is bit for bit the one the request gets when served alone.
+ * alisServe runs a list of requests sorted by arrival time (seconds after the call).  A
+ * setup thread builds each request's world on the host once it has arrived (in arrival
+ * order, at most `slots` ahead of admission).  A built request is admitted, between any two
+ * steps, once a slot is free and the device bytes of all admitted requests stay within the
+ * budget (the first request on an idle device is always admitted); all admitted requests
+ * are stepped together in shared launches (updateWorldsE/updateWorldsH), and finished ones
+ * are retired at least every `stepsPerTick` steps.  Fields stay on the device from
+ * admission to retirement.  Each result is bit for bit the one the request gets when
+ * served alone.  alisServe itself is not reentrant: one call at a time per process.
  */
 #ifndef ALIS_SERVE_H
 #define ALIS_SERVE_H
@@ -33,6 +35,7 @@ typedef struct {
 } alisRequest;
 
 typedef struct {
+	double ready;                         /* seconds after the start: its world is built */
 	double admitted, finished, latency;   /* seconds after the start; finished - arrival */
 	double cellSteps;                     /* grid cells x steps */
 	double bytes;                         /* device bytes of the request */
@@ -49,6 +52,16 @@ typedef struct {
 
 /* Serve R[0..n-1]; results in out[0..n-1].  0, or -1 for an invalid list or configuration. */
 int alisServe(const alisRequest *R, int n, const alisServeConfig *C, alisResult *out);
+
+typedef struct {
+	double makespan;            /* seconds from the start to the last retirement */
+	double stepping;            /* seconds the serving thread spent in shared steps */
+	double setup;               /* seconds the setup thread spent building worlds */
+	long steps;                 /* shared steps taken */
+} alisServeStats;
+
+/* Statistics of the last alisServe call. */
+void alisServeLastStats(alisServeStats *s);
 
 /* Largest sphere radius (cells) a request of grid edge `cells` can hold. */
 int alisServeMaxRadius(int cells);

@@ -47,7 +47,7 @@ def test_headers_parse():
         assert n in api, n
     assert {"alis_dev_commit", "alis_dev_core", "alis_dev_strips", "alis_dev_ghosts", "alis_dev_phaser_update"} <= dev
     assert len(api) > 150 and len(dev) >= 20
-    assert declared("alis_serve.h") == {"alisServe", "alisServeMaxRadius", "alisServeRequests"}
+    assert declared("alis_serve.h") == {"alisServe", "alisServeLastStats", "alisServeMaxRadius", "alisServeRequests"}
 
 
 @pytest.mark.parametrize("header", ["alis.h", "alis_dev.h", "alis_serve.h"])
