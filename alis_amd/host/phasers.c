@@ -244,13 +244,6 @@ const double *phaserDoubles(phaser B, phaserGeom *g)
 	return B->view;
 }
 
-/* Component comp (E1 E2 H1 H2) of face `kind` at face indices (a, b) of frequency q. */
-double complex faceValue(const double *v, const phaserGeom *g, int q, int kind, int comp, int a, int b)
-{
-	const double *p = v + 2 * (((long)q * 4 + comp) * g->ncell + g->off[kind] + (long)a * g->n2[kind] + b);
-	return p[0] + I * p[1];
-}
-
 void neumaierAdd(double *sum, double *comp, double x)
 {
 	double t = *sum + x;
@@ -259,71 +252,6 @@ void neumaierAdd(double *sum, double *comp, double x)
 	*sum = t;
 }
 
-
-/* Outward flux Re(E x H*) through one face (src/phaser.c:142-223): trapezoid over the
- * face's E nodes weighted by the E and H cell widths. */
-static double faceFlux(phaser P, const double *v, const phaserGeom *g, int q, int kind)
-{
-	if (!g->on[kind]) return 0;
-	world W = P->W;
-	double s = 0, c = 0;
-	int ni = P->iMax, nj = P->jMax, nk = P->kMax;
-	if (kind < 2) {          /* x face: E1 = Ey, E2 = Ez, H1 = Hy, H2 = Hz */
-		for (int j = P->jMin+1; j <= nj; j++) {
-			double r = 0;
-			for (int k = P->kMin; k <= nk; k++) {
-				double w = (k == P->kMin || k == nk) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
-				r += w * creal(faceValue(v, g, q, kind, 0, j-P->jMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 3, j-P->jMin, k-P->kMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dy[j] * r);
-		}
-		for (int k = P->kMin+1; k <= nk; k++) {
-			double r = 0;
-			for (int j = P->jMin; j <= nj; j++) {
-				double w = (j == P->jMin || j == nj) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
-				r -= w * creal(faceValue(v, g, q, kind, 1, j-P->jMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 2, j-P->jMin, k-P->kMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dz[k] * r);
-		}
-	}
-	else if (kind < 4) {     /* y face: E1 = Ez, E2 = Ex, H1 = Hz, H2 = Hx */
-		for (int k = P->kMin+1; k <= nk; k++) {
-			double r = 0;
-			for (int i = P->iMin; i <= ni; i++) {
-				double w = (i == P->iMin || i == ni) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
-				r += w * creal(faceValue(v, g, q, kind, 0, i-P->iMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 3, i-P->iMin, k-P->kMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dz[k] * r);
-		}
-		for (int i = P->iMin+1; i <= ni; i++) {
-			double r = 0;
-			for (int k = P->kMin; k <= nk; k++) {
-				double w = (k == P->kMin || k == nk) ? 0.5 * W->CE->dz[k] : W->CE->dz[k];
-				r -= w * creal(faceValue(v, g, q, kind, 1, i-P->iMin, k-P->kMin) * conj(faceValue(v, g, q, kind, 2, i-P->iMin, k-P->kMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dx[i] * r);
-		}
-	}
-	else {                   /* z face: E1 = Ex, E2 = Ey, H1 = Hx, H2 = Hy */
-		for (int i = P->iMin+1; i <= ni; i++) {
-			double r = 0;
-			for (int j = P->jMin; j <= nj; j++) {
-				double w = (j == P->jMin || j == nj) ? 0.5 * W->CE->dy[j] : W->CE->dy[j];
-				r += w * creal(faceValue(v, g, q, kind, 0, i-P->iMin, j-P->jMin) * conj(faceValue(v, g, q, kind, 3, i-P->iMin, j-P->jMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dx[i] * r);
-		}
-		for (int j = P->jMin+1; j <= nj; j++) {
-			double r = 0;
-			for (int i = P->iMin; i <= ni; i++) {
-				double w = (i == P->iMin || i == ni) ? 0.5 * W->CE->dx[i] : W->CE->dx[i];
-				r -= w * creal(faceValue(v, g, q, kind, 1, i-P->iMin, j-P->jMin) * conj(faceValue(v, g, q, kind, 2, i-P->iMin, j-P->jMin)));
-			}
-			neumaierAdd(&s, &c, W->CH->dy[j] * r);
-		}
-	}
-	return s + c;
-}
 
 /* Trapezoid weights of the E nodes lo..hi along one axis (half weight at both ends). */
 static void trapezoid(double *w, const float *d, int lo, int hi)
@@ -338,9 +266,8 @@ static double rowSum(const double *rows, int a)
 	return r;
 }
 
-/* faceFlux with the inner trapezoid sums computed where the accumulators live
- * (alis_dev_phaser_flux_sums): each face is one row term (inner axis = the face's fast axis)
- * and one column term; rows holds 64 partial sums per row, cols one sum per column. */
+/* A face's flux from its inner trapezoid sums (see faceFluxes): rows holds 64 partial sums
+ * per row, cols one sum per column; the outer sums weight them by the H cell widths. */
 static double faceFluxSums(phaser P, int kind, const double *rows, const double *cols)
 {
 	world W = P->W;
@@ -360,9 +287,36 @@ static double faceFluxSums(phaser P, int kind, const double *rows, const double 
 	return s + c;
 }
 
-/* The six faces' fluxes of P from the device sums: one call per frequency, which reads that
- * frequency's accumulators once and returns about 64 + 1 doubles per face row. */
-static void faceFluxesDev(phaser P, phaser B, const phaserGeom *g, int q, double *face)
+/* The inner sums of the flux terms on host accumulators v (the device read layout), in the
+ * form and order of the engine's alis_dev_phaser_flux_sums: 64 in-order lane sums per row
+ * term, in-order column sums, each cell adding w * (xr yr + xi yi). */
+static void fluxSumsHost(const double *v, const phaserGeom *g, int q, int terms[][8], int nterm, const double *w,
+                         double *out, long nout)
+{
+	memset(out, 0, nout * sizeof *out);
+	for (int n = 0; n < nterm; n++) {
+		const int *u = terms[n];
+		const long na = u[2], nb = u[3];
+		const double *x = v + 2 * (((long)q * 4 + u[4]) * g->ncell + u[1]);
+		const double *y = v + 2 * (((long)q * 4 + u[5]) * g->ncell + u[1]);
+		double *o = out + u[7];
+		for (long a = 0; a < na; a++)
+			for (long b = 0; b < nb; b++) {
+				const double *p = x + 2 * (a * nb + b), *r = y + 2 * (a * nb + b);
+				const double t = p[0] * r[0] + p[1] * r[1];
+				if (u[0] == 0) o[a * 64 + b % 64] += w[u[6] + b] * t;
+				else o[b] += w[u[6] + a] * t;
+			}
+	}
+}
+
+/* The six faces' fluxes of frequency q of P (src/phaser.c:142-223): trapezoid sums over each
+ * face's E nodes weighted by the E and H cell widths, as one row term (inner axis = the
+ * face's fast axis) and one column term per face.  The inner sums come from the engine,
+ * which reads that frequency's accumulators once and returns about 64 + 1 doubles per face
+ * row, or, with host accumulators v, from the same sums on the host: either way the same
+ * bits. */
+static void faceFluxes(phaser P, phaser B, const phaserGeom *g, int q, const double *v, double *face)
 {
 	world W = P->W;
 	const int ni = B->iMax-B->iMin+1, nj = B->jMax-B->jMin+1, nk = B->kMax-B->kMin+1;
@@ -388,7 +342,8 @@ static void faceFluxesDev(phaser P, phaser B, const phaserGeom *g, int q, double
 	}
 	if (!nterm) { free(w); return; }
 	double *out = malloc((size_t)nout * sizeof(double));
-	alisFatal("phaser flux", alis_dev_phaser_flux_sums(W->dev, B->devId, q, nterm, &terms[0][0], w, ni + nj + nk, out, nout));
+	if (v) fluxSumsHost(v, g, q, terms, nterm, w, out, nout);
+	else alisFatal("phaser flux", alis_dev_phaser_flux_sums(W->dev, B->devId, q, nterm, &terms[0][0], w, ni + nj + nk, out, nout));
 	for (int kind = 0; kind < 6; kind++)
 		if (g->on[kind]) face[kind] = faceFluxSums(P, kind, out + rowOff[kind], out + colOff[kind]);
 	free(out);
@@ -407,11 +362,11 @@ double phaserFluxD(phaser P)
 	double face[6];
 	if (B->devId >= 0 && W->dev) {
 		phaserFaces(B, &g);
-		faceFluxesDev(P, B, &g, q, face);
+		faceFluxes(P, B, &g, q, 0, face);
 	}
 	else {
 		const double *v = phaserDoubles(B, &g);
-		for (int kind = 0; kind < 6; kind++) face[kind] = faceFlux(P, v, &g, q, kind);
+		faceFluxes(P, B, &g, q, v, face);
 	}
 	double s = 0, c = 0;
 	neumaierAdd(&s, &c, face[1]);

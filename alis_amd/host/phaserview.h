@@ -16,7 +16,6 @@ typedef struct {
 
 void phaserFaces(phaser B, phaserGeom *g);
 const double *phaserDoubles(phaser B, phaserGeom *g);
-double complex faceValue(const double *v, const phaserGeom *g, int q, int kind, int comp, int a, int b);
 void neumaierAdd(double *sum, double *comp, double x);
 double phaserFluxD(phaser P);
 source normalisingSource(world W);

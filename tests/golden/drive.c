@@ -814,6 +814,41 @@ int alis_drive_slices(const char *name, int steps, const char *dir)
 	return 0;
 }
 
+/* The host library's residency control (alis_amd/host/residency.c), which brings a world's
+ * state and its phasers' accumulators home and drops the device copy, and its double
+ * precision net flux behind phaserPoyntingOut (alis_amd/host/phasers.c). */
+void alisInvalidate(world W);
+double phaserFluxD(phaser P);
+
+/* The double net flux (phaserFluxD, phaserPoyntingOut before its rounding to float) of a
+ * phaser array whose x and y faces have rows of 181 cells (longer than the 64 lane sums of
+ * the engine's flux terms): an 88 x 88 x 408 nm vacuum world (dx 2 nm, 6-cell CPML) with an
+ * Ex dipole, phaser box +-16 x +-16 x +-180 nm, 5 wavelengths 500-700 nm.  After `steps`
+ * steps, first with the accumulators on the device (flux sums from the engine), then after
+ * alisInvalidate brought them home (the same sums on the host): out[2q], out[2q + 1].
+ * Returns the number of frequencies (at most maxq). */
+int alis_drive_flux_paths(int steps, double *out, int maxq)
+{
+	dom D = {{-44, 44}, {-44, 44}, {-204, 204}};
+	res R = {2, {2}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {6}};
+	world W = createWorld(D, R, S, "");
+	pointDipole(W, Ex, 2, -2, 10, Band, 400, 800);
+	phaser P = createPhasers(W, 500, 700, 50, -16, 16, -16, 16, -180, 180);
+	for (int s = 0; s < steps; s++) {
+		updateE(W);
+		updateH(W);
+		updatePhaser(W, P);
+	}
+	int n = 0;
+	for (; n < maxq && P[n].f; n++) out[2 * n] = phaserFluxD(P + n);
+	alisInvalidate(W);
+	for (int q = 0; q < n; q++) out[2 * q + 1] = phaserFluxD(P + q);
+	deletePhaser(P);
+	deleteWorld(W);
+	return n;
+}
+
 /* The host library's double-precision far field (farFieldD, alis_amd/host/farfield.c). */
 double complex farFieldD(phaser P, tfield pol(phaser,float,float), double theta, double phi);
 
