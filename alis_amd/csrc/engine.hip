@@ -105,6 +105,8 @@ template<class T> struct CurlArgs {
 	CurlFace<T> face[6];        // xMin yMin zMin xMax yMax zMax
 	const ObjDev<T> *obj;       // dispersive objects (E: scaling and -sum J before the curl;
 	int nobj, m;                //  H: current update on the new E), part m
+	const int *otStart;         // per 64 x 4 tile (k_curl's tiles): its objects are
+	const int *otList;          //  otList[otStart[t] .. otStart[t+1]) ascending; null: all
 	int flatPitch;              // batched launches of packed short rows: row length with ghosts
 };
 
@@ -963,6 +965,7 @@ struct alis_dev_world {
 	DevSlab slab[2][6];
 	std::vector<DevObj> obj;
 	void *objTab = nullptr;     // ObjDev<T>[obj.size()] in device memory
+	void *objTile = nullptr;    // int[]: per-tile object lists (see objectTiles) or null
 	std::vector<DevSrc> src[2];
 	std::vector<DevTfsf> tf;
 	// end-of-half-step programs (k_post launches), built at the first ghosts call after the
@@ -1342,6 +1345,39 @@ int rowSegments(alis_dev_world *D)
 	return 0;
 }
 
+// Per-tile object lists for worlds with more dispersive objects than OBJ_SCAN: for every
+// 64 x 4 tile of the curl (an x plane, 4 rows, 64 cells along k) the objects whose clipped
+// box meets it, in ascending order, so that a cell visits only those (and adds their
+// currents in the same order as a scan of all objects would).  Layout: ntile + 1 starts,
+// then the lists.
+constexpr int OBJ_SCAN = 2;
+
+int objectTiles(alis_dev_world *D)
+{
+	if ((int)D->obj.size() <= OBJ_SCAN) return 0;
+	const int njb = (D->jNUM + 3) / 4, nkb = (D->kNUM + 63) / 64;
+	const long ntile = (long)D->iNUM * njb * nkb;
+	std::vector<int> start(ntile + 1, 0);
+	auto each = [&](const Box &b, auto &&f) {
+		if (b.i1 < b.i0 || b.j1 < b.j0 || b.k1 < b.k0) return;
+		for (int i = b.i0; i <= b.i1; i++)
+			for (int jb = (b.j0 - D->jMIN) / 4; jb <= (b.j1 - D->jMIN) / 4; jb++)
+				for (int kb = (b.k0 - D->kMIN) / 64; kb <= (b.k1 - D->kMIN) / 64; kb++)
+					f(((long)(i - D->iMIN) * njb + jb) * nkb + kb);
+	};
+	for (const DevObj &O : D->obj) each(O.clip, [&](long t) { start[t + 1]++; });
+	for (long t = 0; t < ntile; t++) start[t + 1] += start[t];
+	std::vector<int> table(start.begin(), start.end());
+	table.resize(ntile + 1 + start[ntile]);
+	std::vector<int> fill(start.begin(), start.end() - 1);
+	for (int n = 0; n < (int)D->obj.size(); n++)
+		each(D->obj[n].clip, [&](long t) { table[ntile + 1 + fill[t]++] = n; });
+	if (devAlloc(D, &D->objTile, table.size() * sizeof(int))) return -1;
+	HIPCHECK(hipMemcpyAsync(D->objTile, table.data(), table.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
 // Commit: allocate and upload everything of the descriptor.
 template<class T>
 int commitWorld(alis_dev_world *D)
@@ -1478,6 +1514,7 @@ int commitWorld(alis_dev_world *D)
 		if (devAlloc(D, &D->objTab, tab.size() * sizeof(ObjDev<T>))) return -1;
 		HIPCHECK(hipMemcpyAsync(D->objTab, tab.data(), tab.size() * sizeof(ObjDev<T>), hipMemcpyHostToDevice, D->st));
 		HIPCHECK(hipStreamSynchronize(D->st));
+		if (objectTiles(D)) return -1;
 	}
 
 	// sources
@@ -1541,6 +1578,10 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 	}
 	a.obj = (const ObjDev<T> *)D->objTab;
 	a.nobj = (int)D->obj.size();
+	if (D->objTile) {
+		const long ntile = (long)D->iNUM * ((D->jNUM + 3) / 4) * ((D->kNUM + 63) / 64);
+		a.otStart = (const int *)D->objTile, a.otList = (const int *)D->objTile + ntile + 1;
+	}
 	a.m = m;
 	return a;
 }

@@ -8,6 +8,9 @@
  *              0.5), 16-cell CPML on all faces, x-polarised plane wave with a 400-900 nm
  *              band waveform, a phaser box of 32 frequencies around the TF/SF box.  `cells`
  *              is the grid edge including CPML (256 for the 256^3 workload).
+ *   particles  the nanoblock grid with 64 gold spheres (radius 10-16 nm, pitch 80 nm) in
+ *              glass instead of the block, TF/SF plane wave, no phaser: many dispersive
+ *              objects.
  *   dipole     Ez point dipole (600 nm pulse) in vacuum: dx = 10 nm, Courant 0.5, 10-cell
  *              CPML, 8 fixed probes; `cells` = grid edge including CPML (64).
  *   kpoints    a band sweep: `cells` worlds (k-points along Gamma-X-M), each a photonic
@@ -126,6 +129,33 @@ void *wl_create(const char *name, int cells, int fieldBits, int dftBits)
 		dom DTF = {{-50, 50}, {-50, 50}, {-30, 60}};
 		planewave(c->W, DTF, Ppol, 0, 0, Band, 400, 900);
 		c->P = createPhasers(c->W, 400, 900, 16.5, -70, 70, -70, 70, -50, 80);
+	}
+	else if (!strcmp(name, "particles")) {
+		/* 64 gold spheres (Drude + 2 Lorentz pairs) of radius 10-16 nm on a 4 x 4 x 4 lattice
+		 * of pitch 80 nm in glass, TF/SF plane wave: the nanoblock grid with many dispersive
+		 * objects */
+		float lo, hi;
+		extent(cells, 16, 2, &lo, &hi);
+		dom D = {{lo, hi}, {lo, hi}, {lo, hi}};
+		res R = {1, {2}, {1240}};
+		sur S = {{PML}, {PML}, {PML}, {16}};
+		c->W = createWorld(D, R, S, "");
+		matter Gold = {{1.5}, {{9, 0.07}, {2.6, 0.3, 0.5}, {3.5, 0.6, 0.8}}};
+		matter M[65];
+		object O[65];
+		int n = 0;
+		for (int a = 0; a < 4; a++)
+			for (int b = 0; b < 4; b++)
+				for (int z = 0; z < 4; z++, n++) {
+					const float x = -120 + 80 * a, y = -120 + 80 * b, h = -120 + 80 * z, r = 10 + 2 * (n % 4);
+					M[n] = Gold;
+					O[n] = (object){Ball, {{x - r, x + r}, {y - r, y + r}, {h - r, h + r}}};
+				}
+		M[n] = n(1.5);
+		O[n] = (object){0};
+		putObjectArray(c->W, M, O);
+		dom DTF = {{-170, 170}, {-170, 170}, {-170, 170}};
+		planewave(c->W, DTF, Ppol, 0, 0, Band, 400, 900);
 	}
 	else if (!strcmp(name, "dipole")) {
 		float lo, hi;

@@ -302,6 +302,41 @@ static void nanoblockSetup(scen *c)
 	memcpy(c->F, F, sizeof F);
 	memcpy(c->pts, pts, sizeof pts);
 }
+/* 18 metal spheres (radii 6-10 nm) on a 3 x 3 x 2 lattice of pitch 24 x 24 x 20 nm,
+ * alternately a Drude + 2-Lorentz gold and a Drude silver, in glass, lit by a TF/SF plane
+ * wave: many dispersive objects, each cell inside at most one of them. */
+static void particlesSetup(scen *c)
+{
+	dom DTF = {{40}, {40}, {24}};
+	dom DSF = {{52}, {52}, {36}};
+	res R = {2, {4}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	world W = createWorld(DSF, R, S, "");
+	matter Gold = {{1.5}, {{9, 0.07}, {2.6, 0.3, 0.5}, {3.5, 0.6, 0.8}}};
+	matter Silver = {{1.2}, {{8.9, 0.04}}};
+	matter M[19];
+	object O[19];
+	int n = 0;
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b < 3; b++)
+			for (int z = 0; z < 2; z++, n++) {
+				const float x = -24 + 24 * a, y = -24 + 24 * b, h = -10 + 20 * z, r = 6 + 2 * (n % 3);
+				M[n] = n % 2 ? Silver : Gold;
+				O[n] = (object){Ball, {{x - r, x + r}, {y - r, y + r}, {h - r, h + r}}};
+			}
+	M[n] = n(1.5);
+	O[n] = (object){0};
+	putObjectArray(W, M, O);
+	planewave(W, DTF, Ppol, 0, 0, Pulse, 600, 400);
+	phaser P = createPhasers(W, 500, 700, 100, -46, 46, -46, 46, -30, 30);
+	c->absorbers[c->nabs++] = O[4];
+	c->absorbers[c->nabs++] = O[13];
+	field *F[4] = { Ex, Ez, Hy, Ex };
+	float pts[4][3] = {{0,0,10}, {24,-24,-10}, {12,12,0}, {-30,20,25}};
+	c->W = W, c->P = P, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
+}
 static void obliqueSetup(scen *c)
 {
 	dom DTF = {{50}, {50}, {30}};
@@ -468,6 +503,7 @@ static int setupScenario(scen *c, const char *name)
 	if (!strcmp(name, "pillars")) return pillarsSetup(c), 0;
 	if (!strcmp(name, "hexagonal")) return hexagonalSetup(c), 0;
 	if (!strcmp(name, "multipole")) return multipoleSetup(c), 0;
+	if (!strcmp(name, "particles")) return particlesSetup(c), 0;
 	return fprintf(stderr, "unknown scenario %s\n", name), 2;
 }
 
@@ -822,7 +858,7 @@ double phaserFluxD(phaser P);
 
 /* The double net flux (phaserFluxD, phaserPoyntingOut before its rounding to float) of a
  * phaser array whose x and y faces have rows of 181 cells (longer than the 64 lane sums of
- * the engine's flux terms): an 88 x 88 x 408 nm vacuum world (dx 2 nm, 6-cell CPML) with an
+ * the engine's flux terms): an 88 x 88 x 408 nm vacuum world (dx 2 nm, Courant 0.5, 6-cell CPML) with an
  * Ex dipole, phaser box +-16 x +-16 x +-180 nm, 5 wavelengths 500-700 nm.  After `steps`
  * steps, first with the accumulators on the device (flux sums from the engine), then after
  * alisInvalidate brought them home (the same sums on the host): out[2q], out[2q + 1].
@@ -830,7 +866,7 @@ double phaserFluxD(phaser P);
 int alis_drive_flux_paths(int steps, double *out, int maxq)
 {
 	dom D = {{-44, 44}, {-44, 44}, {-204, 204}};
-	res R = {2, {2}, {1240}};
+	res R = {1, {2}, {1240}};
 	sur S = {{PML}, {PML}, {PML}, {6}};
 	world W = createWorld(D, R, S, "");
 	pointDipole(W, Ex, 2, -2, 10, Band, 400, 800);

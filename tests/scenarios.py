@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parent.parent
 GOLDEN = ROOT / "tests" / "golden"
 MANIFEST = json.loads((GOLDEN / "manifest.json").read_text())
 SCENARIOS = MANIFEST["scenarios"]
-WITH_PHASERS = ("symband", "nanoblock", "oblique", "pillars")
+WITH_PHASERS = ("symband", "nanoblock", "oblique", "pillars", "particles")
 # "kpoints" runs the 40 worlds kp0..kp39 of the k-point sweep together; their output keys
 # start with their own names
 KPOINTS = tuple(f"kp{q}" for q in range(40))

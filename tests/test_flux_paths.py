@@ -3,11 +3,9 @@ before its rounding to float) is the same whether the phaser's accumulators are 
 device (inner trapezoid sums from alis_dev_phaser_flux_sums: 64 lane sums per face row) or
 were brought home by alisInvalidate (the same sums restated on the host in the same order,
 alis_amd/host/phasers.c): equal to the bit, for a phaser whose x and y faces have rows of
-181 cells, filled with field after 240 steps.  (A host sum that added each row in one
-sequence rounds its row sums differently, but the difference rarely survives in the total
-of many rows: this checks that the two paths compute the same flux, not their rounding
-order.)  Float accumulation, where both paths read the same values (with FP64 accumulators
-the host copy holds floats)."""
+181 cells, filled with field after 240 steps.  A host sum that adds each row in one sequence
+gives other bits here (checked against the earlier host code).  Float accumulation, where
+both paths read the same values (with FP64 accumulators the host copy holds floats)."""
 import ctypes
 import os
 
@@ -29,11 +27,11 @@ def _paths(kind):
 
 def test_flux_same_on_host_and_device_oracle():
     dev, host = _paths("oracle")
-    assert np.all(dev != 0) and np.array_equal(dev.view(np.int64), host.view(np.int64))
+    assert np.all(np.isfinite(dev)) and np.all(dev != 0) and np.abs(dev).max() < 1e10 and np.array_equal(dev.view(np.int64), host.view(np.int64))
 
 
 @pytest.mark.gpu
 def test_flux_same_on_host_and_device_gpu():
     print("[gpu] flux with device and host accumulators", flush=True)
     dev, host = _paths("hip")
-    assert np.all(dev != 0) and np.array_equal(dev.view(np.int64), host.view(np.int64))
+    assert np.all(np.isfinite(dev)) and np.all(dev != 0) and np.abs(dev).max() < 1e10 and np.array_equal(dev.view(np.int64), host.view(np.int64))
