@@ -2479,8 +2479,12 @@ struct GatherState {
 	float *pinned = nullptr;
 	size_t pinnedBytes = 0;
 	std::vector<GatherSeg> table;   // the table on the device (valid when tabBytes > 0)
-	hipEvent_t ev = nullptr;
+	hipEvent_t ev = nullptr, done = nullptr;
 };
+
+// Gathers of at most this many bytes are written by the kernel straight into pinned host
+// memory (no copy launch); larger ones are gathered on the device and copied.
+constexpr size_t GATHER_DIRECT_BYTES = 1 << 20;
 GatherState g_gather;
 
 void gatherWorld(alis_dev_world *D, const Box &box, int mask, std::vector<GatherSeg> &seg, std::vector<GatherHost> &out, long &total)
@@ -2553,11 +2557,18 @@ int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 		G.tabBytes = segB;
 		HIPCHECK(hipMemcpyAsync(G.dev, G.table.data(), G.table.size() * sizeof(GatherSeg), hipMemcpyHostToDevice, st));
 	}
-	float *vals = (float *)((char *)G.dev + segB);
+	const bool direct = total * sizeof(float) <= GATHER_DIRECT_BYTES;
+	float *vals = direct ? G.pinned : (float *)((char *)G.dev + segB);
 	k_gather<<<blocksFor(total), 256, 0, st>>>((const GatherSeg *)G.dev, (int)seg.size(), total, vals);
 	HIPCHECK(hipGetLastError());
-	HIPCHECK(hipMemcpyAsync(G.pinned, vals, total * sizeof(float), hipMemcpyDeviceToHost, st));
-	HIPCHECK(hipStreamSynchronize(st));
+	if (!direct) HIPCHECK(hipMemcpyAsync(G.pinned, vals, total * sizeof(float), hipMemcpyDeviceToHost, st));
+	// the caller reads the values at once (get() returns one): poll the completion instead of
+	// a blocking wait, which sleeps and wakes the thread some time after the work is done
+	if (!G.done) HIPCHECK(hipEventCreateWithFlags(&G.done, hipEventDisableTiming));
+	HIPCHECK(hipEventRecord(G.done, st));
+	hipError_t q;
+	while ((q = hipEventQuery(G.done)) == hipErrorNotReady) {}
+	if (q != hipSuccess) return fail("hipEventQuery", q);
 	for (const GatherHost &g : out) {
 		const float *v = G.pinned + g.off;
 		const int nk = g.b.k1 - g.b.k0 + 1;
