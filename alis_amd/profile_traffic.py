@@ -47,9 +47,12 @@ def main():
            "calibration": {"copy_bytes": nbytes, "FETCH_SIZE_kib": cF, "WRITE_SIZE_kib": cW,
                            "fetch_scale": round(fs, 5), "write_scale": round(ws, 5)}}
     # k_curl<float, CM, ...>: CM 0 is the H launch (no 1/eps), 1 or 2 the E launch
+    # (the largest launches of each kind: the main world, not its TF/SF auxiliary world)
     for name, pats in (("E", ("k_curl<float, 1", "k_curl<float, 2")), ("H", ("k_curl<float, 0",))):
+        big = max(int(r["Grid_Size"]) for r in F if any(p in r["Kernel_Name"] for p in pats))
+
         def mine(r):
-            return any(p in r["Kernel_Name"] for p in pats) and int(r["Grid_Size"]) == cells
+            return any(p in r["Kernel_Name"] for p in pats) and int(r["Grid_Size"]) == big
         f = [float(r["Counter_Value"]) for r in F if mine(r)]
         w = [float(r["Counter_Value"]) for r in W if mine(r)]
         fa, wa = sum(f) / len(f), sum(w) / len(w)
