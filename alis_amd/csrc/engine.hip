@@ -605,6 +605,78 @@ __global__ __launch_bounds__(256) void k_phs_flush_batch(const PhsFlush<T> *__re
 	phaserFlushCell<T, TA>(F.a, t, yb * PHS_QPT, F.ring, F.ns, F.ex, F.stride);
 }
 
+// Second-order (three-wave) coupling (src/update.c:344-411): one thread per cell of the box.
+// A component of a pump field at another component's position is the mean of four values
+// (their sum in the field precision, then 0.25 times it in double); each term adds
+// 2 alt d (1/eps) fill (product) to the signal field in the order of the reference, where a
+// product of two values at the component's own position stays in the field precision and
+// any other product is double, as the reference's C expressions promote them.
+template<class T> struct ThreeWaveArgs {
+	T *F[3];                    // signal E (one part)
+	const T *A[3], *B[3];       // pump E of the two pump worlds (the same part)
+	const T *C[3];              // the signal world's 1/eps
+	const T *fill[3];           // the object's fill factors, layout LO
+	Lay L, LO;
+	Box b;                      // the cells
+	float d[3][6];
+	float alt;
+};
+
+template<class T> __device__ __forceinline__ void twAdd(T &f, T c, T g) { f = f + c * g; }
+template<class T> __device__ __forceinline__ void twAdd(T &f, T c, double g) { f = (T)((double)f + (double)c * g); }
+__device__ __forceinline__ void twAdd(double &f, double c, double g) { f = f + c * g; }
+
+template<class T>
+__global__ __launch_bounds__(256) void k_three_wave(ThreeWaveArgs<T> a)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int nk = a.b.k1 - a.b.k0 + 1, nj = a.b.j1 - a.b.j0 + 1, ni = a.b.i1 - a.b.i0 + 1;
+	if (t >= (long)ni * nj * nk) return;
+	const int k = a.b.k0 + (int)(t % nk), j = a.b.j0 + (int)((t / nk) % nj), i = a.b.i0 + (int)(t / ((long)nk * nj));
+	const Lay &L = a.L;
+	const long id = L.at(i, j, k), o = a.LO.at(i, j, k), si = L.si, sj = L.sj;
+	auto at = [&](const T *const *G, int c, long off) { return *glb(G[c] + id + off); };
+	// a pump component at the x, y or z component's position (src/update.c:344-352)
+	auto yx = [&](const T *const *G) { return 0.25 * (double)(at(G, 1, 0) + at(G, 1, -si) + at(G, 1, sj) + at(G, 1, sj - si)); };
+	auto zx = [&](const T *const *G) { return 0.25 * (double)(at(G, 2, 0) + at(G, 2, -si) + at(G, 2, 1) + at(G, 2, 1 - si)); };
+	auto zy = [&](const T *const *G) { return 0.25 * (double)(at(G, 2, 0) + at(G, 2, -sj) + at(G, 2, 1) + at(G, 2, 1 - sj)); };
+	auto xy = [&](const T *const *G) { return 0.25 * (double)(at(G, 0, 0) + at(G, 0, -sj) + at(G, 0, si) + at(G, 0, si - sj)); };
+	auto xz = [&](const T *const *G) { return 0.25 * (double)(at(G, 0, 0) + at(G, 0, -1) + at(G, 0, si) + at(G, 0, si - 1)); };
+	auto yz = [&](const T *const *G) { return 0.25 * (double)(at(G, 1, 0) + at(G, 1, -1) + at(G, 1, sj) + at(G, 1, sj - 1)); };
+	const T ax = at(a.A, 0, 0), ay = at(a.A, 1, 0), az = at(a.A, 2, 0), bx = at(a.B, 0, 0), by = at(a.B, 1, 0), bz = at(a.B, 2, 0);
+	for (int c = 0; c < 3; c++) {
+		T f = *glb(a.F[c] + id);
+		const T w = *glb(a.C[c] + id), fl = *glb(a.fill[c] + o);
+		auto coef = [&](int q) { return (T)((T)((T)((T)(2 * a.alt) * (T)a.d[c][q]) * w) * fl); };
+		const float *d = a.d[c];
+		if (c == 0) {
+			if (d[0]) twAdd(f, coef(0), ax * bx);
+			if (d[1]) twAdd(f, coef(1), yx(a.A) * yx(a.B));
+			if (d[2]) twAdd(f, coef(2), zx(a.A) * zx(a.B));
+			if (d[3]) twAdd(f, coef(3), yx(a.A) * zx(a.B)), twAdd(f, coef(3), zx(a.A) * yx(a.B));
+			if (d[4]) twAdd(f, coef(4), zx(a.A) * (double)bx), twAdd(f, coef(4), (double)ax * zx(a.B));
+			if (d[5]) twAdd(f, coef(5), (double)ax * yx(a.B)), twAdd(f, coef(5), yx(a.A) * (double)bx);
+		}
+		else if (c == 1) {
+			if (d[0]) twAdd(f, coef(0), xy(a.A) * xy(a.B));
+			if (d[1]) twAdd(f, coef(1), ay * by);
+			if (d[2]) twAdd(f, coef(2), zy(a.A) * zy(a.B));
+			if (d[3]) twAdd(f, coef(3), (double)ay * zy(a.B)), twAdd(f, coef(3), zy(a.A) * (double)by);
+			if (d[4]) twAdd(f, coef(4), zy(a.A) * xy(a.B)), twAdd(f, coef(4), xy(a.A) * zy(a.B));
+			if (d[5]) twAdd(f, coef(5), xy(a.A) * (double)by), twAdd(f, coef(5), (double)ay * xy(a.B));
+		}
+		else {
+			if (d[0]) twAdd(f, coef(0), xz(a.A) * xz(a.B));
+			if (d[1]) twAdd(f, coef(1), yz(a.A) * yz(a.B));
+			if (d[2]) twAdd(f, coef(2), az * bz);
+			if (d[3]) twAdd(f, coef(3), yz(a.A) * (double)bz), twAdd(f, coef(3), (double)az * yz(a.B));
+			if (d[4]) twAdd(f, coef(4), (double)az * xz(a.B)), twAdd(f, coef(4), xz(a.A) * (double)bz);
+			if (d[5]) twAdd(f, coef(5), xz(a.A) * yz(a.B)), twAdd(f, coef(5), yz(a.A) * xz(a.B));
+		}
+		*glb(a.F[c] + id) = f;
+	}
+}
+
 // Per-call launch arguments of batched launches from pinned host memory into device memory,
 // in stream order: a copy-engine transfer there would add a round trip between the
 // launches around it (about 25 us measured between the batched curl and post launches).
@@ -3454,6 +3526,42 @@ void alis_dev_spectrum_release(alis_dev_spectrum *S)
 		(void)hipStreamDestroy(g_specStream);
 		g_specStream = nullptr;
 	}
+}
+
+int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P2, int n, const float *d, float alt)
+{
+	if (S == P1 || S == P2) return fail("three-wave coupling: the signal world is one of the pumps");
+	for (alis_dev_world *P : {P1, P2})
+		if (P->fp64 != S->fp64 || P->cplx != S->cplx || P->iMIN != S->iMIN || P->iMAX != S->iMAX || P->jMIN != S->jMIN
+		    || P->jMAX != S->jMAX || P->kMIN != S->kMIN || P->kMAX != S->kMAX || P->L.si != S->L.si || P->L.sj != S->L.sj
+		    || P->L.koff != S->L.koff)
+			return fail("three-wave coupling: the pump and signal worlds differ in grid or precision");
+	if (n < 0 || n >= (int)S->obj.size() || !S->C[0]) return fail("three-wave coupling: no such dispersive object");
+	const DevObj &O = S->obj[n];
+	// the cells of src/update.c:357-359: the object's box without its upper plane
+	const Box b{O.box.i0, O.box.i1 - 1, O.box.j0, O.box.j1 - 1, O.box.k0, O.box.k1 - 1};
+	if (b.i0 < S->iMIN || b.i1 > S->iMAX || b.j0 < S->jMIN || b.j1 > S->jMAX || b.k0 < S->kMIN || b.k1 > S->kMAX)
+		return fail("three-wave coupling: object box outside the grid");
+	if (!b.count()) return 0;
+	// the pumps' state must be current, and their next updates must wait for the reads
+	if (joinStream(P1, S) || joinStream(P2, S)) return -1;
+	auto run = [&](auto zero) -> int {
+		typedef decltype(zero) T;
+		for (int m = 0; m <= S->cplx; m++) {
+			ThreeWaveArgs<T> a{};
+			for (int c = 0; c < 3; c++) {
+				a.F[c] = (T *)S->F[0][m][c], a.A[c] = (const T *)P1->F[0][m][c], a.B[c] = (const T *)P2->F[0][m][c];
+				a.C[c] = (const T *)S->C[c], a.fill[c] = (const T *)O.fill[c];
+			}
+			a.L = S->L, a.LO = O.LO, a.b = b;
+			memcpy(a.d, d, sizeof a.d);
+			a.alt = alt;
+			k_three_wave<T><<<blocksFor(b.count()), 256, 0, S->st>>>(a);
+			HIPCHECK(hipGetLastError());
+		}
+		return 0;
+	};
+	return S->fp64 ? run(0.0) : run(0.0f);
 }
 
 int alis_dev_synchronize(alis_dev_world *D)

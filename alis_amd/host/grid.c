@@ -378,7 +378,7 @@ world cloneWorld(world SW)
 	world W;
 	MALLOC(W, 0, 1);
 	*W = *SW;
-	W->dev = 0, W->hostStale = 0, W->parent = 0;
+	W->dev = 0, W->hostStale = 0, W->parent = 0, W->probes = 0;
 	W->E = newFieldSet(W, 0);
 	W->H = newFieldSet(W, 1);
 	visitCurrents(W, W->E, W->CE, ARR_MAKE);

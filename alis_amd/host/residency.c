@@ -275,10 +275,12 @@ void alisSyncBox(world W, int i0, int i1, int j0, int j1, int k0, int k1)
 typedef struct {
 	world W;
 	int n;
+	int next;                       /* where the next lookup starts: probes are usually read
+	                                   in the same order every step */
 	int (*box)[7];                  /* i0 i1 j0 j1 k0 k1, arrays (alis_dev_sync_boxes) */
 	unsigned long *epoch;
 } probeSet;
-static probeSet *probeSets;
+static probeSet **probeSets;     /* every world's set (W->probes points at its own) */
 static int nProbeSets, capProbeSets;
 static unsigned long epochNow = 1;
 
@@ -287,26 +289,31 @@ void alisStepped(void) { epochNow++; }
 static void forgetProbes(world W)
 {
 	for (int a = 0; a < nProbeSets; a++)
-		if (probeSets[a].W == W) {
-			free(probeSets[a].box), free(probeSets[a].epoch);
+		if (probeSets[a]->W == W) {
+			free(probeSets[a]->box), free(probeSets[a]->epoch), free(probeSets[a]);
 			probeSets[a] = probeSets[--nProbeSets];
-			return;
+			break;
 		}
+	W->probes = 0;
 }
 
 void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask)
 {
 	if (!W->dev || !W->hostStale) return;
-	probeSet *P = 0;
-	for (int a = 0; a < nProbeSets && !P; a++) if (probeSets[a].W == W) P = &probeSets[a];
+	probeSet *P = W->probes;
 	if (!P) {
 		if (nProbeSets == capProbeSets) capProbeSets = capProbeSets ? 2 * capProbeSets : 8, probeSets = realloc(probeSets, capProbeSets * sizeof *probeSets);
-		P = &probeSets[nProbeSets++];
-		*P = (probeSet) {W, 0, malloc(MAXPROBES * sizeof *P->box), malloc(MAXPROBES * sizeof *P->epoch)};
+		P = probeSets[nProbeSets++] = malloc(sizeof *P);
+		*P = (probeSet) {W, 0, 0, malloc(MAXPROBES * sizeof *P->box), malloc(MAXPROBES * sizeof *P->epoch)};
+		W->probes = P;
 	}
 	const int b[7] = {i0, i1, j0, j1, k0, k1, mask};
 	int at = -1;
-	for (int q = 0; q < P->n && at < 0; q++) if (!memcmp(P->box[q], b, sizeof b)) at = q;
+	for (int c = 0; c < P->n && at < 0; c++) {
+		const int q = (P->next + c) % P->n;
+		if (!memcmp(P->box[q], b, sizeof b)) at = q;
+	}
+	if (at >= 0) P->next = (at + 1) % P->n;
 	if (at < 0) {
 		if (P->n == MAXPROBES) {              /* too many distinct probes: copy this one alone */
 			alisSyncBox(W, i0, i1, j0, j1, k0, k1);
@@ -320,12 +327,12 @@ void alisProbe(world W, int i0, int i1, int j0, int j1, int k0, int k1, int mask
 	/* copy every stale box of every probed world that the device holds newer */
 	int m = 0;
 	for (int a = 0; a < nProbeSets; a++)
-		if (probeSets[a].W->dev && probeSets[a].W->hostStale) m += probeSets[a].n;
+		if (probeSets[a]->W->dev && probeSets[a]->W->hostStale) m += probeSets[a]->n;
 	alis_dev_world **D = malloc(m * sizeof *D);
 	int *boxes = malloc(7 * m * sizeof(int));
 	m = 0;
 	for (int a = 0; a < nProbeSets; a++) {
-		probeSet *Q = &probeSets[a];
+		probeSet *Q = probeSets[a];
 		if (!Q->W->dev || !Q->W->hostStale) continue;
 		for (int q = 0; q < Q->n; q++) {
 			if (Q->epoch[q] == epochNow) continue;

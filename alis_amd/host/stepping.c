@@ -140,49 +140,22 @@ void (copyBoundary)(float ***F, float ***G, float cosk, float sink, int iMin, in
 
 
 /*
- * Second-order (three-wave) coupling of the pump worlds' E into WS inside WS's
- * dispersive objects that carry a d tensor (src/update.c:344-411).  Runs on the host
- * arrays: the worlds are synchronised, updated and re-committed at the next step.
+ * Second-order (three-wave) coupling of the pump worlds' E into WS inside WS's dispersive
+ * objects (src/update.c:344-411), on the device (alis_dev_three_wave): the worlds are
+ * committed if they are not yet, and the d tensor of the n-th object is read from the n-th
+ * matter entry, as the reference does.  The signal world may not be one of the pumps (the
+ * reference's sequential loop would then read values it has already changed).
  */
-#define XX(G) (G.x[i][j][k])
-#define YY(G) (G.y[i][j][k])
-#define ZZ(G) (G.z[i][j][k])
-#define YX(G) (0.25 * (G.y[i][j][k] + G.y[i-1][j][k] + G.y[i][j+1][k] + G.y[i-1][j+1][k]))
-#define ZX(G) (0.25 * (G.z[i][j][k] + G.z[i-1][j][k] + G.z[i][j][k+1] + G.z[i-1][j][k+1]))
-#define ZY(G) (0.25 * (G.z[i][j][k] + G.z[i][j-1][k] + G.z[i][j][k+1] + G.z[i][j-1][k+1]))
-#define XY(G) (0.25 * (G.x[i][j][k] + G.x[i][j-1][k] + G.x[i+1][j][k] + G.x[i+1][j-1][k]))
-#define XZ(G) (0.25 * (G.x[i][j][k] + G.x[i][j][k-1] + G.x[i+1][j][k] + G.x[i+1][j][k-1]))
-#define YZ(G) (0.25 * (G.y[i][j][k] + G.y[i][j][k-1] + G.y[i][j+1][k] + G.y[i][j+1][k-1]))
-
-#define TERM(D,v,Jv,G) if (M.D) F.v[i][j][k] += 2 * Alt * M.D * C->v[i][j][k] * C->Jv[n][i][j][k] * (G)
-
-/* The d tensor is read from the n-th matter entry, as the reference does. */
-static void coupleThreeWave(coeffs *C, vfield F, vfield A, vfield B, int n, float Alt)
-{
-	matter M = C->M[n];
-	for (int i = C->iMin[n]; i <= C->iMax[n]-1; i++) for (int j = C->jMin[n]; j <= C->jMax[n]-1; j++) for (int k = C->kMin[n]; k <= C->kMax[n]-1; k++) {
-		TERM(d[0][0], x, Jx, XX(A) * XX(B)); TERM(d[0][1], x, Jx, YX(A) * YX(B)); TERM(d[0][2], x, Jx, ZX(A) * ZX(B));
-		TERM(d[0][3], x, Jx, YX(A) * ZX(B)); TERM(d[0][3], x, Jx, ZX(A) * YX(B));
-		TERM(d[0][4], x, Jx, ZX(A) * XX(B)); TERM(d[0][4], x, Jx, XX(A) * ZX(B));
-		TERM(d[0][5], x, Jx, XX(A) * YX(B)); TERM(d[0][5], x, Jx, YX(A) * XX(B));
-		TERM(d[1][0], y, Jy, XY(A) * XY(B)); TERM(d[1][1], y, Jy, YY(A) * YY(B)); TERM(d[1][2], y, Jy, ZY(A) * ZY(B));
-		TERM(d[1][3], y, Jy, YY(A) * ZY(B)); TERM(d[1][3], y, Jy, ZY(A) * YY(B));
-		TERM(d[1][4], y, Jy, ZY(A) * XY(B)); TERM(d[1][4], y, Jy, XY(A) * ZY(B));
-		TERM(d[1][5], y, Jy, XY(A) * YY(B)); TERM(d[1][5], y, Jy, YY(A) * XY(B));
-		TERM(d[2][0], z, Jz, XZ(A) * XZ(B)); TERM(d[2][1], z, Jz, YZ(A) * YZ(B)); TERM(d[2][2], z, Jz, ZZ(A) * ZZ(B));
-		TERM(d[2][3], z, Jz, YZ(A) * ZZ(B)); TERM(d[2][3], z, Jz, ZZ(A) * YZ(B));
-		TERM(d[2][4], z, Jz, ZZ(A) * XZ(B)); TERM(d[2][4], z, Jz, XZ(A) * ZZ(B));
-		TERM(d[2][5], z, Jz, XZ(A) * YZ(B)); TERM(d[2][5], z, Jz, YZ(A) * XZ(B));
-	}
-}
-
 static void threeWave(world WS, world WP1, world WP2, float Alt)
 {
-	syncWorld(WS), syncWorld(WP1), syncWorld(WP2);
+	if (WS == WP1 || WS == WP2) {
+		printf("alis: nonLinearSHG/TWM: the signal world is one of the pump worlds\n");
+		exit(1);
+	}
+	alisCommit(WS), alisCommit(WP1), alisCommit(WP2);
 	for (int n = 0; n < WS->CE->N; n++)
-		for (int m = 0; m <= WS->complexField; m++)
-			coupleThreeWave(WS->CE, WS->E[m], WP1->E[m], WP2->E[m], n, Alt);
-	alisDropDevice(WS);
+		alisFatal("three-wave coupling", alis_dev_three_wave(WS->dev, WP1->dev, WP2->dev, n, &WS->CE->M[n].d[0][0], Alt));
+	WS->hostStale = 1;
 }
 
 void nonLinearSHG(world WS, world WP, float Alt) { threeWave(WS, WP, WP, Alt); }

@@ -147,6 +147,7 @@ struct world {
 	int isAux;          /* 1 for a TF/SF auxiliary world                       */
 	world parent;       /* owning world of an auxiliary world                  */
 	int expandX, expandY; /* row aliasing of a TF/SF auxiliary world          */
+	void *probes;       /* the boxes get() reads (host/residency.c), or 0      */
 };
 
 world createWorld(dom, res, sur, char*, ...);

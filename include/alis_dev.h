@@ -258,6 +258,14 @@ int alis_dev_spectrum_add(alis_dev_spectrum *S, double tau, const double *v);
 int alis_dev_spectrum_read(alis_dev_spectrum *S, double *out);
 void alis_dev_spectrum_release(alis_dev_spectrum *S);
 
+/* Second-order (three-wave) coupling, nonLinearSHG / nonLinearTWM (src/update.c:344-411):
+ * over the cells of the n-th dispersive object of S (its box without the upper plane), for
+ * each part, the E field of S gains 2 alt d[a][b] (1/eps)(fill) times products of the pump
+ * worlds' E components (P1, P2; averaged to the component's position), d being 18 floats
+ * [3][6], with the reference's float and double promotions.  S, P1 and P2 share one grid
+ * and precision, and S is neither pump; the pumps join S's stream. */
+int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P2, int n, const float *d, float alt);
+
 /* Wait for all work of D. */
 int alis_dev_synchronize(alis_dev_world *D);
 

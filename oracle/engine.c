@@ -142,6 +142,10 @@ static void astore(world *D, arr *a, float ***h, int i0, int i1, int j0, int j1,
 
 #define EPS(D, c, i, j, k) ((D)->W.eps[c][i][j][k])
 
+/* Where a pump value of the three-wave coupling sits (ops.h): at the component's own
+ * position, or the mean of four values of another component. */
+enum { TW_OWN = -1, TW_YX, TW_ZX, TW_ZY, TW_XY, TW_XZ, TW_YZ };
+
 #define T float
 #define SUF(name) name##_f
 #define V(a, i, j, k) (((float *)(a)->p)[aix((a), (i), (j), (k))])
@@ -686,6 +690,27 @@ int alis_dev_absorption(alis_dev_world *D, int n, int i0, int i1, int j0, int j1
 	const int b[6] = {i0, i1, j0, j1, k0, k1};
 	if (D->fp64) absorption_d(D, n, b, dzE, dzH, out);
 	else absorption_f(D, n, b, dzE, dzH, out);
+	return 0;
+}
+
+/* Second-order coupling (src/update.c:344-411) between worlds of one grid; the signal's own
+ * E never feeds the coupling (it is refused as a pump). */
+int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P2, int n, const float *d, float alt)
+{
+	if (S == P1 || S == P2) return fail("three-wave coupling: the signal world is one of the pumps");
+	for (int p = 0; p < 2; p++) {
+		const alis_dev_world *P = p ? P2 : P1;
+		if (P->fp64 != S->fp64 || P->W.complexField != S->W.complexField || P->W.iMIN != S->W.iMIN || P->W.iMAX != S->W.iMAX
+		    || P->W.jMIN != S->W.jMIN || P->W.jMAX != S->W.jMAX || P->W.kMIN != S->W.kMIN || P->W.kMAX != S->W.kMAX)
+			return fail("three-wave coupling: the pump and signal worlds differ in grid or precision");
+	}
+	if (n < 0 || n >= S->W.nobj || !S->hasEps) return fail("three-wave coupling: no such dispersive object");
+	const alis_dev_ade *o = &S->obj[n];
+	const int b[6] = {o->i0, o->i1 - 1, o->j0, o->j1 - 1, o->k0, o->k1 - 1};
+	if (b[0] < S->W.iMIN || b[1] > S->W.iMAX || b[2] < S->W.jMIN || b[3] > S->W.jMAX || b[4] < S->W.kMIN || b[5] > S->W.kMAX)
+		return fail("three-wave coupling: object box outside the grid");
+	if (S->fp64) threeWave_d(S, P1, P2, n, d, alt, b);
+	else threeWave_f(S, P1, P2, n, d, alt, b);
 	return 0;
 }
 

@@ -397,3 +397,59 @@ static void SUF(absorption)(world *D, int n, const int *b, const double *dzE, co
 			for (int q = 0; q < 3; q++) r[q] = sum[q], r[3 + q] = first[q], r[6 + q] = last[q];
 		}
 }
+
+/* ---- second-order (three-wave) coupling: nonLinearSHG / nonLinearTWM (src/update.c:344-411) ----
+ * Over the object's box without its upper plane, every nonzero d[c][q] adds its terms to the
+ * signal's E component c in the reference's order.  A pump value is either at the
+ * component's own position or the mean of four values (0.25 times their sum taken in T);
+ * with T = float a product of two own-position values stays float and any product with a
+ * mean is double, as the reference's expressions promote them. */
+static double SUF(twMean)(arr *G, int kind, int i, int j, int k)
+{
+	/* component, then the four (di, dj, dk) */
+	static const int tab[6][13] = {
+		{1, 0,0,0, -1,0,0, 0,1,0, -1,1,0},  {2, 0,0,0, -1,0,0, 0,0,1, -1,0,1},
+		{2, 0,0,0, 0,-1,0, 0,0,1, 0,-1,1},  {0, 0,0,0, 0,-1,0, 1,0,0, 1,-1,0},
+		{0, 0,0,0, 0,0,-1, 1,0,0, 1,0,-1},  {1, 0,0,0, 0,0,-1, 0,1,0, 0,1,-1}};
+	const int *t = tab[kind];
+	arr *A = &G[t[0]];
+	T s = V(A, i + t[1], j + t[2], k + t[3]);
+	for (int q = 1; q < 4; q++) s = s + V(A, i + t[1 + 3 * q], j + t[2 + 3 * q], k + t[3 + 3 * q]);
+	return 0.25 * s;
+}
+
+static void SUF(threeWave)(world *S, world *P1, world *P2, int n, const float *d, float alt, const int *b)
+{
+	/* per component c: (d index, kind of the first pump's value, kind of the second's) */
+	static const int terms[3][9][3] = {
+		{{0, TW_OWN, TW_OWN}, {1, TW_YX, TW_YX}, {2, TW_ZX, TW_ZX}, {3, TW_YX, TW_ZX}, {3, TW_ZX, TW_YX},
+		 {4, TW_ZX, TW_OWN}, {4, TW_OWN, TW_ZX}, {5, TW_OWN, TW_YX}, {5, TW_YX, TW_OWN}},
+		{{0, TW_XY, TW_XY}, {1, TW_OWN, TW_OWN}, {2, TW_ZY, TW_ZY}, {3, TW_OWN, TW_ZY}, {3, TW_ZY, TW_OWN},
+		 {4, TW_ZY, TW_XY}, {4, TW_XY, TW_ZY}, {5, TW_XY, TW_OWN}, {5, TW_OWN, TW_XY}},
+		{{0, TW_XZ, TW_XZ}, {1, TW_YZ, TW_YZ}, {2, TW_OWN, TW_OWN}, {3, TW_YZ, TW_OWN}, {3, TW_OWN, TW_YZ},
+		 {4, TW_OWN, TW_XZ}, {4, TW_XZ, TW_OWN}, {5, TW_XZ, TW_YZ}, {5, TW_YZ, TW_XZ}}};
+	const alis_dev_ade *o = &S->obj[n];
+	for (int m = 0; m <= S->W.complexField; m++) {
+		arr *F = S->F[0][m], *A = P1->F[0][m], *B = P2->F[0][m];
+		for (int i = b[0]; i <= b[1]; i++)
+			for (int j = b[2]; j <= b[3]; j++)
+				for (int k = b[4]; k <= b[5]; k++)
+					for (int c = 0; c < 3; c++)
+						for (int t = 0; t < 9; t++) {
+							const int *u = terms[c][t];
+							const float dq = d[c * 6 + u[0]];
+							if (!dq) continue;
+							const T coef = (T)(2 * alt) * (T)dq * (T)EPS(S, c, i, j, k) * (T)o->fill[c][i][j][k];
+							T *f = &V(&F[c], i, j, k);
+							if (u[1] == TW_OWN && u[2] == TW_OWN) {
+								const T g = V(&A[c], i, j, k) * V(&B[c], i, j, k);
+								*f = *f + coef * g;
+							}
+							else {
+								const double ga = u[1] == TW_OWN ? (double)V(&A[c], i, j, k) : SUF(twMean)(A, u[1], i, j, k);
+								const double gb = u[2] == TW_OWN ? (double)V(&B[c], i, j, k) : SUF(twMean)(B, u[2], i, j, k);
+								*f = (T)((double)*f + (double)coef * (ga * gb));
+							}
+						}
+	}
+}

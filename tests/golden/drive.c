@@ -18,6 +18,7 @@ static phaser LASTP;
 #define MAXSCEN 64
 static world RUNW[MAXSCEN];   /* everything a run created, freed by releaseRun */
 static phaser RUNP[MAXSCEN];
+static world RUNPUMP[MAXSCEN];  /* their pump worlds */
 static char OUTDIR[512] = ".";   /* where scenarios write their spectrum files */
 static int RUNSTEPS;            /* steps of the current run */
 static const char *RUNNAME[MAXSCEN];
@@ -32,6 +33,7 @@ static void releaseRun(void)
 	for (int a = 0; a < NRUN; a++) {
 		if (RUNP[a]) deletePhaser(RUNP[a]);
 		deleteWorld(RUNW[a]);
+		if (RUNPUMP[a]) deleteWorld(RUNPUMP[a]);
 	}
 #endif
 	NRUN = 0;
@@ -253,6 +255,8 @@ typedef struct {
 	object absorbers[4];        /* dispersive objects whose absorbed power is recorded */
 	int nabs;
 	double poyntingRun;         /* poyntingOut of the observer box every 25 steps, summed */
+	world pump;                 /* a pump world coupled into W by nonLinearSHG, or 0 */
+	float alt;
 } scen;
 static void dipoleSetup(scen *c)
 {
@@ -334,6 +338,32 @@ static void particlesSetup(scen *c)
 	field *F[4] = { Ex, Ez, Hy, Ex };
 	float pts[4][3] = {{0,0,10}, {24,-24,-10}, {12,12,0}, {-30,20,25}};
 	c->W = W, c->P = P, c->np = 4;
+	memcpy(c->F, F, sizeof F);
+	memcpy(c->pts, pts, sizeof pts);
+}
+/* Second-harmonic generation: a pump world with a 900 nm Ex dipole and a signal world on the
+ * same grid, both holding a slab of a nonlinear crystal (a Drude pole and a full d tensor);
+ * every step couples the pump's E into the signal (nonLinearSHG), whose only field is what
+ * the coupling puts there. */
+static void shgSetup(scen *c)
+{
+	dom D = {{40}, {40}, {40}};
+	res R = {2, {4}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	matter Crystal = {{4.8}, {{2.0, 0.1}},
+	                  {{0.004, 0.001, 0.002, 0.0005, 0.003, -0.002}, {-0.002, 0.004, 0.001, 0.003, 0.0007, 0.001},
+	                   {0.003, 0.003, 0.008, -0.001, 0.002, 0.0015}}};
+	object Slab = {Box, {{-24, 24}, {-20, 28}, {-12, 16}}};
+	world WP = createWorld(D, R, S, "");
+	putObjects(WP, Crystal, Slab, Air);
+	pointDipole(WP, Ex, 2, -2, 4, Pulse, 900, 300);
+	world WS = createWorld(D, R, S, "");
+	putObjects(WS, Crystal, Slab, Air);
+	c->W = WS, c->pump = WP, c->alt = 1;
+	c->absorbers[c->nabs++] = Slab;
+	field *F[4] = { Ex, Ey, Ez, Hy };
+	float pts[4][3] = {{0,0,0}, {8,-4,8}, {-12,12,-4}, {20,0,8}};
+	c->np = 4;
 	memcpy(c->F, F, sizeof F);
 	memcpy(c->pts, pts, sizeof pts);
 }
@@ -504,6 +534,7 @@ static int setupScenario(scen *c, const char *name)
 	if (!strcmp(name, "hexagonal")) return hexagonalSetup(c), 0;
 	if (!strcmp(name, "multipole")) return multipoleSetup(c), 0;
 	if (!strcmp(name, "particles")) return particlesSetup(c), 0;
+	if (!strcmp(name, "shg")) return shgSetup(c), 0;
 	return fprintf(stderr, "unknown scenario %s\n", name), 2;
 }
 
@@ -601,6 +632,8 @@ static int run(const char *names, int steps)
 	world Ws[MAXSCEN];
 	for (int a = 0; a < n; a++) Ws[a] = c[a].W;
 	for (int s = 1; s <= steps; s++) {
+		for (int a = 0; a < n; a++)
+			if (c[a].pump) updateE(c[a].pump), updateH(c[a].pump);
 		if (n == 1) {
 			updateE(Ws[0]);
 			updateH(Ws[0]);
@@ -621,10 +654,12 @@ static int run(const char *names, int steps)
 			for (int a = 0; a < n; a++) updateE(Ws[a]), updateH(Ws[a]);
 #endif
 		}
+		for (int a = 0; a < n; a++)
+			if (c[a].pump) nonLinearSHG(c[a].W, c[a].pump, c[a].alt);
 		for (int a = 0; a < n; a++) stepScenario(&c[a], phasersDone);
 	}
 	for (int a = 0; a < n; a++) dumpScenario(&c[a], steps > 0);
-	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P, RUNNAME[a] = c[a].name;
+	for (int a = 0; a < n; a++) RUNW[a] = c[a].W, RUNP[a] = c[a].P, RUNPUMP[a] = c[a].pump, RUNNAME[a] = c[a].name;
 	NRUN = n;
 	LAST = c[0].W, LASTP = c[0].P;
 	return 0;

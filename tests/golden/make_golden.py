@@ -34,6 +34,8 @@ SCENARIOS = {
     "pillars": 200,
     # 18 dispersive spheres (gold and silver) lit by a TF/SF plane wave
     "particles": 200,
+    # a pump and a signal world coupled every step by nonLinearSHG (second-order d tensor)
+    "shg": 200,
     # 40 Bloch k-point worlds of a photonic-crystal slab (kp0..kp39), stepped together on the
     # GPU (80 batch entries): probes and writeSpectrum files
     "kpoints": 200,
