@@ -7,36 +7,39 @@
 #include <sys/time.h>
 #include "internal.h"
 
-/* Progress line "[elapsed/estimated] [n-1/N: %]" at most once a second; returns n <= N
- * (src/update.c:14-45).  When the loop ends, device state is copied to the host arrays so
- * that code after the loop may read them directly. */
+/* Loop predicate with a progress line (src/update.c:14-45): returns n <= N.  Step 0 (or the
+ * first call) starts the clock; afterwards, at most once per wall-clock second and after the
+ * last step, it prints "\r[str ][hh:mm:ss/hh:mm:ss] [n-1/N: p%]": the time since the start
+ * and the estimated total, the remaining (N - n + 1) steps timed at the pace since the last
+ * line (an estimate that leaves more than an hour is rounded down to whole minutes).  When
+ * the loop ends, device state is copied to the host arrays, so that code after the loop may
+ * read them directly. */
+static void clockText(unsigned long long us, char out[16])
+{
+	const unsigned long long s = us / 1000000ULL;
+	snprintf(out, 16, "%02llu:%02llu:%02llu", s / 3600, s / 60 % 60, s % 60);
+}
+
 int (timer)(long int n, long int N, char *str, ...)
 {
-	static long int initialStep, previousStep;
-	static unsigned long long int initialTime, previousTime;
-	unsigned long long int currentTime, elapsedTime, estimatedTime;
-	struct timeval now;
-	gettimeofday(&now, 0);
-	currentTime = now.tv_sec * 1000000LL + now.tv_usec;
-
-	if (!n || !initialStep) {
-		initialStep = previousStep = n;
-		initialTime = previousTime = currentTime;
+	static struct { long step; unsigned long long us; } first, last;
+	struct timeval tv;
+	gettimeofday(&tv, 0);
+	const unsigned long long now = tv.tv_sec * 1000000ULL + tv.tv_usec;
+	if (!n || !first.step) {
+		first.step = last.step = n, first.us = last.us = now;
 	}
-	else if (n > N || currentTime/1000000LL > previousTime/1000000LL) {
-		elapsedTime = currentTime - initialTime;
-		estimatedTime = elapsedTime + (N-n+1) * (currentTime-previousTime) / (n-previousStep);
-		if (estimatedTime - elapsedTime > 3600000000LL) estimatedTime -= estimatedTime%60000000LL;
-		previousStep = n;
-		previousTime = currentTime;
-		if (str[0]) printf("\r%s ", str);
-		else printf("\r");
-		printf("[%02lld:%02lld:%02lld/%02lld:%02lld:%02lld] [%ld/%ld: %.1f%%]",
-			elapsedTime/3600000000LL, elapsedTime%3600000000LL/60000000LL, elapsedTime%60000000LL/1000000LL,
-			estimatedTime/3600000000LL, estimatedTime%3600000000LL/60000000LL, estimatedTime%60000000LL/1000000LL,
-			(n-1), N, 100.0*(n-1)/N);
-		fflush(stdout);
+	else if (n > N || now / 1000000ULL > last.us / 1000000ULL) {
+		const unsigned long long elapsed = now - first.us;
+		unsigned long long total = elapsed + (N - n + 1) * (now - last.us) / (n - last.step);
+		if (total - elapsed > 3600ULL * 1000000ULL) total -= total % (60ULL * 1000000ULL);
+		last.step = n, last.us = now;
+		char e[16], t[16];
+		clockText(elapsed, e), clockText(total, t);
+		printf("\r%s%s", str, str[0] ? " " : "");
+		printf("[%s/%s] [%ld/%ld: %.1f%%]", e, t, n - 1, N, 100.0 * (n - 1) / N);
 		if (n > N) printf("\n");
+		fflush(stdout);
 	}
 	if (n > N) alisSyncAll();
 	return n <= N;
