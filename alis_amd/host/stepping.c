@@ -14,10 +14,10 @@
  * line (an estimate that leaves more than an hour is rounded down to whole minutes).  When
  * the loop ends, device state is copied to the host arrays, so that code after the loop may
  * read them directly. */
-static void clockText(unsigned long long us, char out[16])
+static void clockText(unsigned long long us, char out[32])
 {
 	const unsigned long long s = us / 1000000ULL;
-	snprintf(out, 16, "%02llu:%02llu:%02llu", s / 3600, s / 60 % 60, s % 60);
+	snprintf(out, 32, "%02llu:%02llu:%02llu", s / 3600, s / 60 % 60, s % 60);
 }
 
 int (timer)(long int n, long int N, char *str, ...)
@@ -34,7 +34,7 @@ int (timer)(long int n, long int N, char *str, ...)
 		unsigned long long total = elapsed + (N - n + 1) * (now - last.us) / (n - last.step);
 		if (total - elapsed > 3600ULL * 1000000ULL) total -= total % (60ULL * 1000000ULL);
 		last.step = n, last.us = now;
-		char e[16], t[16];
+		char e[32], t[32];
 		clockText(elapsed, e), clockText(total, t);
 		printf("\r%s%s", str, str[0] ? " " : "");
 		printf("[%s/%s] [%ld/%ld: %.1f%%]", e, t, n - 1, N, 100.0 * (n - 1) / N);
