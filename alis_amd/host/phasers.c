@@ -7,12 +7,6 @@
 #include "internal.h"
 #include "phaserview.h"
 
-#define NEWFACE(P,S,E1,E2,H1,H2,i0,i1,j0,j1,k0,k1) do { \
-	(P).S.E1 = makeComplexField(i0, i1, j0, j1, k0, k1); \
-	(P).S.E2 = makeComplexField(i0, i1, j0, j1, k0, k1); \
-	(P).S.H1 = makeComplexField(i0, i1, j0, j1, k0, k1); \
-	(P).S.H2 = makeComplexField(i0, i1, j0, j1, k0, k1); } while (0)
-
 /* Source used for spectral normalisation: the one whose duration set W->N, looked up in W
  * and then in the first TF/SF auxiliary world (src/phaser.c:25-26). */
 source normalisingSource(world W)
@@ -22,53 +16,85 @@ source normalisingSource(world W)
 	return S;
 }
 
+static cfield *faceOf(phaser P, int kind);
+
+/* The tangential components a face normal to `axis` records, as E1, E2, H1, H2
+ * (src/phaser.c:59-64): x faces Ey Ez Hy Hz, y faces Ez Ex Hz Hx, z faces Ex Ey Hx Hy. */
+static void tangential(cfield *F, int axis, float complex ****c[4])
+{
+	float complex ****e[3] = {&F->Ex, &F->Ey, &F->Ez}, ****h[3] = {&F->Hx, &F->Hy, &F->Hz};
+	const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+	c[0] = e[a1], c[1] = e[a2], c[2] = h[a1], c[3] = h[a2];
+}
+
+/* The face arrays of face `kind` (xMin xMax yMin yMax zMin zMax) over the box of Q with the
+ * face's axis fixed at its lower or upper index. */
+static void makeFace(phaser Q, int kind)
+{
+	const int axis = kind / 2;
+	int lo[3] = {Q->iMin, Q->jMin, Q->kMin}, hi[3] = {Q->iMax, Q->jMax, Q->kMax};
+	if (kind % 2) lo[axis] = hi[axis];
+	else hi[axis] = lo[axis];
+	float complex ****c[4];
+	tangential(faceOf(Q, kind), axis, c);
+	for (int q = 0; q < 4; q++) *c[q] = makeComplexField(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+}
+
+/* 1 / |sum over steps s = 1 .. N-1 of the source waveform at s dt times e^{2 pi i f s dt}|
+ * (float complex, as src/phaser.c:38-42), per unit of wavelength for line and plane
+ * sources (:43-44). */
+static float inverseSpectrum(world W, source S, float f)
+{
+	float complex sum = 0;
+	const float complex step = 2 * PI * I * f * W->dt;
+	for (int s = 1; s < W->N; s++) sum += S->waveForm(W, S, W->dt * s, S->phase) * cexpf(step * s);
+	float A = 1 / cabsf(sum);
+	if (W->sourceType == 2) A /= sqrtf(f * S->wavelength);
+	if (W->sourceType == 3) A /= f * S->wavelength;
+	return A;
+}
+
 /* Phasers at wavelengths LMin..LMax (frequencies equally spaced in 1/L by dL) on the faces
- * of a box; the array ends with an f = 0 entry.  A is the inverse source spectrum at f
- * (per unit wavelength in 3-D), so accumulated fields are normalised to a unit source
- * (src/phaser.c:21-66). */
+ * of a box; the array ends with an f = 0 entry (src/phaser.c:21-66).  A normalises the
+ * accumulated fields to a unit source; a box that spans whole periods of a periodic x or y
+ * axis records how many (xNUM, yNUM) and the Bloch phase per period.  Faces exist where the
+ * world's surface beyond them is a PML, and a lower face only when the box has depth. */
 phaser (createPhasers)(world W, float LMin, float LMax, float dL, float xMin, float xMax, float yMin, float yMax, float zMin, float zMax, ...)
 {
-	int nMin = floor(LMin/dL), nMax = ceil(LMax/dL);
+	const int nMin = floor(LMin/dL), nMax = ceil(LMax/dL), count = nMax - nMin + 1;
+	const float boxLo[3] = {xMin, yMin, zMin}, boxHi[3] = {xMax, yMax, zMax};
 	source S = normalisingSource(W);
 	phaser P;
-	MALLOC(P, 0, 2-nMin+nMax);
-
-	for (int n = 0; n <= nMax-nMin; n++) {
-		P[n].W = W;
-		P[n].N = nMin * nMax * dL / W->dt;
-		P[n].f = (nMax-n) / (dL * nMin * nMax);
-		P[n].A = 2.0 / (P->W->N ? P->W->N : P->N);
-		P[n].iMin = MAX(W->iMin, W->xtoi(W, xMin)), P[n].iMax = MIN(W->iMax, W->xtoi(W, xMax));
-		P[n].jMin = MAX(W->jMin, W->ytoj(W, yMin)), P[n].jMax = MIN(W->jMax, W->ytoj(W, yMax));
-		P[n].kMin = MAX(W->kMin, W->ztok(W, zMin)), P[n].kMax = MIN(W->kMax, W->ztok(W, zMax));
-		P[n].devId = -1;
-
-		if (S && W->N) {
-			float complex c = 0;
-			float complex iwdt = 2 * PI * I * P[n].f * W->dt;
-			for (int s = 1; s < W->N; s++) c += S->waveForm(W, S, W->dt*s, S->phase) * cexpf(iwdt*s);
-			P[n].A = 1 / cabsf(c);
-			if (W->sourceType == 2) P[n].A /= sqrtf(P[n].f * S->wavelength);
-			if (W->sourceType == 3) P[n].A /= P[n].f * S->wavelength;
+	MALLOC(P, 0, count + 1);
+	for (int n = 0; n < count; n++) {
+		phaser Q = P + n;
+		Q->W = W;
+		Q->N = nMin * nMax * dL / W->dt;
+		Q->f = (nMax-n) / (dL * nMin * nMax);
+		Q->A = 2.0 / (W->N ? W->N : P[0].N);
+		Q->devId = -1;
+		int *lo[3] = {&Q->iMin, &Q->jMin, &Q->kMin}, *hi[3] = {&Q->iMax, &Q->jMax, &Q->kMax};
+		for (int a = 0; a < 3; a++) {
+			const axis X = axisOf(W, a);
+			*lo[a] = MAX(*X.Min, (*X.node)(W, boxLo[a]));
+			*hi[a] = MIN(*X.Max, (*X.node)(W, boxHi[a]));
 		}
-		if ((xMax-xMin < INF && xMin < W->xMin && W->xMax < xMax) && (W->xMaxSur==SYM || W->xMaxSur==PBC || W->xMaxSur==BBC)) {
-			P[n].xNUM = floor((xMax - xMin) / (W->xMax - W->xMin) + 0.5);
-			P[n].expikx = (W->xMaxSur==BBC) ? (W->coskx+I*W->sinkx) : 1;
+		if (S && W->N) Q->A = inverseSpectrum(W, S, Q->f);
+		for (int a = 0; a < 2; a++) {
+			const axis X = axisOf(W, a);
+			const int periodic = *X.maxSur == SYM || *X.maxSur == PBC || *X.maxSur == BBC;
+			if (!(boxHi[a] - boxLo[a] < INF && boxLo[a] < *X.lo && *X.hi < boxHi[a] && periodic)) continue;
+			*(a ? &Q->yNUM : &Q->xNUM) = floor((boxHi[a] - boxLo[a]) / (*X.hi - *X.lo) + 0.5);
+			*(a ? &Q->expiky : &Q->expikx) = *X.maxSur == BBC ? *X.cos + I * *X.sin : 1;
 		}
-		if ((yMax-yMin < INF && yMin < W->yMin && W->yMax < yMax) && (W->yMaxSur==SYM || W->yMaxSur==PBC || W->yMaxSur==BBC)) {
-			P[n].yNUM = floor((yMax - yMin) / (W->yMax - W->yMin) + 0.5);
-			P[n].expiky = (W->yMaxSur==BBC) ? (W->cosky+I*W->sinky) : 1;
+		for (int kind = 0; kind < 6; kind++) {
+			const axis X = axisOf(W, kind / 2);
+			const int upper = kind % 2, flat = *lo[kind / 2] == *hi[kind / 2];
+			if ((upper ? *X.maxSur : *X.minSur) == PML && (upper || !flat)) makeFace(Q, kind);
 		}
-		phaser Q = &P[n];
-		if (W->xMinSur==PML && Q->iMin != Q->iMax) NEWFACE(*Q, xMin, Ey, Ez, Hy, Hz, Q->iMin, Q->iMin, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
-		if (W->yMinSur==PML && Q->jMin != Q->jMax) NEWFACE(*Q, yMin, Ez, Ex, Hz, Hx, Q->iMin, Q->iMax, Q->jMin, Q->jMin, Q->kMin, Q->kMax);
-		if (W->zMinSur==PML && Q->kMin != Q->kMax) NEWFACE(*Q, zMin, Ex, Ey, Hx, Hy, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMin);
-		if (W->xMaxSur==PML) NEWFACE(*Q, xMax, Ey, Ez, Hy, Hz, Q->iMax, Q->iMax, Q->jMin, Q->jMax, Q->kMin, Q->kMax);
-		if (W->yMaxSur==PML) NEWFACE(*Q, yMax, Ez, Ex, Hz, Hx, Q->iMin, Q->iMax, Q->jMax, Q->jMax, Q->kMin, Q->kMax);
-		if (W->zMaxSur==PML) NEWFACE(*Q, zMax, Ex, Ey, Hx, Hy, Q->iMin, Q->iMax, Q->jMin, Q->jMax, Q->kMax, Q->kMax);
 	}
-	P[0].count = nMax - nMin + 1;
-	P[nMax-nMin+1].devId = -1;
+	P[0].count = count;
+	P[count].devId = -1;
 	alisRegisterPhaser(P);
 	return P;
 }
@@ -86,10 +112,6 @@ phaser choosePhaser(phaser P, float wavelength)
 	return (1/wavelength - P->f < (P-1)->f - 1/wavelength) ? P : P-1;
 }
 
-/* Accumulate E at W->E->t and the face-averaged H at W->H->t into P and every later
- * phaser of its array (src/phaser.c:83-110).  The phase factors are computed here: in the
- * reference's float arithmetic for FP32 accumulators, in double (with the double pi) for
- * FP64 accumulators. */
 /* Make the phaser array B device-resident (its world committed, accumulators uploaded from
  * the host cfields when it is not registered yet). */
 void alisPhaserDevice(phaser B)
@@ -105,7 +127,9 @@ void alisPhaserDevice(phaser B)
 }
 
 /* Phase factors e^{2 pi i f t} of the frequencies of P (to the end of its array) at the E
- * and H times of W (src/phaser.c:97-98), as re/im pairs in eE/eH. */
+ * and H times of W (src/phaser.c:97-98), as re/im pairs in eE/eH: in the reference's float
+ * arithmetic for FP32 accumulators, in double (with the double pi) for FP64 accumulators.
+ * updatePhaser accumulates E at W->E->t and the face-averaged H at W->H->t (src/phaser.c:83-110). */
 static void phaserFactors(world W, phaser P, int nq, double *eE, double *eH)
 {
 	for (int q = 0; q < nq; q++) {
