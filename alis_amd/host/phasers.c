@@ -7,13 +7,17 @@
 #include "internal.h"
 #include "phaserview.h"
 
-/* Source used for spectral normalisation: the one whose duration set W->N, looked up in W
- * and then in the first TF/SF auxiliary world (src/phaser.c:25-26). */
+/* Source used for spectral normalisation (src/phaser.c:25-26): the E or else the H source
+ * whose duration is W's step count, looked up in W and then in W's first TF/SF auxiliary
+ * world. */
 source normalisingSource(world W)
 {
-	source S = W->SE && W->N==W->SE->N ? W->SE : W->SH && W->N==W->SH->N ? W->SH : 0;
-	for (world w = W; !S && w->S && w->S->W; S = w->SE && W->N==w->SE->N ? w->SE : w->SH && W->N==w->SH->N ? w->SH : 0) w = W->S->W;
-	return S;
+	world look[2] = {W, W->S ? W->S->W : 0};
+	for (int a = 0; a < 2 && look[a]; a++) {
+		if (look[a]->SE && look[a]->SE->N == W->N) return look[a]->SE;
+		if (look[a]->SH && look[a]->SH->N == W->N) return look[a]->SH;
+	}
+	return 0;
 }
 
 static cfield *faceOf(phaser P, int kind);
