@@ -896,10 +896,12 @@ constexpr long FLUX_SCRATCH_DOUBLES = 1L << 19;
 template<class TA>
 __global__ __launch_bounds__(256) void k_flux_sums(const typename Pair<TA>::V *__restrict__ acc, long ncell,
                                                    const int *__restrict__ terms, int nterm,
-                                                   const double *__restrict__ w, double *__restrict__ out)
+                                                   const double *__restrict__ w, double *__restrict__ out, long nout)
 {
 	typedef typename Pair<TA>::V V;
 	const int b = (int)blockIdx.x;
+	// blockIdx.y: the frequency of a range launch (its accumulators and its outputs)
+	acc += (long)blockIdx.y * 4 * ncell, out += (long)blockIdx.y * nout;
 	int t = 0;
 	while (t + 1 < nterm && b >= terms[FLUX_TERM * (t + 1) + 8]) t++;
 	const int *T = terms + FLUX_TERM * t;
@@ -2784,17 +2786,19 @@ namespace {
 // A checked flux-sums request: the kernel's term table and the doubles of device memory it
 // needs (outputs, weights, then the table).
 struct FluxPlan {
-	int id = 0, q = 0, nterm = 0;
+	int id = 0, q = 0, nq = 1, nterm = 0;
 	long nw = 0, nout = 0, blocks = 0, doubles = 0;
 	std::vector<int> t;
 };
 
-int fluxPlan(alis_dev_world *D, int id, int q, int nterm, const int *terms, long nw, long nout, FluxPlan &F)
+// Frequencies q .. q+nq-1, each with nout outputs.
+int fluxPlan(alis_dev_world *D, int id, int q, int nq, int nterm, const int *terms, long nw, long nout, FluxPlan &F)
 {
 	if (alis_dev_phaser_size(D, id) < 0) return fail("no such phaser");
 	const DevPhaser &P = D->phs[id];
-	if (q < 0 || q >= P.nf || nterm < 1 || nterm > 12 || nw < 1 || nout < 1) return fail("flux sums: bad frequency, term count or sizes");
-	F.id = id, F.q = q, F.nterm = nterm, F.nw = nw, F.nout = nout;
+	if (q < 0 || nq < 1 || q + nq > P.nf || nterm < 1 || nterm > 12 || nw < 1 || nout < 1)
+		return fail("flux sums: bad frequency range, term count or sizes");
+	F.id = id, F.q = q, F.nq = nq, F.nterm = nterm, F.nw = nw, F.nout = nout;
 	F.t.assign((size_t)FLUX_TERM * nterm, 0);
 	F.blocks = 0;
 	for (int n = 0; n < nterm; n++) {
@@ -2807,7 +2811,7 @@ int fluxPlan(alis_dev_world *D, int id, int q, int nterm, const int *terms, long
 		F.t[(size_t)FLUX_TERM * n + 8] = (int)F.blocks;
 		F.blocks += u[0] == 0 ? (na + 3) / 4 : (nb + 255) / 256;
 	}
-	F.doubles = nout + nw + ((long)F.t.size() + 1) / 2;
+	F.doubles = nq * nout + nw + ((long)F.t.size() + 1) / 2;
 	return 0;
 }
 
@@ -2817,17 +2821,19 @@ int fluxRun(alis_dev_world *D, const FluxPlan &F, const double *w, double *out, 
 {
 	DevPhaser &P = D->phs[F.id];
 	if (phaserFlushAny(D, P)) return -1;
-	double *dOut = dev, *dW = dOut + F.nout;
+	const long nall = (long)F.nq * F.nout;
+	double *dOut = dev, *dW = dOut + nall;
 	int *dT = (int *)(dW + F.nw);
-	k_fill<double><<<blocksFor(F.nout), 256, 0, D->st>>>(dOut, F.nout, 0.0);
+	k_fill<double><<<blocksFor(nall), 256, 0, D->st>>>(dOut, nall, 0.0);
 	HIPCHECK(hipGetLastError());
 	HIPCHECK(hipMemcpyAsync(dW, w, F.nw * sizeof(double), hipMemcpyHostToDevice, D->st));
 	HIPCHECK(hipMemcpyAsync(dT, F.t.data(), F.t.size() * sizeof(int), hipMemcpyHostToDevice, D->st));
 	const long base = (long)F.q * 4 * P.ncell;
-	if (D->dft64) k_flux_sums<double><<<(unsigned)F.blocks, 256, 0, D->st>>>((const Pair<double>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut);
-	else k_flux_sums<float><<<(unsigned)F.blocks, 256, 0, D->st>>>((const Pair<float>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut);
+	const dim3 grid((unsigned)F.blocks, (unsigned)F.nq);
+	if (D->dft64) k_flux_sums<double><<<grid, 256, 0, D->st>>>((const Pair<double>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut, F.nout);
+	else k_flux_sums<float><<<grid, 256, 0, D->st>>>((const Pair<float>::V *)P.acc + base, P.ncell, dT, F.nterm, dW, dOut, F.nout);
 	HIPCHECK(hipGetLastError());
-	HIPCHECK(hipMemcpyAsync(out, dOut, F.nout * sizeof(double), hipMemcpyDeviceToHost, D->st));
+	HIPCHECK(hipMemcpyAsync(out, dOut, nall * sizeof(double), hipMemcpyDeviceToHost, D->st));
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
 }
@@ -2863,7 +2869,16 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
                               const double *w, long nw, double *out, long nout)
 {
 	FluxPlan F;
-	if (fluxPlan(D, id, q, nterm, terms, nw, nout, F)) return -1;
+	if (fluxPlan(D, id, q, 1, nterm, terms, nw, nout, F)) return -1;
+	if (ensureScratch(D, F.doubles)) return -1;
+	return fluxRun(D, F, w, out, (double *)D->scratch);
+}
+
+int alis_dev_phaser_flux_sums_range(alis_dev_world *D, int id, int q0, int nq, int nterm, const int *terms,
+                                    const double *w, long nw, double *out, long nout)
+{
+	FluxPlan F;
+	if (fluxPlan(D, id, q0, nq, nterm, terms, nw, nout, F)) return -1;
 	if (ensureScratch(D, F.doubles)) return -1;
 	return fluxRun(D, F, w, out, (double *)D->scratch);
 }
@@ -2872,7 +2887,7 @@ int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nter
                                       const double *w, long nw, double *out, long nout)
 {
 	FluxPlan F;
-	if (fluxPlan(D, id, q, nterm, terms, nw, nout, F)) return -1;
+	if (fluxPlan(D, id, q, 1, nterm, terms, nw, nout, F)) return -1;
 	if (F.doubles > FLUX_SCRATCH_DOUBLES)
 		return fail("flux sums without allocation: " + std::to_string(F.doubles * 8) + " bytes of outputs, weights and terms exceed the " +
 		            std::to_string(FLUX_SCRATCH_DOUBLES * 8) + "-byte scratch array");

@@ -127,6 +127,7 @@ void alisPhaserDevice(phaser B)
 		int rc = alis_dev_phaser_add(B->W->dev, &d, &B->devId);
 		free(d.cf);
 		alisFatal("phaser registration", rc);
+		B->faceFluxValid = 0;
 	}
 }
 
@@ -170,7 +171,7 @@ void updatePhaser(world W, phaser P)
 	phaserFactors(W, P, nq, eE, eH);
 	alisFatal("phaser update", alis_dev_phaser_update(W->dev, B->devId, q0, nq, eE, eH));
 	B->hostStale = 1;
-	B->viewValid = 0;
+	B->viewValid = B->faceFluxValid = 0;
 }
 
 /* updatePhaser(W[a], P[a]) for a = 0..n-1 in one launch (the phasers' worlds share their
@@ -187,7 +188,7 @@ void updatePhaserArrays(world *W, phaser *P, int n)
 		eE[a] = malloc(4 * (nq[a] ? nq[a] : 1) * sizeof(double)), eH[a] = eE[a] + 2 * nq[a];
 		phaserFactors(W[a], P[a], nq[a], eE[a], eH[a]);
 		B->hostStale = 1;
-		B->viewValid = 0;
+		B->viewValid = B->faceFluxValid = 0;
 	}
 	alisFatal("phaser update", alis_dev_phaser_update_many(D, id, q0, nq, (const double *const *)eE, (const double *const *)eH, n));
 	for (int a = 0; a < n; a++) free(eE[a]);
@@ -199,7 +200,7 @@ void deletePhaser(phaser P)
 	phaser B = P;
 	if (B->devId >= 0 && B->W->dev) alis_dev_phaser_remove(B->W->dev, B->devId);
 	alisForgetPhaser(B);
-	free(B->view);
+	free(B->view), free(B->faceFlux);
 	for (; P->f; P++) {
 		cfield *F[6] = { &P->xMin, &P->xMax, &P->yMin, &P->yMax, &P->zMin, &P->zMax };
 		for (int a = 0; a < 6; a++) {
@@ -338,12 +339,12 @@ static void fluxSumsHost(const double *v, const phaserGeom *g, int q, int terms[
 	}
 }
 
-/* The six faces' fluxes of frequency q of P (src/phaser.c:142-223): trapezoid sums over each
- * face's E nodes weighted by the E and H cell widths, as one row term (inner axis = the
- * face's fast axis) and one column term per face.  The inner sums come from the engine,
- * which reads that frequency's accumulators once and returns about 64 + 1 doubles per face
- * row, or, with host accumulators v, from the same sums on the host: either way the same
- * bits. */
+/* The six faces' fluxes (src/phaser.c:142-223): trapezoid sums over each face's E nodes
+ * weighted by the E and H cell widths, as one row term (inner axis = the face's fast axis)
+ * and one column term per face.  With host accumulators v: frequency q of P, the inner sums
+ * computed here.  Otherwise every frequency of the array, face[6 n + kind] for frequency n,
+ * the inner sums from one engine launch, which reads the accumulators once and returns
+ * about 64 + 1 doubles per face row and frequency.  Either way the same bits. */
 static void faceFluxes(phaser P, phaser B, const phaserGeom *g, int q, const double *v, double *face)
 {
 	world W = P->W;
@@ -368,12 +369,16 @@ static void faceFluxes(phaser P, phaser B, const phaserGeom *g, int q, const dou
 		r[0] = 0, r[1] = off, r[2] = na, r[3] = nb, r[4] = rx, r[5] = ry, r[6] = rw, r[7] = (int)rowOff[kind];
 		t[0] = 1, t[1] = off, t[2] = na, t[3] = nb, t[4] = rx ? 0 : 1, t[5] = rx ? 3 : 2, t[6] = cw, t[7] = (int)colOff[kind];
 	}
+	const int nq = v ? 1 : B->count;
+	for (int n = 0; n < nq; n++)
+		for (int kind = 0; kind < 6; kind++) face[6 * n + kind] = 0;
 	if (!nterm) { free(w); return; }
-	double *out = malloc((size_t)nout * sizeof(double));
+	double *out = malloc((size_t)nq * nout * sizeof(double));
 	if (v) fluxSumsHost(v, g, q, terms, nterm, w, out, nout);
-	else alisFatal("phaser flux", alis_dev_phaser_flux_sums(W->dev, B->devId, q, nterm, &terms[0][0], w, ni + nj + nk, out, nout));
-	for (int kind = 0; kind < 6; kind++)
-		if (g->on[kind]) face[kind] = faceFluxSums(P, kind, out + rowOff[kind], out + colOff[kind]);
+	else alisFatal("phaser flux", alis_dev_phaser_flux_sums_range(W->dev, B->devId, 0, nq, nterm, &terms[0][0], w, ni + nj + nk, out, nout));
+	for (int n = 0; n < nq; n++)
+		for (int kind = 0; kind < 6; kind++)
+			if (g->on[kind]) face[6 * n + kind] = faceFluxSums(P, kind, out + (long)n * nout + rowOff[kind], out + (long)n * nout + colOff[kind]);
 	free(out);
 	free(w);
 }
@@ -389,8 +394,14 @@ double phaserFluxD(phaser P)
 	world W = P->W;
 	double face[6];
 	if (B->devId >= 0 && W->dev) {
-		phaserFaces(B, &g);
-		faceFluxes(P, B, &g, q, 0, face);
+		/* every frequency of the array at once, kept until the accumulators change */
+		if (!B->faceFluxValid) {
+			phaserFaces(B, &g);
+			if (!B->faceFlux) B->faceFlux = malloc((size_t)6 * B->count * sizeof(double));
+			faceFluxes(P, B, &g, q, 0, B->faceFlux);
+			B->faceFluxValid = 1;
+		}
+		memcpy(face, B->faceFlux + 6 * q, sizeof face);
 	}
 	else {
 		const double *v = phaserDoubles(B, &g);

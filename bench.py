@@ -130,10 +130,13 @@ def run(alis, w, hip, name, cells, fbits, dbits, warmup, steps, farfield=None, f
            "hbm_bytes": alis.alis_dev_bytes(dev),
            "worlds": nw, "parts": 1 + cplx.value}
     if flux:
-        # net flux of every frequency (phaserPoyntingOut): one k_flux_sums launch per frequency
-        # reads that frequency's face accumulators; the host adds the returned row sums
+        # net flux of every frequency (phaserPoyntingOut) after one more step: the first call
+        # adds the pending phaser samples and sums every frequency's faces in one k_flux_sums
+        # launch; the host keeps the face fluxes until the next update, so the other calls
+        # only add them up
         vals = (ctypes.c_double * 64)()
-        w.wl_flux(ctx, vals, 1)                          # flushes the pending phaser samples
+        w.wl_flux(ctx, vals, 1)                          # sizes the device workspace
+        w.wl_step(ctx, 1)
         hip.hipDeviceSynchronize()
         t0 = time.perf_counter()
         n = w.wl_flux(ctx, vals, 64)

@@ -283,6 +283,8 @@ struct phaser {
 	int hostStale;      /* cfield host copies older than the device accumulators */
 	double *view;       /* double copy of the accumulators (see phasers.c), or 0  */
 	int viewValid;      /* view matches the accumulators                          */
+	double *faceFlux;   /* [frequency][face] net flux terms (see phasers.c), or 0  */
+	int faceFluxValid;  /* faceFlux matches the accumulators                      */
 };
 
 phaser createPhaser(world, float, float, float, float, float, float, float, ...);

@@ -166,6 +166,10 @@ int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n);
  * buffers. */
 int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
                               const double *w, long nw, double *out, long nout);
+/* The same sums for the nq frequencies q0 .. q0+nq-1 of the array in one launch: out holds
+ * nq blocks of nout doubles, block q - q0 the outputs of frequency q. */
+int alis_dev_phaser_flux_sums_range(alis_dev_world *D, int id, int q0, int nq, int nterm, const int *terms,
+                                    const double *w, long nw, double *out, long nout);
 /* The same sums, allocating nothing: no counterpart in the reference; for callers that
  * cannot allocate device memory (code under graph capture, tight serving loops).  Outputs,
  * weights and the term table live in a file-scope device array of 4 MiB per GPU (held until

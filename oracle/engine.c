@@ -523,6 +523,17 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
 
 /* The allocation-free form of the device entry: the same sums, refused beyond the size of
  * the device's scratch array (outputs, weights and term table in 524288 doubles). */
+int alis_dev_phaser_flux_sums_range(alis_dev_world *D, int id, int q0, int nq, int nterm, const int *terms,
+                                    const double *w, long nw, double *out, long nout)
+{
+	phs *Q = getPhaser(D, id);
+	if (!Q) return fail("no such phaser");
+	if (q0 < 0 || nq < 1 || q0 + nq > Q->nf) return fail("flux sums: bad frequency range, term count or sizes");
+	for (int q = q0; q < q0 + nq; q++)
+		if (alis_dev_phaser_flux_sums(D, id, q, nterm, terms, w, nw, out + (long)(q - q0) * nout, nout)) return -1;
+	return 0;
+}
+
 int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nterm, const int *terms,
                                       const double *w, long nw, double *out, long nout)
 {
