@@ -57,6 +57,12 @@ struct Lay {
 	__host__ __device__ long at(int i, int j, int k) const { return (long)(i - i0) * si + (long)(j - j0) * sj + (k - k0) + koff; }
 };
 
+// Lay::at for a layout read in place from a table in global memory.
+template<class L> __device__ inline long layAt(L &lay, int i, int j, int k)
+{
+	return (long)(i - lay.i0) * lay.si + (long)(j - lay.j0) * lay.sj + (k - lay.k0) + lay.koff;
+}
+
 struct Box {
 	int i0, i1, j0, j1, k0, k1;
 	long count() const { return (i1 < i0 || j1 < j0 || k1 < k0) ? 0 : (long)(i1-i0+1)*(j1-j0+1)*(k1-k0+1); }
@@ -73,6 +79,8 @@ template<> struct Quad<double> { typedef double V __attribute__((vector_size(32)
 
 template<class T> struct CurlFace {
 	T *pa, *pb;                 // the two auxiliary components of this face
+	T *qa, *qb;                 // where their new values go (the same arrays but in k_step
+	                            // and k_fix, which write the world's other set of arrays)
 	const T *bc;                // recursion coefficients (b, ca, cb, 0) along the face axis
 	                            // (index - MIN), four to an index for one vector load
 	Lay L;
@@ -94,6 +102,7 @@ template<class T> struct ObjDev {
 
 template<class T> struct CurlArgs {
 	T *F0, *F1, *F2;
+	T *O0, *O1, *O2;            // where the new values go (F* but in k_step and k_fix)
 	const T *G0, *G1, *G2;
 	const T *C0, *C1, *C2;
 	const typename Quad<T>::V *seg;  // k_curl E: per 64-cell row segment (tile), 1/eps
@@ -105,6 +114,8 @@ template<class T> struct CurlArgs {
 	CurlFace<T> face[6];        // xMin yMin zMin xMax yMax zMax
 	const ObjDev<T> *obj;       // dispersive objects (E: scaling and -sum J before the curl;
 	int nobj, m;                //  H: current update on the new E), part m
+	const ObjDev<T> *objOut;    // the table whose currents receive the new values (obj but
+	                            //  in k_step and k_fix)
 	const int *otStart;         // per 64 x 4 tile (k_curl's tiles): its objects are
 	const int *otList;          //  otList[otStart[t] .. otStart[t+1]) ascending; null: all
 	int flatPitch;              // batched launches of packed short rows: row length with ghosts
@@ -170,7 +181,7 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 template<class T, int CM, bool NT, bool NTG, bool MZ>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
-	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ;
+	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true;
 #include "curl_cell.inc"
 }
 
@@ -189,7 +200,7 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
-	constexpr bool SHARE = false, NTP = NT, MERGEZ = false;
+	constexpr bool SHARE = false, NTP = NT, MERGEZ = false, UNI = true, SEP = false, wr = true;
 #include "curl_cell.inc"
 }
 
@@ -259,6 +270,186 @@ __device__ inline void boxCell(long t, int ni, int nj, int nk, int &di, int &dj,
 	dj = (int)(r % nj);
 	di = (int)(r / nj);
 	(void)ni;
+}
+
+// One time step of a world in one pass (the curl halves of src/update.c:275-341): the E
+// update and then the H update on the new E, read from the world's current set of arrays
+// (fields, CPML psi, dispersive currents) and written to its other set, so that no
+// workgroup overwrites a value another one still reads.  A workgroup sweeps a chunk of x
+// planes for R rows (each wave one row, all of its k cells, NC per lane): per plane it
+// updates E, passes the new values through LDS, then updates H from them, so each field
+// element comes from HBM once and goes back once per step (the separate half steps read
+// the other field a second time).  The E values H needs beyond the workgroup's cells -- row
+// j0 - 1 (wave 0) and plane i0 - 1 (the sweep's first iteration) -- are computed again and
+// not stored; outside the update range the arrays' values are read (cells no step writes).
+// The end-of-E-half program (strips, sources, ghosts) runs afterwards on the new set, and
+// k_fix recomputes H on the cells whose E inputs it changed.
+// Workgroups are dealt to the 8 XCDs by chunk: all row groups of a chunk run on one XCD,
+// side by side, so the rows one group reads of its neighbour's come from that XCD's L2.
+template<class T> struct StepArgs {
+	CurlArgs<T> e, h;           // the two halves (alt +1 and -1): F, G, psi and currents of
+	                            // the current set; O*, qa/qb and objOut of the other
+};
+
+// sa: the halves' arguments (device memory, read as needed rather than held in registers
+// across the sweep); R rows per group; wave (c, w) of the workgroup (threadIdx.y, .z) owns
+// the 64 cells from kMIN + 64 c of row j0 - 1 + w, so every wave has one cell per lane per
+// half step and plane, as in k_curl; P = 64 * ceil(nk / 64) + 1 values per LDS row and
+// component, whose slot 0 holds k = kMIN - 1.  LDS: two planes (this one and the one
+// before, alternating) of R + 1 rows of the three E components, so one barrier per plane
+// separates the E writes from the H reads.  The values a plane's E update streams from the
+// current set -- E and H at the cell, H one cell up along each axis -- are loaded into
+// registers a plane ahead (H at the cell two planes ahead: it is the next plane's i + 1
+// neighbour first), so their latency passes while the plane before is updated.
+template<class T, int CME>
+__global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 4) void k_step(const StepArgs<T> *__restrict__ sa, int R, int P, int chunk, int nchunk, int ngroup)
+{
+	extern __shared__ __align__(16) unsigned char stepLds[];
+	const int lane = (int)threadIdx.x, c = (int)threadIdx.y, w = (int)threadIdx.z;
+	const int b = (int)blockIdx.x, slot = b >> 3;
+	const int g = slot % ngroup, q = (b & 7) + 8 * (slot / ngroup);
+	if (q >= nchunk) return;
+	const int iMIN = sa->e.iMIN, jMIN = sa->e.jMIN, kMIN = sa->e.kMIN, nj = sa->e.nj, nk = sa->e.nk;
+	const int j = jMIN + g * R - 1 + w, k = kMIN + c * 64 + lane, x = 1 + c * 64 + lane;
+	const int ib = iMIN + q * chunk, ie = min(ib + chunk, iMIN + sa->e.ni) - 1;
+	const bool rowIn = j >= jMIN && j < jMIN + nj, live = j < jMIN + nj && k < kMIN + nk;
+	const int plane = (R + 1) * 3 * P;
+	const Lay L = sa->e.L;
+	const T *const E0 = sa->e.F0, *const E1 = sa->e.F1, *const E2 = sa->e.F2;
+	const T *const H0 = sa->e.G0, *const H1 = sa->e.G1, *const H2 = sa->e.G2;
+	// pipeline: e (E at plane i), h (H at i), u (H at i + 1), n (H x, z at j + 1 and x, y at
+	// k + 1, plane i), s (E at k = kMIN - 1, plane i; lane x == 1); the next plane's in p*
+	T e0 = 0, e1 = 0, e2 = 0, h0 = 0, h1 = 0, h2 = 0, u0 = 0, u1 = 0, u2 = 0;
+	T n0 = 0, n1 = 0, n2 = 0, n3 = 0, s0 = 0, s1 = 0, s2 = 0;
+	auto loadE = [&](int ii, T &a0, T &a1, T &a2) {
+		const long id = L.at(ii, j, k);
+		a0 = ldS<true>(E0 + id), a1 = ldS<true>(E1 + id), a2 = ldS<true>(E2 + id);
+	};
+	auto loadH = [&](int ii, T &a0, T &a1, T &a2) {
+		const long id = L.at(ii, j, k);
+		a0 = *glb(H0 + id), a1 = *glb(H1 + id), a2 = *glb(H2 + id);
+	};
+	auto loadN = [&](int ii, T &a0, T &a1, T &a2, T &a3) {
+		const long id = L.at(ii, j, k);
+		a0 = *glb(H0 + id + L.sj), a1 = *glb(H2 + id + L.sj), a2 = *glb(H0 + id + 1), a3 = *glb(H1 + id + 1);
+	};
+	auto loadS = [&](int ii, T &a0, T &a1, T &a2) {
+		const long id = L.at(ii, j, kMIN - 1);
+		a0 = *glb(E0 + id), a1 = *glb(E1 + id), a2 = *glb(E2 + id);
+	};
+	if (live) {
+		loadE(ib - 1, e0, e1, e2);
+		if (x == 1) loadS(ib - 1, s0, s1, s2);
+		if (rowIn) {
+			loadH(ib - 1, h0, h1, h2), loadH(ib, u0, u1, u2), loadN(ib - 1, n0, n1, n2, n3);
+		}
+	}
+	for (int i = ib - 1; i <= ie; i++) {
+		T pe0 = 0, pe1 = 0, pe2 = 0, pu0 = 0, pu1 = 0, pu2 = 0, pn0 = 0, pn1 = 0, pn2 = 0, pn3 = 0, ps0 = 0, ps1 = 0, ps2 = 0;
+		if (live && i < ie) {
+			loadE(i + 1, pe0, pe1, pe2);
+			if (x == 1) loadS(i + 1, ps0, ps1, ps2);
+			if (rowIn) {
+				loadN(i + 1, pn0, pn1, pn2, pn3);
+				if (i + 2 <= ie + 1) loadH(i + 2, pu0, pu1, pu2);
+			}
+		}
+		T *const mine = (T *)stepLds + (i & 1) * plane + w * 3 * P;
+		const bool own = w > 0 && rowIn && i >= ib;
+		if (live) {
+			T ex = e0, ey = e1, ez = e2;
+			if (rowIn && i >= iMIN) {
+				const T given0 = e0, given1 = e1, given2 = e2;
+				const T gx = h0, gy = h1, gz = h2;
+				const T g0j = n0, g2j = n1, g0k = n2, g1k = n3, g1i = u1, g2i = u2;
+				const CurlArgs<T> &a = sa->e;
+				const bool wr = own;
+				constexpr int CM = CME;
+				constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true;
+#define CURL_GIVEN
+#define CURL_GIVEN_F
+#include "curl_cell.inc"
+#undef CURL_GIVEN_F
+#undef CURL_GIVEN
+				ex = fx, ey = fy, ez = fz;
+			}
+			mine[x] = ex, mine[P + x] = ey, mine[2 * P + x] = ez;
+			if (x == 1) mine[0] = s0, mine[P] = s1, mine[2 * P] = s2;
+		}
+		__syncthreads();
+		if (own && live) {
+			const T *const below = mine - 3 * P;
+			const T *const before = (T *)stepLds + ((i + 1) & 1) * plane + w * 3 * P;   // plane i - 1
+			const T given0 = h0, given1 = h1, given2 = h2;
+			const T gx = mine[x], gy = mine[P + x], gz = mine[2 * P + x];
+			const T g0k = mine[x - 1], g1k = mine[P + x - 1];
+			const T g0j = below[x], g2j = below[2 * P + x];
+			const T g1i = before[P + x], g2i = before[2 * P + x];
+			const CurlArgs<T> &a = sa->h;
+			constexpr int CM = 0;
+			constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, wr = true;
+#define CURL_GIVEN
+#define CURL_GIVEN_F
+#include "curl_cell.inc"
+#undef CURL_GIVEN_F
+#undef CURL_GIVEN
+		}
+		e0 = pe0, e1 = pe1, e2 = pe2, h0 = u0, h1 = u1, h2 = u2, u0 = pu0, u1 = pu1, u2 = pu2;
+		n0 = pn0, n1 = pn1, n2 = pn2, n3 = pn3, s0 = ps0, s1 = ps1, s2 = ps2;
+	}
+}
+
+// Cells copied from one set of a world's arrays to the other (see the carry lists of
+// buildPost): one thread per cell of a list of boxes.
+constexpr int MAXCARRY = 16;
+template<class T> struct CarryArgs {
+	const T *src[MAXCARRY];
+	T *dst[MAXCARRY];
+	Box b[MAXCARRY];
+	long off[MAXCARRY + 1];
+	Lay L;
+	int n;
+};
+
+template<class T>
+__global__ __launch_bounds__(256) void k_carry(CarryArgs<T> a)
+{
+	const long t = (long)blockIdx.x * 256 + threadIdx.x;
+	if (t >= a.off[a.n]) return;
+	int q = 0;
+	while (q + 1 < a.n && a.off[q + 1] <= t) q++;
+	const Box &B = a.b[q];
+	int di, dj, dk;
+	boxCell(t - a.off[q], B.i1 - B.i0 + 1, B.j1 - B.j0 + 1, B.k1 - B.k0 + 1, di, dj, dk);
+	const long id = a.L.at(B.i0 + di, B.j0 + dj, B.k0 + dk);
+	a.dst[q][id] = a.src[q][id];
+}
+
+// H again on the cells whose E inputs the end-of-E-half program changed after k_step (the
+// boxes of its operations, one cell further up each axis): from the old set's H, psi and
+// currents and the new set's final E, into the new set -- the values the separate half
+// steps give.  One thread per cell of a list of boxes.
+constexpr int MAXFIX = 16;
+struct FixBoxes {
+	Box b[MAXFIX];
+	long off[MAXFIX + 1];       // first thread of each box; off[n] = all
+	int n;
+};
+
+template<class T>
+__global__ __launch_bounds__(256) void k_fix(CurlArgs<T> a, FixBoxes f)
+{
+	const long t0 = (long)blockIdx.x * 256 + threadIdx.x;
+	if (t0 >= f.off[f.n]) return;
+	int q = 0;
+	while (q + 1 < f.n && f.off[q + 1] <= t0) q++;
+	const Box &B = f.b[q];
+	int di, dj, dk;
+	boxCell(t0 - f.off[q], B.i1 - B.i0 + 1, B.j1 - B.j0 + 1, B.k1 - B.k0 + 1, di, dj, dk);
+	const int i = B.i0 + di, j = B.j0 + dj, k = B.k0 + dk;
+	constexpr int CM = 0;
+	constexpr bool NT = false, NTG = false, SHARE = true, NTP = false, MERGEZ = false, UNI = false, SEP = true, wr = true;
+#include "curl_cell.inc"
 }
 
 // The end of a half step in one launch: TF/SF strips and sources (additive operations,
@@ -948,7 +1139,9 @@ struct DevSlab {
 	int lo = 0, hi = -1;
 	Lay L{};
 	Box b{};                    // host extents of the slab array
+	long n = 0;                 // elements of each array
 	void *p[2][2] = {};         // [m][a/b]
+	void *alt[2][2] = {};       // the other set (k_step), allocated at the first fused step
 };
 
 struct DevObj {
@@ -960,6 +1153,7 @@ struct DevObj {
 	float JJ[3][MAXPOLE] = {}, JK[3][MAXPOLE] = {}, JE[3][MAXPOLE] = {}, KJ[3][MAXPOLE] = {}, KK[3][MAXPOLE] = {}, KE[3][MAXPOLE] = {};
 	void *fill[3] = {};
 	void *J[2][3][MAXPOLE] = {}, *K[2][3][MAXPOLE] = {};
+	void *Jalt[2][3][MAXPOLE] = {}, *Kalt[2][3][MAXPOLE] = {};   // the other set (k_step)
 };
 
 struct DevSrc {
@@ -1059,10 +1253,30 @@ struct alis_dev_world {
 	long bytes = 0;
 	bool timing = false;
 	hipEvent_t halfDone = nullptr;  // the end of the last postHalf (bound to its last launch)
-	std::vector<hipEvent_t> evPairs[2];
+	std::vector<hipEvent_t> evPairs[3];  // E curl, H curl, fused step (k_step)
 	std::vector<hipEvent_t> evFree;  // timing events already read, for reuse
-	double evMs[2] = {0, 0};
-	long evCount[2] = {0, 0};
+	double evMs[3] = {0, 0, 0};
+	long evCount[3] = {0, 0, 0};
+	// Fused steps (k_step): the world holds a second set of the arrays a step updates and
+	// alternates between the sets; an E half step is deferred until its H half arrives.
+	int fuseState = 0;          // 0 not decided yet, 1 fused steps allowed, -1 not
+	bool isAux = false;         // the TF/SF auxiliary world of another world
+	int inBatch = 0;            // live batches whose launch arguments name this world's arrays
+	int pendE = 0;              // 1: E curl deferred; 2: and its end-of-half-step program
+	bool altReady = false;      // the other set exists
+	bool altStale = false;      // cells no step writes may differ between the sets
+	int par = 0;                // which set is current (0: the one allocated at commit)
+	void *Falt[2][2][3] = {};
+	void *objTabAlt = nullptr;  // the object table naming the other set's currents
+	std::vector<PostSeg> postAlt[2];  // end-of-half-step programs of the other set
+	bool postBuiltAlt = false;
+	void *stepArgs = nullptr;   // StepArgs<T>[2 sets][2 parts]: k_step's arguments per current set
+	struct Carry { int axis; Box b; };
+	std::vector<Carry> carry[2];  // cells a fused step's end-of-half programs take from the other set
+	bool carryDue[2] = {false, false};
+	std::vector<Box> fix;       // boxes k_fix recomputes after the E program
+	bool fixOk = false;         // every additive operation lies in the update range and the
+	                            //  boxes fit one k_fix launch
 };
 
 namespace {
@@ -1451,6 +1665,39 @@ int objectTiles(alis_dev_world *D)
 }
 
 // Commit: allocate and upload everything of the descriptor.
+// The device table of D's dispersive objects (ObjDev<T>), naming the currents of its
+// commit-time set (alt false) or of its other set (alt true, see k_step).
+template<class T>
+int uploadObjTable(alis_dev_world *D, bool alt, void **dst)
+{
+	std::vector<ObjDev<T>> tab(D->obj.size());
+	for (size_t n = 0; n < D->obj.size(); n++) {
+		const DevObj &O = D->obj[n];
+		ObjDev<T> &t = tab[n];
+		memset(&t, 0, sizeof t);
+		t.i0 = O.clip.i0, t.i1 = O.clip.i1, t.j0 = O.clip.j0, t.j1 = O.clip.j1, t.k0 = O.clip.k0, t.k1 = O.clip.k1;
+		t.LO = O.LO;
+		t.NJ = O.NJ, t.NK = O.NK;
+		t.scale = O.a[0] || O.a[1] || O.a[2];
+		for (int c = 0; c < 3; c++) {
+			t.a[c] = (T)O.a[c];
+			t.fill[c] = (const T *)O.fill[c];
+			for (int p = 0; p < MAXPOLE; p++) {
+				t.JJ[c][p] = (T)O.JJ[c][p], t.JK[c][p] = (T)O.JK[c][p], t.JE[c][p] = (T)O.JE[c][p];
+				t.KJ[c][p] = (T)O.KJ[c][p], t.KK[c][p] = (T)O.KK[c][p], t.KE[c][p] = (T)O.KE[c][p];
+				for (int m = 0; m <= D->cplx; m++) {
+					t.J[m][c][p] = (T *)(alt ? O.Jalt : O.J)[m][c][p];
+					t.K[m][c][p] = (T *)(alt ? O.Kalt : O.K)[m][c][p];
+				}
+			}
+		}
+	}
+	if (devAlloc(D, dst, tab.size() * sizeof(ObjDev<T>))) return -1;
+	HIPCHECK(hipMemcpyAsync(*dst, tab.data(), tab.size() * sizeof(ObjDev<T>), hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
 template<class T>
 int commitWorld(alis_dev_world *D)
 {
@@ -1518,6 +1765,7 @@ int commitWorld(alis_dev_world *D)
 				S.L.si = (long)(b.j1 - b.j0 + 1) * D->L.sj;
 			}
 			const long n = (long)(b.i1 - b.i0 + 1) * S.L.si;
+			S.n = n;
 			for (int m = 0; m <= D->cplx; m++)
 				for (int s2 = 0; s2 < 2; s2++) {
 					float ***hostp = W->slab[h][m][f][s2];
@@ -1564,28 +1812,7 @@ int commitWorld(alis_dev_world *D)
 		D->obj.push_back(O);
 	}
 	if (!D->obj.empty()) {
-		std::vector<ObjDev<T>> tab(D->obj.size());
-		for (size_t n = 0; n < D->obj.size(); n++) {
-			const DevObj &O = D->obj[n];
-			ObjDev<T> &t = tab[n];
-			memset(&t, 0, sizeof t);
-			t.i0 = O.clip.i0, t.i1 = O.clip.i1, t.j0 = O.clip.j0, t.j1 = O.clip.j1, t.k0 = O.clip.k0, t.k1 = O.clip.k1;
-			t.LO = O.LO;
-			t.NJ = O.NJ, t.NK = O.NK;
-			t.scale = O.a[0] || O.a[1] || O.a[2];
-			for (int c = 0; c < 3; c++) {
-				t.a[c] = (T)O.a[c];
-				t.fill[c] = (const T *)O.fill[c];
-				for (int p = 0; p < MAXPOLE; p++) {
-					t.JJ[c][p] = (T)O.JJ[c][p], t.JK[c][p] = (T)O.JK[c][p], t.JE[c][p] = (T)O.JE[c][p];
-					t.KJ[c][p] = (T)O.KJ[c][p], t.KK[c][p] = (T)O.KK[c][p], t.KE[c][p] = (T)O.KE[c][p];
-					for (int m = 0; m <= D->cplx; m++) t.J[m][c][p] = (T *)O.J[m][c][p], t.K[m][c][p] = (T *)O.K[m][c][p];
-				}
-			}
-		}
-		if (devAlloc(D, &D->objTab, tab.size() * sizeof(ObjDev<T>))) return -1;
-		HIPCHECK(hipMemcpyAsync(D->objTab, tab.data(), tab.size() * sizeof(ObjDev<T>), hipMemcpyHostToDevice, D->st));
-		HIPCHECK(hipStreamSynchronize(D->st));
+		if (uploadObjTable<T>(D, false, &D->objTab)) return -1;
 		if (objectTiles(D)) return -1;
 	}
 
@@ -1630,6 +1857,7 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 {
 	CurlArgs<T> a{};
 	a.F0 = (T *)D->F[h][m][0], a.F1 = (T *)D->F[h][m][1], a.F2 = (T *)D->F[h][m][2];
+	a.O0 = a.F0, a.O1 = a.F1, a.O2 = a.F2;
 	a.G0 = (const T *)D->F[1-h][m][0], a.G1 = (const T *)D->F[1-h][m][1], a.G2 = (const T *)D->F[1-h][m][2];
 	if (h == 0) a.C0 = (const T *)D->C[0], a.C1 = (const T *)D->C[1], a.C2 = (const T *)D->C[2];
 	if (h == 0) a.seg = (const typename Quad<T>::V *)D->seg;
@@ -1645,10 +1873,11 @@ CurlArgs<T> curlArgs(alis_dev_world *D, int h, int m)
 		if (!S.on) continue;
 		const int ax = f % 3;
 		P.pa = (T *)S.p[m][0], P.pb = (T *)S.p[m][1];
+		P.qa = P.pa, P.qb = P.pb;
 		P.bc = (const T *)D->cpk[h][ax];
 		P.L = S.L, P.lo = S.lo, P.hi = S.hi;
 	}
-	a.obj = (const ObjDev<T> *)D->objTab;
+	a.obj = a.objOut = (const ObjDev<T> *)D->objTab;
 	a.nobj = (int)D->obj.size();
 	if (D->objTile) {
 		const long ntile = (long)D->iNUM * ((D->jNUM + 3) / 4) * ((D->kNUM + 63) / 64);
@@ -1799,6 +2028,66 @@ int stripOps(alis_dev_world *D, int s, int h, std::vector<PostOp<T>> &ops)
 	return 0;
 }
 
+// The parts of box b outside box r, as disjoint boxes.
+std::vector<Box> outsideParts(Box b, const Box &r)
+{
+	std::vector<Box> out;
+	auto take = [&](const Box &x) { if (x.count()) out.push_back(x); };
+	take(Box{b.i0, std::min(b.i1, r.i0 - 1), b.j0, b.j1, b.k0, b.k1});
+	take(Box{std::max(b.i0, r.i1 + 1), b.i1, b.j0, b.j1, b.k0, b.k1});
+	b.i0 = std::max(b.i0, r.i0), b.i1 = std::min(b.i1, r.i1);
+	if (b.i0 > b.i1) return out;
+	take(Box{b.i0, b.i1, b.j0, std::min(b.j1, r.j0 - 1), b.k0, b.k1});
+	take(Box{b.i0, b.i1, std::max(b.j0, r.j1 + 1), b.j1, b.k0, b.k1});
+	b.j0 = std::max(b.j0, r.j0), b.j1 = std::min(b.j1, r.j1);
+	if (b.j0 > b.j1) return out;
+	take(Box{b.i0, b.i1, b.j0, b.j1, b.k0, std::min(b.k1, r.k0 - 1)});
+	take(Box{b.i0, b.i1, b.j0, b.j1, std::max(b.k0, r.k1 + 1), b.k1});
+	return out;
+}
+
+// The H cells a fused step computes again after the E program (k_fix): H at a cell reads E
+// there and one cell below along each axis, so each box the program writes, grown by one
+// cell upwards and clipped to the update range.  Boxes that overlap much (the two strips of
+// one TF/SF face) are merged, and then the pairs whose union adds the fewest cells until at
+// most MAXFIX remain; computing H again on a cell the program did not reach reproduces the
+// value k_step wrote.
+std::vector<Box> fixBoxes(const std::vector<Box> &ops, const Box &range)
+{
+	std::vector<Box> v;
+	for (Box b : ops) {
+		b = Box{std::max(b.i0, range.i0), std::min(b.i1 + 1, range.i1), std::max(b.j0, range.j0), std::min(b.j1 + 1, range.j1),
+		        std::max(b.k0, range.k0), std::min(b.k1 + 1, range.k1)};
+		if (b.count()) v.push_back(b);
+	}
+	auto unite = [](const Box &a, const Box &b) {
+		return Box{std::min(a.i0, b.i0), std::max(a.i1, b.i1), std::min(a.j0, b.j0), std::max(a.j1, b.j1),
+		           std::min(a.k0, b.k0), std::max(a.k1, b.k1)};
+	};
+	for (bool merged = true; merged;) {
+		merged = false;
+		for (size_t x = 0; x < v.size() && !merged; x++)
+			for (size_t y = x + 1; y < v.size() && !merged; y++)
+				if (unite(v[x], v[y]).count() * 10 <= (v[x].count() + v[y].count()) * 11) {
+					v[x] = unite(v[x], v[y]);
+					v.erase(v.begin() + y);
+					merged = true;
+				}
+	}
+	while (v.size() > (size_t)MAXFIX) {
+		size_t bx = 0, by = 1;
+		long best = -1;
+		for (size_t x = 0; x < v.size(); x++)
+			for (size_t y = x + 1; y < v.size(); y++) {
+				const long grow = unite(v[x], v[y]).count() - v[x].count() - v[y].count();
+				if (best < 0 || grow < best) best = grow, bx = x, by = y;
+			}
+		v[bx] = unite(v[bx], v[by]);
+		v.erase(v.begin() + by);
+	}
+	return v;
+}
+
 // Build the end-of-half-step programs of D: strips of every TF/SF box, sources of the
 // chain, ghost copies; split into launches where a copy chain would read a cell that its
 // launch writes.
@@ -1808,6 +2097,10 @@ int buildPost(alis_dev_world *D)
 	if ((int)D->tf.size() > 32) return fail("more than 32 TF/SF boxes in one world");
 	std::vector<std::vector<PostOp<T>>> segOps[2];
 	std::vector<int> segAdd[2];
+	const Box range{D->iMIN, D->iMAX, D->jMIN, D->jMAX, D->kMIN, D->kMAX};
+	std::vector<Box> eOps;      // the cells the E program writes
+	std::vector<PostOp<T>> copyOps[2];
+	D->fixOk = true;
 	for (int h = 0; h < 2; h++) {
 		if ((int)D->src[h].size() > MAXSRC) return fail("more than 128 sources in one chain");
 		std::vector<PostOp<T>> adds, copies;
@@ -1842,6 +2135,16 @@ int buildPost(alis_dev_world *D)
 			v.erase(std::remove_if(v.begin(), v.end(), [](const PostOp<T> &o) { return o.b.count() == 0; }), v.end());
 		};
 		nonEmpty(adds), nonEmpty(copies);
+		// fused steps keep the two sets equal on the cells no step writes, which an additive
+		// operation outside the update range (applied on some steps only) would break
+		for (const PostOp<T> &o : adds)
+			if (o.b.i0 < range.i0 || o.b.i1 > range.i1 || o.b.j0 < range.j0 || o.b.j1 > range.j1 || o.b.k0 < range.k0 || o.b.k1 > range.k1)
+				D->fixOk = false;
+		if (h == 0) {
+			for (const PostOp<T> &o : adds) eOps.push_back(o.b);
+			for (const PostOp<T> &o : copies) eOps.push_back(o.b);
+		}
+		copyOps[h] = copies;
 		if (copies.size() <= (size_t)MAXCHAIN && adds.size() + copies.size() <= (size_t)MAXPOST && !postConflict(adds, copies)) {
 			std::vector<PostOp<T>> all = adds;
 			all.insert(all.end(), copies.begin(), copies.end());
@@ -1867,12 +2170,32 @@ int buildPost(alis_dev_world *D)
 	}
 	for (int h = 0; h < 2; h++) {
 		D->post[h].clear();
+		D->carry[h].clear();
 		for (size_t g = 0; g < segOps[h].size(); g++) {
 			const std::vector<PostOp<T>> &v = segOps[h][g];
 			std::vector<PostPiece> pieces;
 			PostSeg seg;
 			buildPieces(v, segAdd[h][g], pieces, seg.total);
 			if (pieces.empty()) continue;
+			// A copy chain starts from a cell no earlier copy of its launch writes.  Where that
+			// cell lies outside the update range and a copy of the program writes it (later:
+			// the program reads the value the previous step left there), a fused step's new
+			// set holds the value of two steps before: those cells are carried over first.
+			for (const PostPiece &P : pieces) {
+				if (P.kind != OP_COPY) continue;
+				const PostOp<T> &o = v[P.list[0]];
+				int axis = -1;
+				for (int a = 0; a < 3; a++) if (o.F == (T *)D->F[h][0][a]) axis = a;
+				const Box base{P.b.i0 + P.si, P.b.i1 + P.si, P.b.j0 + P.sj, P.b.j1 + P.sj, P.b.k0 + P.sk, P.b.k1 + P.sk};
+				for (const PostOp<T> &t : copyOps[h]) {
+					if (t.F != o.F) continue;
+					for (const Box &part : outsideParts(t.b, range)) {
+						const Box x{std::max(base.i0, part.i0), std::min(base.i1, part.i1), std::max(base.j0, part.j0),
+						            std::min(base.j1, part.j1), std::max(base.k0, part.k0), std::min(base.k1, part.k1)};
+						if (x.count() && axis >= 0) D->carry[h].push_back({axis, x});
+					}
+				}
+			}
 			seg.npieces = (int)pieces.size();
 			if (devAlloc(D, &seg.ops, v.size() * sizeof(PostOp<T>))) return -1;
 			if (devAlloc(D, &seg.pieces, pieces.size() * sizeof(PostPiece))) return -1;
@@ -1884,6 +2207,12 @@ int buildPost(alis_dev_world *D)
 	}
 	HIPCHECK(hipStreamSynchronize(D->st));
 	D->postBuilt = true;
+	D->fix = fixBoxes(eOps, range);
+	long fixCells = 0;
+	for (const Box &b : D->fix) fixCells += b.count();
+	if (D->fix.size() > (size_t)MAXFIX || fixCells * 4 > range.count()) D->fixOk = false;   // not worth it
+	for (int h = 0; h < 2; h++)
+		if (D->carry[h].size() * (1 + D->cplx) > (size_t)MAXCARRY) D->fixOk = false;
 	return 0;
 }
 
@@ -1892,6 +2221,21 @@ template<class T>
 int postHalf(alis_dev_world *D, int h)
 {
 	if (!D->postBuilt && buildPost<T>(D)) return -1;
+	if (D->carryDue[h]) {
+		D->carryDue[h] = false;
+		CarryArgs<T> c{};
+		for (const alis_dev_world::Carry &x : D->carry[h])
+			for (int m = 0; m <= D->cplx; m++) {
+				c.src[c.n] = (const T *)D->Falt[h][m][x.axis], c.dst[c.n] = (T *)D->F[h][m][x.axis];
+				c.b[c.n] = x.b, c.off[c.n + 1] = c.off[c.n] + x.b.count();
+				c.n++;
+			}
+		c.L = D->L;
+		if (c.n) {
+			k_carry<T><<<blocksFor(c.off[c.n]), 256, 0, D->st>>>(c);
+			HIPCHECK(hipGetLastError());
+		}
+	}
 	PostArgs<T> a{};
 	a.strips = D->pendStrips[h];
 	for (size_t w = 0; w < D->pendSrc[h].size(); w++) a.src[w] = D->pendSrc[h][w];
@@ -1950,6 +2294,236 @@ int injectRecord(alis_dev_world *D, int h, int index, float wave, const float *c
 	D->pendSrc[h][index >> 5] |= 1u << (index & 31);
 	D->pendWave[h][index] = wave;
 	return 0;
+}
+
+// ---- fused steps (k_step) ----
+
+// Exchange the roles of D's two sets of arrays: what k_step wrote becomes current.
+void swapSets(alis_dev_world *D)
+{
+	for (int h = 0; h < 2; h++)
+		for (int m = 0; m < 2; m++)
+			for (int a = 0; a < 3; a++) std::swap(D->F[h][m][a], D->Falt[h][m][a]);
+	for (int h = 0; h < 2; h++)
+		for (DevSlab &S : D->slab[h])
+			for (int m = 0; m < 2; m++) std::swap(S.p[m][0], S.alt[m][0]), std::swap(S.p[m][1], S.alt[m][1]);
+	for (DevObj &O : D->obj)
+		for (int m = 0; m < 2; m++)
+			for (int c = 0; c < 3; c++)
+				for (int p = 0; p < MAXPOLE; p++) std::swap(O.J[m][c][p], O.Jalt[m][c][p]), std::swap(O.K[m][c][p], O.Kalt[m][c][p]);
+	std::swap(D->objTab, D->objTabAlt);
+	for (int h = 0; h < 2; h++) std::swap(D->post[h], D->postAlt[h]);
+	std::swap(D->postBuilt, D->postBuiltAlt);
+	D->par ^= 1;
+}
+
+// The other set: allocated at the first fused step, and made equal to the current one then
+// and whenever cells outside a step's reach may have changed (alis_dev_write_field).
+template<class T>
+int ensureAlt(alis_dev_world *D)
+{
+	if (!D->altReady) {
+		for (int h = 0; h < 2; h++)
+			for (int m = 0; m <= D->cplx; m++)
+				for (int a = 0; a < 3; a++)
+					if (devAlloc(D, &D->Falt[h][m][a], D->fsize * sizeof(T))) return -1;
+		for (int h = 0; h < 2; h++)
+			for (DevSlab &S : D->slab[h])
+				for (int m = 0; S.on && m <= D->cplx; m++)
+					for (int x = 0; x < 2; x++)
+						if (devAlloc(D, &S.alt[m][x], S.n * sizeof(T))) return -1;
+		for (DevObj &O : D->obj)
+			for (int m = 0; m <= D->cplx; m++)
+				for (int c = 0; c < 3; c++)
+					for (int p = 0; p < O.NK; p++) {
+						if (devAlloc(D, &O.Jalt[m][c][p], O.box.count() * sizeof(T))) return -1;
+						if (p >= O.NJ && devAlloc(D, &O.Kalt[m][c][p], O.box.count() * sizeof(T))) return -1;
+					}
+		if (!D->obj.empty() && uploadObjTable<T>(D, true, &D->objTabAlt)) return -1;
+		D->altReady = true, D->altStale = true;
+	}
+	if (D->altStale) {
+		auto copy = [&](void *dst, const void *src, long n) {
+			return hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, D->st);
+		};
+		for (int h = 0; h < 2; h++)
+			for (int m = 0; m <= D->cplx; m++)
+				for (int a = 0; a < 3; a++) HIPCHECK(copy(D->Falt[h][m][a], D->F[h][m][a], D->fsize));
+		for (int h = 0; h < 2; h++)
+			for (DevSlab &S : D->slab[h])
+				for (int m = 0; S.on && m <= D->cplx; m++)
+					for (int x = 0; x < 2; x++) HIPCHECK(copy(S.alt[m][x], S.p[m][x], S.n));
+		for (DevObj &O : D->obj)
+			for (int m = 0; m <= D->cplx; m++)
+				for (int c = 0; c < 3; c++)
+					for (int p = 0; p < O.NK; p++) {
+						HIPCHECK(copy(O.Jalt[m][c][p], O.J[m][c][p], O.box.count()));
+						if (p >= O.NJ) HIPCHECK(copy(O.Kalt[m][c][p], O.K[m][c][p], O.box.count()));
+					}
+		D->altStale = false;
+	}
+	return 0;
+}
+
+// Launch shape of k_step for D: NC waves per row (64 cells each), R rows per group and the
+// row below them (R + 1 = 16 / NC rows, at most 9, fewer where the two planes of LDS rows
+// would pass 64 KiB), P values per LDS row and component, x planes per chunk (8;
+// ALIS_STEP_CHUNK and ALIS_STEP_ROWS override).
+struct StepShape { int NC, R, P, chunk, ngroup, nchunk; size_t lds; };
+
+StepShape stepShape(const alis_dev_world *D)
+{
+	StepShape s{};
+	s.NC = (D->kNUM + 63) / 64;
+	s.P = 64 * s.NC + 1;
+	const size_t row = 2 * 3 * (size_t)s.P * D->esz;
+	s.R = std::min({8, 16 / std::max(s.NC, 1) - 1, (int)(65536 / row) - 1});
+	if (const char *e = getenv("ALIS_STEP_ROWS")) s.R = std::min(s.R, atoi(e));
+	if (s.R < 1) s.R = 1;
+	s.lds = (s.R + 1) * row;
+	s.ngroup = (D->jNUM + s.R - 1) / s.R;
+	s.chunk = 8;
+	if (const char *e = getenv("ALIS_STEP_CHUNK")) s.chunk = std::max(1, atoi(e));
+	s.nchunk = (D->iNUM + s.chunk - 1) / s.chunk;
+	return s;
+}
+
+// Whether D's steps run fused: decided once per world (again after a TF/SF link).  Off
+// unless ALIS_FUSE is 1 (worlds that give k_step two workgroups per CU and at least 3 rows
+// per group) or force (every world whose results it can reproduce): on the 256^3 nanoblock
+// k_step takes 560 us against 286 us for the two separate curl launches.
+template<class T>
+int fusedSteps(alis_dev_world *D, bool &on)
+{
+	on = false;
+	if (D->isAux || D->inBatch) return 0;
+	if (!D->fuseState) {
+		D->fuseState = -1;
+		const char *env = getenv("ALIS_FUSE");
+		for (const DevTfsf &t : D->tf)
+			if (!t.aux) return 0;           // its programs cannot be built yet
+		if (!D->postBuilt && buildPost<T>(D)) return -1;
+		const StepShape s = stepShape(D);
+		const bool force = env && !strcmp(env, "force");   // every world where the results allow it
+		const bool on = force || (env && !strcmp(env, "1"));
+		if (on && D->fixOk && s.NC <= 8 && (force || (s.R >= 3 && (long)s.ngroup * s.nchunk >= 512)))
+			D->fuseState = 1;
+	}
+	on = D->fuseState > 0;
+	return 0;
+}
+
+// Point the results of a half's arguments (built on the current set) at the other set.
+template<class T>
+void toOtherSet(alis_dev_world *D, int h, int m, CurlArgs<T> &a)
+{
+	a.O0 = (T *)D->Falt[h][m][0], a.O1 = (T *)D->Falt[h][m][1], a.O2 = (T *)D->Falt[h][m][2];
+	for (int f = 0; f < 6; f++)
+		if (a.face[f].on) a.face[f].qa = (T *)D->slab[h][f].alt[m][0], a.face[f].qb = (T *)D->slab[h][f].alt[m][1];
+	a.objOut = (const ObjDev<T> *)D->objTabAlt;
+}
+
+// k_step's argument table: for each set as the current one, each part.
+template<class T>
+int stepTable(alis_dev_world *D)
+{
+	StepArgs<T> t[2][2] = {};
+	for (int pass = 0; pass < 2; pass++) {
+		for (int m = 0; m <= D->cplx; m++) {
+			StepArgs<T> &s = t[D->par][m];
+			s.e = curlArgs<T>(D, 0, m), s.h = curlArgs<T>(D, 1, m);
+			toOtherSet<T>(D, 0, m, s.e), toOtherSet<T>(D, 1, m, s.h);
+		}
+		swapSets(D);
+	}
+	if (devAlloc(D, &D->stepArgs, sizeof t)) return -1;
+	HIPCHECK(hipMemcpyAsync(D->stepArgs, t, sizeof t, hipMemcpyHostToDevice, D->st));
+	HIPCHECK(hipStreamSynchronize(D->st));
+	return 0;
+}
+
+// A whole step of D after its deferred E half: k_step from the current set into the other,
+// the sets exchanged, the E half's end-of-step program on the new set, H again where that
+// program changed E (k_fix: old H, psi and currents, final E).
+template<class T>
+int fusedStep(alis_dev_world *D)
+{
+	if (ensureAlt<T>(D)) return -1;
+	if (!D->stepArgs && stepTable<T>(D)) return -1;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (D->timing && timingEvents(D, e0, e1)) return -1;
+	const StepShape z = stepShape(D);
+	const dim3 grid((unsigned)(8 * z.ngroup * ((z.nchunk + 7) / 8))), block(64, z.NC, z.R + 1);
+	for (int m = 0; m <= D->cplx; m++) {
+		const StepArgs<T> *sa = (const StepArgs<T> *)D->stepArgs + 2 * D->par + m;
+		hipEvent_t s0 = m == 0 ? e0 : nullptr, s1 = m == D->cplx ? e1 : nullptr;
+		if (D->seg) hipExtLaunchKernelGGL(k_step<T, 2>, grid, block, z.lds, D->st, s0, s1, 0, sa, z.R, z.P, z.chunk, z.nchunk, z.ngroup);
+		else if (D->C[0]) hipExtLaunchKernelGGL(k_step<T, 1>, grid, block, z.lds, D->st, s0, s1, 0, sa, z.R, z.P, z.chunk, z.nchunk, z.ngroup);
+		else hipExtLaunchKernelGGL(k_step<T, 0>, grid, block, z.lds, D->st, s0, s1, 0, sa, z.R, z.P, z.chunk, z.nchunk, z.ngroup);
+		HIPCHECK(hipGetLastError());
+	}
+	if (e1) {
+		D->evPairs[2].push_back(e0);
+		D->evPairs[2].push_back(e1);
+	}
+	swapSets(D);
+	D->pendE = 0;
+	D->carryDue[0] = D->carryDue[1] = true;   // both programs of this step run on the new set
+	if (postHalf<T>(D, 0)) return -1;
+	if (D->fix.empty()) return 0;
+	FixBoxes f{};
+	f.n = (int)D->fix.size();
+	for (int q = 0; q < f.n; q++) f.b[q] = D->fix[q], f.off[q + 1] = f.off[q] + D->fix[q].count();
+	for (int m = 0; m <= D->cplx; m++) {
+		CurlArgs<T> a = curlArgs<T>(D, 1, m);       // the new set: results, and the final E
+		a.O0 = a.F0, a.O1 = a.F1, a.O2 = a.F2;
+		a.F0 = (T *)D->Falt[1][m][0], a.F1 = (T *)D->Falt[1][m][1], a.F2 = (T *)D->Falt[1][m][2];
+		for (int g = 0; g < 6; g++)
+			if (a.face[g].on) {
+				CurlFace<T> &P = a.face[g];
+				P.qa = P.pa, P.qb = P.pb;
+				P.pa = (T *)D->slab[1][g].alt[m][0], P.pb = (T *)D->slab[1][g].alt[m][1];
+			}
+		a.objOut = a.obj;
+		a.obj = (const ObjDev<T> *)D->objTabAlt;
+		k_fix<T><<<blocksFor(f.off[f.n]), 256, 0, D->st>>>(a, f);
+		HIPCHECK(hipGetLastError());
+	}
+	return 0;
+}
+
+// Complete a deferred E half step the separate way (a call that reads or changes the world
+// between its E and H halves).
+template<class T>
+int flushE(alis_dev_world *D)
+{
+	const int p = D->pendE;
+	if (!p) return 0;
+	D->pendE = 0;
+	if (coreHalf<T>(D, 0)) return -1;
+	return p == 2 ? postHalf<T>(D, 0) : 0;
+}
+
+int flushE(alis_dev_world *D)
+{
+	return D->fp64 ? flushE<double>(D) : flushE<float>(D);
+}
+
+template<class T>
+int coreEntry(alis_dev_world *D, int h)
+{
+	if (h == 1 && D->pendE == 2) return fusedStep<T>(D);
+	if (flushE<T>(D)) return -1;
+	if (h == 0) {
+		D->carryDue[1] = false;
+		bool on;
+		if (fusedSteps<T>(D, on)) return -1;
+		if (on) {
+			D->pendE = 1;
+			return 0;
+		}
+	}
+	return coreHalf<T>(D, h);
 }
 
 template<class T>
@@ -2248,6 +2822,7 @@ struct alis_dev_batch {
 	size_t postBytes[2] = {};
 	hipEvent_t postDone[2] = {};    // the last copy from postPinned has been made
 	std::vector<void *> allocs;
+	bool held = false;              // counted in its worlds' inBatch
 };
 
 namespace {
@@ -2473,8 +3048,11 @@ int alis_dev_link_tfsf(alis_dev_world *D, int s, alis_dev_world *aux)
 {
 	if (s < 0 || s >= (int)D->tf.size()) return fail("TF/SF index out of range");
 	if (aux->fp64 != D->fp64) return fail("TF/SF auxiliary world precision differs");
+	if (flushE(D) || flushE(aux)) return -1;
 	D->tf[s].aux = aux;
-	D->postBuilt = false;
+	D->postBuilt = D->postBuiltAlt = false;
+	D->fuseState = 0;
+	aux->isAux = true;
 	// The auxiliary world keeps its own stream, so its half steps run beside the main
 	// world's curl launches; postHalf orders its strips against them with events.
 	return 0;
@@ -2496,7 +3074,7 @@ void alis_dev_release(alis_dev_world *D)
 			hipHostFree(P.exPinned), hipEventDestroy(P.exDone);
 		}
 	if (D->halfDone) hipEventDestroy(D->halfDone);
-	for (int h = 0; h < 2; h++) for (hipEvent_t e : D->evPairs[h]) (void)hipEventDestroy(e);
+	for (int h = 0; h < 3; h++) for (hipEvent_t e : D->evPairs[h]) (void)hipEventDestroy(e);
 	for (hipEvent_t e : D->evFree) (void)hipEventDestroy(e);
 	if (D->scratch) hipFree(D->scratch);
 	if (D->ffWork) hipFree(D->ffWork);
@@ -2514,7 +3092,8 @@ void alis_dev_release(alis_dev_world *D)
 
 int alis_dev_core(alis_dev_world *D, int h)
 {
-	return D->fp64 ? coreHalf<double>(D, h) : coreHalf<float>(D, h);
+	if (h < 0 || h > 1) return fail("no such half step");
+	return D->fp64 ? coreEntry<double>(D, h) : coreEntry<float>(D, h);
 }
 
 int alis_dev_strips(alis_dev_world *D, int s, int h)
@@ -2533,11 +3112,17 @@ int alis_dev_inject(alis_dev_world *D, int h, int index, float wave, const float
 int alis_dev_ghosts(alis_dev_world *D, int h)
 {
 	if (h < 0 || h > 1) return fail("no such half step");
+	if (h == 0 && D->pendE == 1) {      // deferred with its E core (see alis_dev_core)
+		D->pendE = 2;
+		return 0;
+	}
+	if (flushE(D)) return -1;
 	return D->fp64 ? postHalf<double>(D, h) : postHalf<float>(D, h);
 }
 
 int alis_dev_sync(alis_dev_world *D, int i0, int i1, int j0, int j1, int k0, int k1)
 {
+	if (flushE(D)) return -1;
 	Box b{i0, i1, j0, j1, k0, k1};
 	return D->fp64 ? syncHost<double>(D, b) : syncHost<float>(D, b);
 }
@@ -2594,6 +3179,7 @@ void gatherWorld(alis_dev_world *D, const Box &box, int mask, std::vector<Gather
 
 int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 {
+	for (int e = 0; e < n; e++) if (flushE(D[e])) return -1;
 	if (n < 1) return 0;
 	std::vector<GatherSeg> seg;
 	std::vector<GatherHost> out;
@@ -2657,18 +3243,22 @@ int alis_dev_sync_boxes(alis_dev_world *const *D, const int *box, int n)
 
 int alis_dev_read_field(alis_dev_world *D, int h, int m, int axis, double *out)
 {
+	if (flushE(D)) return -1;
 	if (m > D->cplx || h < 0 || h > 1 || axis < 0 || axis > 2) return fail("no such field array");
 	return D->fp64 ? readField<double>(D, h, m, axis, out) : readField<float>(D, h, m, axis, out);
 }
 
 int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double *in)
 {
+	if (flushE(D)) return -1;
 	if (m > D->cplx || h < 0 || h > 1 || axis < 0 || axis > 2) return fail("no such field array");
+	D->altStale = true;         // it may write cells no step writes (see ensureAlt)
 	return D->fp64 ? writeField<double>(D, h, m, axis, in) : writeField<float>(D, h, m, axis, in);
 }
 
 int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *id)
 {
+	if (flushE(D)) return -1;
 	DevPhaser Q;
 	Q.nf = P->nf;
 	Q.iMin = P->iMin, Q.iMax = P->iMax, Q.jMin = P->jMin, Q.jMax = P->jMax, Q.kMin = P->kMin, Q.kMax = P->kMax;
@@ -2733,6 +3323,7 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 
 int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const double *expE, const double *expH)
 {
+	if (flushE(D)) return -1;
 	if (id < 0 || id >= (int)D->phs.size() || !D->phs[id].live) return fail("no such phaser");
 	DevPhaser &P = D->phs[id];
 	if (q0 < 0 || nq < 0 || q0 + nq > P.nf) return fail("phaser frequency range out of bounds");
@@ -2744,6 +3335,7 @@ int alis_dev_phaser_update(alis_dev_world *D, int id, int q0, int nq, const doub
 int alis_dev_phaser_update_many(alis_dev_world *const *D, const int *id, const int *q0, const int *nq,
                                 const double *const *expE, const double *const *expH, int n)
 {
+	for (int e = 0; e < n; e++) if (flushE(D[e])) return -1;
 	if (n < 1) return 0;
 	for (int e = 0; e < n; e++) {
 		if (id[e] < 0 || id[e] >= (int)D[e]->phs.size() || !D[e]->phs[id[e]].live) return fail("no such phaser");
@@ -2766,6 +3358,7 @@ long alis_dev_phaser_size(alis_dev_world *D, int id)
 
 int alis_dev_phaser_read(alis_dev_world *D, int id, double *out, long n)
 {
+	if (flushE(D)) return -1;
 	long need = alis_dev_phaser_size(D, id);
 	if (need < 0 || n < need) return fail("phaser read buffer too small");
 	DevPhaser &P = D->phs[id];
@@ -2868,6 +3461,7 @@ int fluxScratch(alis_dev_world *D, double **out)
 int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const int *terms,
                               const double *w, long nw, double *out, long nout)
 {
+	if (flushE(D)) return -1;
 	FluxPlan F;
 	if (fluxPlan(D, id, q, 1, nterm, terms, nw, nout, F)) return -1;
 	if (ensureScratch(D, F.doubles)) return -1;
@@ -2877,6 +3471,7 @@ int alis_dev_phaser_flux_sums(alis_dev_world *D, int id, int q, int nterm, const
 int alis_dev_phaser_flux_sums_range(alis_dev_world *D, int id, int q0, int nq, int nterm, const int *terms,
                                     const double *w, long nw, double *out, long nout)
 {
+	if (flushE(D)) return -1;
 	FluxPlan F;
 	if (fluxPlan(D, id, q0, nq, nterm, terms, nw, nout, F)) return -1;
 	if (ensureScratch(D, F.doubles)) return -1;
@@ -2886,6 +3481,7 @@ int alis_dev_phaser_flux_sums_range(alis_dev_world *D, int id, int q0, int nq, i
 int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nterm, const int *terms,
                                       const double *w, long nw, double *out, long nout)
 {
+	if (flushE(D)) return -1;
 	FluxPlan F;
 	if (fluxPlan(D, id, q, 1, nterm, terms, nw, nout, F)) return -1;
 	if (F.doubles > FLUX_SCRATCH_DOUBLES)
@@ -2898,6 +3494,7 @@ int alis_dev_phaser_flux_sums_noalloc(alis_dev_world *D, int id, int q, int nter
 
 int alis_dev_phaser_sync(alis_dev_world *D, int id)
 {
+	if (flushE(D)) return -1;
 	long need = alis_dev_phaser_size(D, id);
 	if (need < 0) return fail("no such phaser");
 	DevPhaser &Q = D->phs[id];
@@ -3313,6 +3910,7 @@ int alis_dev_farfield(alis_dev_world *D, int id, int q, const alis_dev_ff_geom *
 
 int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out)
 {
+	for (int w = 0; w < n; w++) if (flushE(D[w])) return -1;
 	*out = nullptr;
 	if (n < 1) return fail("empty batch");
 	alis_dev_batch *B = new alis_dev_batch;
@@ -3326,6 +3924,8 @@ int alis_dev_batch_create(alis_dev_world *const *D, int n, alis_dev_batch **out)
 	B->st = D[0]->st;
 	int rc = B->fp64 ? batchBuildCurl<double>(B) : batchBuildCurl<float>(B);
 	if (rc) { alis_dev_batch_release(B); return rc; }
+	for (alis_dev_world *W : B->W) W->inBatch++;   // their arrays stay where the arguments name them
+	B->held = true;
 	*out = B;
 	return 0;
 }
@@ -3359,6 +3959,7 @@ void alis_dev_batch_release(alis_dev_batch *B)
 {
 	if (!B) return;
 	for (alis_dev_world *W : B->W) hipStreamSynchronize(W->st);   // the batch's launches ran on one of these
+	if (B->held) for (alis_dev_world *W : B->W) W->inBatch--;
 	for (void *p : B->allocs) hipFree(p);
 	for (int h = 0; h < 2; h++) {
 		if (B->postPinned[h]) hipHostFree(B->postPinned[h]);
@@ -3429,6 +4030,7 @@ int absorption(alis_dev_world *D, int n, const Box &b, const double *dzE, const 
 
 int alis_dev_plane_sums(alis_dev_world *D, int nterm, const int *terms, const double *w, long nw, double *out, long nout)
 {
+	if (flushE(D)) return -1;
 	if (nterm < 1 || nw < 1 || nout < 1) return fail("plane sums: no terms or no buffers");
 	return D->fp64 ? planeSums<double>(D, nterm, terms, w, nw, out, nout) : planeSums<float>(D, nterm, terms, w, nw, out, nout);
 }
@@ -3436,6 +4038,7 @@ int alis_dev_plane_sums(alis_dev_world *D, int nterm, const int *terms, const do
 int alis_dev_absorption(alis_dev_world *D, int n, int i0, int i1, int j0, int j1, int k0, int k1,
                         const double *dzE, const double *dzH, double *out)
 {
+	if (flushE(D)) return -1;
 	if (n < 0 || n >= (int)D->obj.size()) return fail("absorption: no such dispersive object");
 	if (!D->C[0]) return fail("absorption: the world has no permittivity arrays");
 	const Box &o = D->obj[n].clip;
@@ -3545,6 +4148,7 @@ void alis_dev_spectrum_release(alis_dev_spectrum *S)
 
 int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P2, int n, const float *d, float alt)
 {
+	if (flushE(S) || flushE(P1) || flushE(P2)) return -1;
 	if (S == P1 || S == P2) return fail("three-wave coupling: the signal world is one of the pumps");
 	for (alis_dev_world *P : {P1, P2})
 		if (P->fp64 != S->fp64 || P->cplx != S->cplx || P->iMIN != S->iMIN || P->iMAX != S->iMAX || P->jMIN != S->jMIN
@@ -3581,6 +4185,7 @@ int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P
 
 int alis_dev_synchronize(alis_dev_world *D)
 {
+	if (flushE(D)) return -1;
 	HIPCHECK(hipStreamSynchronize(D->st));
 	return 0;
 }
@@ -3593,7 +4198,7 @@ double alis_dev_eps_bytes(const alis_dev_world *D)
 int alis_dev_timing(alis_dev_world *D, int enable)
 {
 	HIPCHECK(hipStreamSynchronize(D->st));
-	for (int h = 0; h < 2; h++) {
+	for (int h = 0; h < 3; h++) {
 		D->evFree.insert(D->evFree.end(), D->evPairs[h].begin(), D->evPairs[h].end());
 		D->evPairs[h].clear();
 		D->evMs[h] = 0, D->evCount[h] = 0;
@@ -3611,6 +4216,7 @@ int alis_dev_timing(alis_dev_world *D, int enable)
 
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches)
 {
+	if (h < 0 || h > 2) return fail("no such timed launch");
 	HIPCHECK(hipStreamSynchronize(D->st));
 	auto &v = D->evPairs[h];
 	for (size_t a = 0; a + 1 < v.size(); a += 2) {

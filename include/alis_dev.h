@@ -97,7 +97,14 @@ void alis_dev_release(alis_dev_world *D);
 /* Half-step core: for E (h=0) conductivity scaling and -sum J of dispersive objects, the
  * curl of H and the CPML slabs (src/update.c:87-125 with Alt=+1); for H (h=1) the curl of
  * E with CPML (Alt=-1) followed by the ADE current update on the new E
- * (src/update.c:128-158).  Both real and imaginary parts of a Bloch world. */
+ * (src/update.c:128-158).  Both real and imaginary parts of a Bloch world.
+ * Fused steps (opt-in: the environment variable ALIS_FUSE=1 for worlds large enough to
+ * fill the GPU, ALIS_FUSE=force for any world; never a TF/SF auxiliary world or a world in a
+ * batch): the E core and its alis_dev_ghosts are deferred, and the H core runs both halves
+ * in one pass over the grid (k_step) followed by the E half's end-of-step program and a
+ * recomputation of H where that program changed E; the results are those of the separate
+ * half steps, bit for bit.  Any other call on the world (or on a world that reads it) first
+ * completes a deferred E half step the separate way. */
 int alis_dev_core(alis_dev_world *D, int h);
 
 /* Face-strip corrections of the s-th TF/SF box from its auxiliary world
@@ -274,7 +281,8 @@ int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P
 int alis_dev_synchronize(alis_dev_world *D);
 
 /* Timing of the curl kernels with device events on the world's stream: enable (resets the
- * totals) or disable, then read total milliseconds and launch count per half (h = 0 E, 1 H). */
+ * totals) or disable, then read total milliseconds and launch count per half (h = 0 E,
+ * 1 H) or of the fused step kernel (h = 2, see alis_dev_core). */
 int alis_dev_timing(alis_dev_world *D, int enable);
 int alis_dev_timing_read(alis_dev_world *D, int h, double *ms, long *launches);
 
