@@ -161,7 +161,7 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // (149 us either way, so E keeps the plain grid); groups of 2 or 4 rows per XCD, which would
 // also keep the j+1 row local, lose 5-11 %.
 
-template<class T, int CM, bool NT = true, bool NTG = false, bool MZ = false>
+template<class T, int CM, bool NT = true, bool NTG = false, bool MZ = false, bool PC = false>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k);
 
 template<class T, int CM, bool NT = true, bool NTG = false, int BY = 4>
@@ -178,10 +178,10 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 // neighbour values, and the psi slabs go through the cache (the short rows of small worlds
 // cover a slab with partial lines that neighbouring tiles complete).  Measured on 64 k-point
 // worlds: 5 % (E) and 14 % (H) faster than reloading and streaming; slower in k_curl.
-template<class T, int CM, bool NT, bool NTG, bool MZ>
+template<class T, int CM, bool NT, bool NTG, bool MZ, bool PC>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
-	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true;
+	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true, NTC = NT && !PC;
 #include "curl_cell.inc"
 }
 
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	const int j = a.jMIN + by * BY + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
-	constexpr bool SHARE = false, NTP = NT, MERGEZ = false, UNI = true, SEP = false, wr = true;
+	constexpr bool SHARE = false, NTP = NT, MERGEZ = false, UNI = true, SEP = false, wr = true, NTC = NT;
 #include "curl_cell.inc"
 }
 
@@ -229,12 +229,15 @@ __device__ __forceinline__ int batchEntry(const int *start, int n, int b)
 // one launch per class of entries.  H launches (alt < 0), which read the next plane of E, run each entry's
 // tiles in descending order, so that plane was read as a centre just before (64 k-point
 // worlds: H curl 331 -> 306 us; the 256^3 k_curl shows no difference).
-template<class T, int TPB, int FLAT = -1, int CM = -1>
+// PAIR: every entry is both parts of a Bloch world (arguments 2e and 2e + 1, one geometry):
+// each thread updates its cell of the real part and then of the imaginary part, which reads
+// the same 1/eps values again, from the cache (the real part loads them cached).
+template<class T, int TPB, int FLAT = -1, int CM = -1, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
 {
 	const int b = (int)blockIdx.x;
 	const int e = batchEntry(start, n, b);
-	const CurlArgs<T> &a = args[e];
+	const CurlArgs<T> &a = args[PAIR ? 2 * e : e];
 	if (FLAT == 1 || (FLAT < 0 && a.flatPitch)) {
 		// packed rows shorter than a wavefront: a tile is 256 consecutive cells of one x
 		// plane from the plane's first (ghost) row on (ghost rows, row ghosts and plane
@@ -247,9 +250,15 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 			const int off = (t % per) * 256 + lid, jj = off / pitch - 1, kk = off - (jj + 1) * pitch;
 			if (jj < 0 || jj >= a.nj || kk < 1 || kk > a.nk) continue;
 			const int i = a.iMIN + t / per, j = a.jMIN + jj, k = a.kMIN - 1 + kk;
-			if (CM == 2 || (CM < 0 && a.seg)) curlAt<T, 2, true, false, true>(a, i, j, k);
-			else if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true>(a, i, j, k);
-			else curlAt<T, 0, true, false, true>(a, i, j, k);
+			if (CM == 2 || (CM < 0 && a.seg)) curlAt<T, 2, true, false, true, PAIR>(a, i, j, k);
+			else if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true, PAIR>(a, i, j, k);
+			else curlAt<T, 0, true, false, true, PAIR>(a, i, j, k);
+			if (PAIR) {
+				const CurlArgs<T> &a1 = args[2 * e + 1];
+				if (CM == 2 || (CM < 0 && a1.seg)) curlAt<T, 2, true, false, true>(a1, i, j, k);
+				else if (CM == 1 || (CM < 0 && a1.C0)) curlAt<T, 1, true, false, true>(a1, i, j, k);
+				else curlAt<T, 0, true, false, true>(a1, i, j, k);
+			}
 		}
 		return;
 	}
@@ -260,6 +269,12 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 		if (CM == 2 || (CM < 0 && a.seg)) curlCell<T, 2>(a, bx, by, bz);
 		else if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1>(a, bx, by, bz);
 		else curlCell<T, 0>(a, bx, by, bz);
+		if (PAIR) {
+			const CurlArgs<T> &a1 = args[2 * e + 1];
+			if (CM == 2 || (CM < 0 && a1.seg)) curlCell<T, 2>(a1, bx, by, bz);
+			else if (CM == 1 || (CM < 0 && a1.C0)) curlCell<T, 1>(a1, bx, by, bz);
+			else curlCell<T, 0>(a1, bx, by, bz);
+		}
 	}
 }
 
@@ -365,7 +380,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 4) void k_step(const Ste
 				const CurlArgs<T> &a = sa->e;
 				const bool wr = own;
 				constexpr int CM = CME;
-				constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true;
+				constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, NTC = true;
 #define CURL_GIVEN
 #define CURL_GIVEN_F
 #include "curl_cell.inc"
@@ -387,7 +402,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 4) void k_step(const Ste
 			const T g1i = before[P + x], g2i = before[2 * P + x];
 			const CurlArgs<T> &a = sa->h;
 			constexpr int CM = 0;
-			constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, wr = true;
+			constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, wr = true, NTC = true;
 #define CURL_GIVEN
 #define CURL_GIVEN_F
 #include "curl_cell.inc"
@@ -448,7 +463,7 @@ __global__ __launch_bounds__(256) void k_fix(CurlArgs<T> a, FixBoxes f)
 	boxCell(t0 - f.off[q], B.i1 - B.i0 + 1, B.j1 - B.j0 + 1, B.k1 - B.k0 + 1, di, dj, dk);
 	const int i = B.i0 + di, j = B.j0 + dj, k = B.k0 + dk;
 	constexpr int CM = 0;
-	constexpr bool NT = false, NTG = false, SHARE = true, NTP = false, MERGEZ = false, UNI = false, SEP = true, wr = true;
+	constexpr bool NT = false, NTG = false, SHARE = true, NTP = false, MERGEZ = false, UNI = false, SEP = true, wr = true, NTC = false;
 #include "curl_cell.inc"
 }
 
@@ -486,6 +501,7 @@ constexpr int MAXSRC = 128;     // sources per chain (E or H) of one world
 constexpr int MAXCHAIN = 24;    // copies per launch
 constexpr int MAXPOST = 48;     // operations per launch
 constexpr int CURL_TPB = 2;     // tiles per workgroup of a batched curl launch (measured best of 1..16)
+constexpr int CURL_TPB_PAIR = 1;  // the same for entries of both parts of a Bloch world
 
 struct PostPiece {
 	int kind;                   // OP_COPY, or OP_STRIP for an additive piece
@@ -2805,7 +2821,7 @@ struct alis_dev_batch {
 	void *curlStart[2] = {};    // int[] workgroup starts (from 0 within each class)
 	// the entries in classes of one tile kind and one 1/eps source, each class contiguous in
 	// the tables and run by its own specialised launch
-	struct CurlClass { int first, n, blocks, flat, cm; };
+	struct CurlClass { int first, arg, n, blocks, flat, cm, pair; };   // first: start index; arg: argument index
 	std::vector<CurlClass> curlClass[2];
 	bool postReady = false;
 	struct Round {
@@ -2838,9 +2854,13 @@ template<class T>
 int batchBuildCurl(alis_dev_batch *B)
 {
 	for (int h = 0; h < 2; h++) {
-		struct Entry { CurlArgs<T> a; long tiles; int cls; };
+		// an entry: one part of a world, or both parts of a Bloch world (pair)
+		struct Entry { CurlArgs<T> a[2]; long tiles; int cls; };
+		const char *pe = getenv("ALIS_BATCH_PAIRS");
+		const bool pairs = !(pe && !strcmp(pe, "0"));
 		std::vector<Entry> all;
-		for (alis_dev_world *D : B->W)
+		for (alis_dev_world *D : B->W) {
+			const bool pair = pairs && D->cplx;
 			for (int m = 0; m <= D->cplx; m++) {
 				CurlArgs<T> a = curlArgs<T>(D, h, m);
 				// tiles as k_curl_batch counts them: flat tiles for packed rows shorter than a
@@ -2851,20 +2871,25 @@ int batchBuildCurl(alis_dev_batch *B)
 				                        : (long)((D->kNUM + 63) / 64) * ((D->jNUM + 3) / 4) * D->iNUM;
 				// 1/eps: 2 per-segment table (and arrays where a segment is mixed), 1 arrays, 0 none
 				const int cm = a.seg ? 2 : a.C0 ? 1 : 0;
-				all.push_back(Entry{a, tiles, (flat ? 3 : 0) + cm});
+				if (pair && m == 1) all.back().a[1] = a;
+				else all.push_back(Entry{{a, a}, tiles, (pair ? 6 : 0) + (flat ? 3 : 0) + cm});
 			}
+		}
 		std::stable_sort(all.begin(), all.end(), [](const Entry &x, const Entry &y) { return x.cls < y.cls; });
 		std::vector<CurlArgs<T>> args;
 		std::vector<int> start;
 		B->curlClass[h].clear();
 		for (const Entry &e : all) {
-			if (B->curlClass[h].empty() || B->curlClass[h].back().flat * 3 + B->curlClass[h].back().cm != e.cls)
-				B->curlClass[h].push_back({(int)args.size(), 0, 0, e.cls / 3, e.cls % 3});
+			const int pair = e.cls / 6, flat = e.cls / 3 % 2, cm = e.cls % 3;
+			if (B->curlClass[h].empty() || B->curlClass[h].back().pair * 6 + B->curlClass[h].back().flat * 3 + B->curlClass[h].back().cm != e.cls)
+				B->curlClass[h].push_back({(int)start.size(), (int)args.size(), 0, 0, flat, cm, pair});
 			alis_dev_batch::CurlClass &c = B->curlClass[h].back();
-			args.push_back(e.a);
+			args.push_back(e.a[0]);
+			if (pair) args.push_back(e.a[1]);
 			start.push_back(c.blocks);
 			c.n++;
-			c.blocks += (int)((e.tiles + CURL_TPB - 1) / CURL_TPB);
+			const int tpb = pair ? CURL_TPB_PAIR : CURL_TPB;
+			c.blocks += (int)((e.tiles + tpb - 1) / tpb);
 		}
 		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
 		if (batchAlloc(B, &B->curlStart[h], start.size() * sizeof(int))) return -1;
@@ -2929,12 +2954,25 @@ int batchCore(alis_dev_batch *B, int h)
 	const std::vector<alis_dev_batch::CurlClass> &cls = B->curlClass[h];
 	for (size_t q = 0; q < cls.size(); q++) {
 		const alis_dev_batch::CurlClass &c = cls[q];
-		const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h] + c.first;
+		const CurlArgs<T> *args = (const CurlArgs<T> *)B->curl[h] + c.arg;
 		const int *start = (const int *)B->curlStart[h] + c.first;
 		const dim3 grid(c.blocks), block(64, 4);
 		hipEvent_t s0 = q == 0 ? e0 : nullptr, s1 = q + 1 == cls.size() ? e1 : nullptr;
 #define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, F, C>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
-		if (c.flat) {
+#define CURL_PAIR(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB_PAIR, F, C, true>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
+		if (c.pair) {
+			if (c.flat) {
+				if (c.cm == 2) CURL_PAIR(1, 2);
+				else if (c.cm == 1) CURL_PAIR(1, 1);
+				else CURL_PAIR(1, 0);
+			}
+			else {
+				if (c.cm == 2) CURL_PAIR(0, 2);
+				else if (c.cm == 1) CURL_PAIR(0, 1);
+				else CURL_PAIR(0, 0);
+			}
+		}
+		else if (c.flat) {
 			if (c.cm == 2) CURL_BATCH(1, 2);
 			else if (c.cm == 1) CURL_BATCH(1, 1);
 			else CURL_BATCH(1, 0);
@@ -2944,6 +2982,7 @@ int batchCore(alis_dev_batch *B, int h)
 			else if (c.cm == 1) CURL_BATCH(0, 1);
 			else CURL_BATCH(0, 0);
 		}
+#undef CURL_PAIR
 #undef CURL_BATCH
 		HIPCHECK(hipGetLastError());
 	}

@@ -406,7 +406,10 @@ def batched(alis, w, hip, args):
     ksteps = max(args.steps // 2, 512)
     with quiet():
         rb = run(alis, w, hip, "kpoints", args.kpoints, 32, 64, args.warmup, ksteps)
-    rb["eps_bytes"] = 3 * 4 if rb["eps"] else 0       # batched launches read the 1/eps arrays
+    # batched launches read the 1/eps arrays: once for both parts of a Bloch world when a
+    # thread updates both (pair entries, the default; ALIS_BATCH_PAIRS=0 separates them)
+    pairs = os.environ.get("ALIS_BATCH_PAIRS") != "0"
+    rb["eps_bytes"] = 3 * 4 / (rb["parts"] if pairs else 1) if rb["eps"] else 0
     vb = rb["cells"] * ksteps / rb["sec"] / 1e6
     rlb = roofline(rb, 4, 8, ksteps)
     return {
