@@ -100,6 +100,17 @@ template<class T> struct ObjDev {
 	T *J[2][3][MAXPOLE], *K[2][3][MAXPOLE];
 };
 
+// A z face of a phaser whose samples the H curl stores (k_curl, ROWS): of the face (i0..i1,
+// j0..j1) at k, the cells (si0..si1, sj0..sj1), at their places from `base` in the ring slot
+// [4][cells] (E x, E y, H x(k) + H x(k+1), H y(k) + H y(k+1)), as k_phs_sample stores them.
+template<class T> struct PhsZ {
+	int k, i0, i1, j0, j1;      // the face
+	int si0, si1, sj0, sj1;     // the cells sampled here
+	long cells, base;
+	T *slot;
+};
+constexpr int MAXZS = 4;
+
 template<class T> struct CurlArgs {
 	T *F0, *F1, *F2;
 	T *O0, *O1, *O2;            // where the new values go (F* but in k_step and k_fix)
@@ -119,6 +130,8 @@ template<class T> struct CurlArgs {
 	const int *otStart;         // per 64 x 4 tile (k_curl's tiles): its objects are
 	const int *otList;          //  otList[otStart[t] .. otStart[t+1]) ascending; null: all
 	int flatPitch;              // batched launches of packed short rows: row length with ghosts
+	int nzs;                    // H launches of k_curl: phaser z faces to sample (see PhsZ)
+	PhsZ<T> zs[MAXZS];
 };
 
 // Fields and coefficients live in global memory, but to the compiler a pointer that a kernel
@@ -202,6 +215,20 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
 	constexpr bool SHARE = false, NTP = NT, MERGEZ = false, UNI = true, SEP = false, wr = true, NTC = NT;
 #include "curl_cell.inc"
+	// H: the samples of the phasers' z faces from the values at hand -- the new H here and at
+	// k + 1 (the next lane: the host takes faces whose k + 1 lies in the same wavefront) and E,
+	// final after its end-of-half-step program -- in place of k_phs_sample reading a strided
+	// plane (the host takes faces that the H program does not change)
+	if (ROWS && a.nzs) {
+		const T hx1 = __shfl_down(fx, 1), hy1 = __shfl_down(fy, 1);
+		for (int z = 0; z < a.nzs; z++) {
+			const PhsZ<T> &Z = a.zs[z];
+			if (k != Z.k || i < Z.si0 || i > Z.si1 || j < Z.sj0 || j > Z.sj1) continue;
+			const long t = Z.base + (long)(i - Z.i0) * (Z.j1 - Z.j0 + 1) + (j - Z.j0);
+			T *const q = Z.slot;
+			q[t] = gx, q[Z.cells + t] = gy, q[2 * Z.cells + t] = fx + hx1, q[3 * Z.cells + t] = fy + hy1;
+		}
+	}
 }
 
 // The entry owning workgroup b of a batch launch: start[] is increasing; every wave finds it
@@ -618,6 +645,8 @@ struct PhsFaceGeom {
 	int fixed;                  // the face's fixed index
 	int n2;                     // cells along the fast face axis
 	long off;                   // first cell of this face in the concatenated cell list
+	int si0, si1, sj0, sj1;     // z faces: cells (i, j) the H curl sampled (k_phs_sample skips
+	                            // them), none when si0 > si1
 };
 
 constexpr int PHS_QPT = 4;      // frequencies per thread (measured: 1 < 2 < 4)
@@ -683,10 +712,20 @@ __device__ inline void phaserBlock(int w, int ny, int &xb, int &yb)
 // a flush adds the pending samples to the accumulators in sample order, the same additions as
 // one immediate update per sample, reading and writing the accumulators once for all of them.
 template<class T>
-__global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict__ slot)
+__global__ __launch_bounds__(256) void k_phs_sample(PhsArgs<T> a, T *__restrict__ slot, long n)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= a.ncell) return;
+	if (t >= n) return;
+	{
+		int f = 0;
+		while (f + 1 < a.nface && t >= a.face[f + 1].off) f++;
+		const PhsFaceGeom &g = a.face[f];
+		if (g.kind >= 4 && g.si0 <= g.si1) {        // cells the H curl sampled
+			const long local = t - g.off;
+			const int i = a.iMin + (int)(local / g.n2), j = a.jMin + (int)(local % g.n2);
+			if (i >= g.si0 && i <= g.si1 && j >= g.sj0 && j <= g.sj1) return;
+		}
+	}
 	T e1, e2, h1, h2;
 	phaserFields(a, t, e1, e2, h1, h2);
 	slot[t] = e1, slot[a.ncell + t] = e2, slot[2 * a.ncell + t] = h1, slot[3 * a.ncell + t] = h2;
@@ -1222,6 +1261,8 @@ struct DevPhaser {
 	double *exDev = nullptr;
 	hipEvent_t exDone = nullptr;  // the staging has been copied
 	int pend = 0, pq0 = 0, pnq = 0;  // samples pending and their frequency range
+	bool pre = false;           // the H curl stored the z faces' sample of slot preSlot
+	int preSlot = -1;           //  since the fields last changed
 };
 
 constexpr int PHS_RING = 8;     // samples per deferred flush
@@ -1290,6 +1331,7 @@ struct alis_dev_world {
 	struct Carry { int axis; Box b; };
 	std::vector<Carry> carry[2];  // cells a fused step's end-of-half programs take from the other set
 	bool carryDue[2] = {false, false};
+	std::vector<Box> hWrites;   // the boxes the end-of-H-half program writes
 	std::vector<Box> fix;       // boxes k_fix recomputes after the E program
 	bool fixOk = false;         // every additive operation lies in the update range and the
 	                            //  boxes fit one k_fix launch
@@ -1913,9 +1955,64 @@ int timingEvents(alis_dev_world *D, hipEvent_t &e0, hipEvent_t &e1)
 	return 0;
 }
 
+// The phasers' samples the last H curl stored no longer hold once the fields change.
+void dropPresamples(alis_dev_world *D)
+{
+	for (DevPhaser &P : D->phs) P.pre = false;
+}
+
+// The z faces of D's phasers that the H curl can sample (PhsZ): a face whose k + 1 lies in
+// the same wavefront as k (k_curl's waves cover 64 cells from kMIN on), over the rectangle of
+// its cells that the end-of-H-half program leaves alone at k and k + 1 (shrunk from the edges
+// its boxes reach: the ghost planes of periodic boundaries), so the values the curl has are
+// the ones the phaser update would read; k_phs_sample samples the other cells.
+template<class T>
+int zSamples(alis_dev_world *D, PhsZ<T> *zs, std::vector<DevPhaser *> &owners)
+{
+	int n = 0;
+	if (!D->postBuilt) return 0;
+	for (DevPhaser &P : D->phs) {
+		if (!P.live || !P.ncell) continue;
+		const int before = n;
+		bool ok = true;
+		for (int f = 0; f < P.nface && ok; f++) {
+			const PhsFaceGeom &g = P.face[f];
+			if (g.kind < 4) continue;
+			const int k = g.fixed;
+			if ((k - D->kMIN) % 64 == 63 || k + 1 > D->kMAX || n == MAXZS) { ok = false; break; }
+			int i0 = P.iMin, i1 = P.iMax, j0 = P.jMin, j1 = P.jMax;
+			for (const Box &b : D->hWrites) {
+				if (b.k1 < k || b.k0 > k + 1 || b.i1 < i0 || b.i0 > i1 || b.j1 < j0 || b.j0 > j1) continue;
+				if (b.i1 - b.i0 < b.j1 - b.j0 && b.i1 < (i0 + i1) / 2) i0 = b.i1 + 1;          // a plane at low i
+				else if (b.i1 - b.i0 < b.j1 - b.j0 && b.i0 > (i0 + i1) / 2) i1 = b.i0 - 1;
+				else if (b.j1 < (j0 + j1) / 2) j0 = b.j1 + 1;
+				else if (b.j0 > (j0 + j1) / 2) j1 = b.j0 - 1;
+				else ok = false;
+			}
+			if (i0 > i1 || j0 > j1) ok = false;
+			zs[n++] = PhsZ<T>{k, P.iMin, P.iMax, P.jMin, P.jMax, i0, i1, j0, j1, P.ncell, g.off,
+			                  (T *)P.ring + (size_t)P.pend * 4 * P.ncell};
+		}
+		if (!ok || n == before) {
+			n = before;
+			continue;
+		}
+		// the rectangles, for k_phs_sample to skip
+		for (int f = 0, z = before; f < P.nface; f++) {
+			PhsFaceGeom &g = P.face[f];
+			if (g.kind < 4) continue;
+			g.si0 = zs[z].si0, g.si1 = zs[z].si1, g.sj0 = zs[z].sj0, g.sj1 = zs[z].sj1;
+			z++;
+		}
+		owners.push_back(&P);
+	}
+	return n;
+}
+
 template<class T>
 int coreHalf(alis_dev_world *D, int h)
 {
+	dropPresamples(D);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (D->timing) {
 		if (timingEvents(D, e0, e1)) return -1;
@@ -1927,8 +2024,10 @@ int coreHalf(alis_dev_world *D, int h)
 	                         : dim3((unsigned)((rows + 7) / 8 * 8 * ((D->kNUM + 63) / 64))), block(64, 4);
 	// timing events are bound to the launches themselves (their start and end), so timing
 	// adds no marker between launches
+	std::vector<DevPhaser *> sampled;
 	for (int m = 0; m <= D->cplx; m++) {
 		CurlArgs<T> a = curlArgs<T>(D, h, m);
+		if (h == 1 && m == 0) a.nzs = zSamples<T>(D, a.zs, sampled);   // phasers sample part 0
 		hipEvent_t s0 = m == 0 ? e0 : nullptr, s1 = m == D->cplx ? e1 : nullptr;
 		if (h == 0 && D->seg) hipExtLaunchKernelGGL(k_curl<T, 2, false>, grid, block, 0, D->st, s0, s1, 0, a);
 		else if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, 1, false>, grid, block, 0, D->st, s0, s1, 0, a);
@@ -1940,6 +2039,7 @@ int coreHalf(alis_dev_world *D, int h)
 		D->evPairs[h].push_back(e1);
 	}
 	HIPCHECK(hipGetLastError());
+	for (DevPhaser *P : sampled) P->pre = true, P->preSlot = P->pend;
 	return 0;
 }
 
@@ -2159,6 +2259,11 @@ int buildPost(alis_dev_world *D)
 		if (h == 0) {
 			for (const PostOp<T> &o : adds) eOps.push_back(o.b);
 			for (const PostOp<T> &o : copies) eOps.push_back(o.b);
+		}
+		else {
+			D->hWrites.clear();
+			for (const PostOp<T> &o : adds) D->hWrites.push_back(o.b);
+			for (const PostOp<T> &o : copies) D->hWrites.push_back(o.b);
 		}
 		copyOps[h] = copies;
 		if (copies.size() <= (size_t)MAXCHAIN && adds.size() + copies.size() <= (size_t)MAXPOST && !postConflict(adds, copies)) {
@@ -2464,6 +2569,7 @@ int stepTable(alis_dev_world *D)
 template<class T>
 int fusedStep(alis_dev_world *D)
 {
+	dropPresamples(D);
 	if (ensureAlt<T>(D)) return -1;
 	if (!D->stepArgs && stepTable<T>(D)) return -1;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2661,7 +2767,10 @@ int phaserUpdate(alis_dev_world *D, DevPhaser &P, int qa, int qn, const double *
 		P.pq0 = qa, P.pnq = qn;
 	}
 	PhsArgs<T> a = phaserArgs<T>(D, P, qa, qn);
-	k_phs_sample<T><<<blocksFor(P.ncell), 256, 0, D->st>>>(a, (T *)P.ring + (size_t)P.pend * 4 * P.ncell);
+	const bool pre = P.pre && P.preSlot == P.pend;   // the H curl stored the z faces' inner cells
+	P.pre = false;
+	for (int f = 0; !pre && f < a.nface; f++) a.face[f].si0 = 1, a.face[f].si1 = 0;
+	k_phs_sample<T><<<blocksFor(P.ncell), 256, 0, D->st>>>(a, (T *)P.ring + (size_t)P.pend * 4 * P.ncell, P.ncell);
 	HIPCHECK(hipGetLastError());
 	double *x = P.exPinned + (size_t)P.pend * 4 * qn;
 	for (int q = 0; q < qn; q++)
@@ -2716,6 +2825,7 @@ int phaserBatchUpdate(alis_dev_world *L, alis_dev_world *const *D, const int *id
 			HIPCHECK(hipEventSynchronize(P.exDone));
 			P.pq0 = q0[e], P.pnq = nq[e];
 		}
+		P.pre = false;              // this path samples every face itself
 		PhsArgs<T> a = phaserArgs<T>(D[e], P, q0[e], nq[e]);
 		a.acc = (T *)P.ring + (size_t)P.pend * 4 * P.ncell;
 		sa.push_back(a);
@@ -2947,6 +3057,7 @@ template<class T>
 int batchCore(alis_dev_batch *B, int h)
 {
 	alis_dev_world *D0 = B->W[0];
+	for (alis_dev_world *W : B->W) dropPresamples(W);
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	if (D0->timing) {
 		if (timingEvents(D0, e0, e1)) return -1;
@@ -3292,6 +3403,7 @@ int alis_dev_write_field(alis_dev_world *D, int h, int m, int axis, const double
 	if (flushE(D)) return -1;
 	if (m > D->cplx || h < 0 || h > 1 || axis < 0 || axis > 2) return fail("no such field array");
 	D->altStale = true;         // it may write cells no step writes (see ensureAlt)
+	dropPresamples(D);
 	return D->fp64 ? writeField<double>(D, h, m, axis, in) : writeField<float>(D, h, m, axis, in);
 }
 
@@ -3316,6 +3428,7 @@ int alis_dev_phaser_add(alis_dev_world *D, const alis_dev_phaser_desc *P, int *i
 		long cells = kind < 2 ? (long)nj * nk : kind < 4 ? (long)ni * nk : (long)ni * nj;
 		g.n2 = kind < 4 ? nk : nj;
 		g.off = off;
+		g.si0 = 1, g.si1 = 0, g.sj0 = 1, g.sj1 = 0;
 		off += cells;
 		Q.face[Q.nface++] = g;
 	}
@@ -4188,6 +4301,7 @@ void alis_dev_spectrum_release(alis_dev_spectrum *S)
 int alis_dev_three_wave(alis_dev_world *S, alis_dev_world *P1, alis_dev_world *P2, int n, const float *d, float alt)
 {
 	if (flushE(S) || flushE(P1) || flushE(P2)) return -1;
+	dropPresamples(S);
 	if (S == P1 || S == P2) return fail("three-wave coupling: the signal world is one of the pumps");
 	for (alis_dev_world *P : {P1, P2})
 		if (P->fp64 != S->fp64 || P->cplx != S->cplx || P->iMIN != S->iMIN || P->iMAX != S->iMAX || P->jMIN != S->jMIN
