@@ -198,7 +198,7 @@ __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k
 #include "curl_cell.inc"
 }
 
-template<class T, int CM, bool ROWS, bool NT = true, bool NTG = false, int BY = 4>
+template<class T, int CM, bool ROWS, bool NT = true, bool NTG = false, int BY = 4, bool ZS = false>
 __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 {
 	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
@@ -219,11 +219,14 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	// k + 1 (the next lane: the host takes faces whose k + 1 lies in the same wavefront) and E,
 	// final after its end-of-half-step program -- in place of k_phs_sample reading a strided
 	// plane (the host takes faces that the H program does not change)
-	if (ROWS && a.nzs) {
-		const T hx1 = __shfl_down(fx, 1), hy1 = __shfl_down(fy, 1);
+	if (ZS && ROWS) {
 		for (int z = 0; z < a.nzs; z++) {
 			const PhsZ<T> &Z = a.zs[z];
-			if (k != Z.k || i < Z.si0 || i > Z.si1 || j < Z.sj0 || j > Z.sj1) continue;
+			// the wavefront's row and 64 cells (bx) hold a sampled cell of the face: the same
+			// for all its lanes, so only these waves exchange values
+			if (i < Z.si0 || i > Z.si1 || j < Z.sj0 || j > Z.sj1 || (Z.k - a.kMIN) >> 6 != bx) continue;
+			const T hx1 = __shfl_down(fx, 1), hy1 = __shfl_down(fy, 1);
+			if (k != Z.k) continue;
 			const long t = Z.base + (long)(i - Z.i0) * (Z.j1 - Z.j0 + 1) + (j - Z.j0);
 			T *const q = Z.slot;
 			q[t] = gx, q[Z.cells + t] = gy, q[2 * Z.cells + t] = fx + hx1, q[3 * Z.cells + t] = fy + hy1;
@@ -2006,6 +2009,19 @@ int zSamples(alis_dev_world *D, PhsZ<T> *zs, std::vector<DevPhaser *> &owners)
 		}
 		owners.push_back(&P);
 	}
+	// worth it where the sampled cells are many: the sampling code costs every H wavefront
+	// a little (256^3 nanoblock, 9 800 face cells: H curl +3 us against 0.7 us saved), the
+	// strided reads it replaces cost a cache line a value (641x641x320 metasurface, two faces
+	// of 408 000 cells: 70 us a step saved)
+	// (ALIS_PRESAMPLE=force: every eligible face, 0: none)
+	long cells = 0;
+	for (int z = 0; z < n; z++) cells += (long)(zs[z].si1 - zs[z].si0 + 1) * (zs[z].sj1 - zs[z].sj0 + 1);
+	const char *env = getenv("ALIS_PRESAMPLE");
+	const bool force = env && !strcmp(env, "force"), off = env && !strcmp(env, "0");
+	if (off || (!force && 2 * cells < (long)D->iNUM * D->jNUM)) {
+		owners.clear();
+		return 0;
+	}
 	return n;
 }
 
@@ -2032,6 +2048,7 @@ int coreHalf(alis_dev_world *D, int h)
 		if (h == 0 && D->seg) hipExtLaunchKernelGGL(k_curl<T, 2, false>, grid, block, 0, D->st, s0, s1, 0, a);
 		else if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, 1, false>, grid, block, 0, D->st, s0, s1, 0, a);
 		else if (h == 0) hipExtLaunchKernelGGL(k_curl<T, 0, false>, grid, block, 0, D->st, s0, s1, 0, a);
+		else if (a.nzs) hipExtLaunchKernelGGL(k_curl<T, 0, true, true, false, 4, true>, grid, block, 0, D->st, s0, s1, 0, a);
 		else hipExtLaunchKernelGGL(k_curl<T, 0, true>, grid, block, 0, D->st, s0, s1, 0, a);
 	}
 	if (e1) {
