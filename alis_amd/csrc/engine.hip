@@ -47,6 +47,9 @@ int fail(const std::string &what)
 
 #define HIPCHECK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(#call, e_); } while (0)
 
+#ifndef K_SHUFFLE
+#define K_SHUFFLE true       // k_curl: k neighbours passed between lanes (see curl_cell.inc, KSH)
+#endif
 constexpr int ROW_ALIGN_BYTES = 128;
 constexpr int MAXPOLE = ALIS_DEV_MAXPOLES;
 
@@ -195,10 +198,11 @@ template<class T, int CM, bool NT, bool NTG, bool MZ, bool PC>
 __device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
 {
 	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true, NTC = NT && !PC;
+	constexpr int KSH = 0;
 #include "curl_cell.inc"
 }
 
-template<class T, int CM, bool ROWS, bool NT = true, bool NTG = false, int BY = 4, bool ZS = false>
+template<class T, int CM, bool ROWS, bool NT = true, bool NTG = false, int BY = 4, bool ZS = false, bool KSHUF = K_SHUFFLE>
 __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 {
 	int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
@@ -214,6 +218,9 @@ __global__ __launch_bounds__(64 * BY) void k_curl(CurlArgs<T> a)
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
 	constexpr bool SHARE = false, NTP = NT, MERGEZ = false, UNI = true, SEP = false, wr = true, NTC = NT;
+	// E only: the H launch (previous lane, lane 0 loading) measured 137.8 -> 145.7 us with it,
+	// the E launch 147.8 -> 146.6 us (256^3 nanoblock)
+	constexpr int KSH = KSHUF && !ROWS ? 1 : 0;
 #include "curl_cell.inc"
 	// H: the samples of the phasers' z faces from the values at hand -- the new H here and at
 	// k + 1 (the next lane: the host takes faces whose k + 1 lies in the same wavefront) and E,
@@ -411,6 +418,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 4) void k_step(const Ste
 				const bool wr = own;
 				constexpr int CM = CME;
 				constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, NTC = true;
+				constexpr int KSH = 0;
 #define CURL_GIVEN
 #define CURL_GIVEN_F
 #include "curl_cell.inc"
@@ -433,6 +441,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 4) void k_step(const Ste
 			const CurlArgs<T> &a = sa->h;
 			constexpr int CM = 0;
 			constexpr bool NT = true, NTG = false, SHARE = true, NTP = true, MERGEZ = false, UNI = true, SEP = true, wr = true, NTC = true;
+			constexpr int KSH = 0;
 #define CURL_GIVEN
 #define CURL_GIVEN_F
 #include "curl_cell.inc"
@@ -494,6 +503,7 @@ __global__ __launch_bounds__(256) void k_fix(CurlArgs<T> a, FixBoxes f)
 	const int i = B.i0 + di, j = B.j0 + dj, k = B.k0 + dk;
 	constexpr int CM = 0;
 	constexpr bool NT = false, NTG = false, SHARE = true, NTP = false, MERGEZ = false, UNI = false, SEP = true, wr = true, NTC = false;
+	constexpr int KSH = 0;
 #include "curl_cell.inc"
 }
 
