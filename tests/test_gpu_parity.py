@@ -174,15 +174,15 @@ def test_observers_device_sums_bitwise(prec):
 
 @pytest.mark.parametrize("name", WITH_PHASERS)
 def test_phaser_read_between_flushes(name):
-    """Phaser updates reach the accumulators eight samples at a time; a read after 13 steps
-    (five samples pending) returns the running sums of every update: in float accumulation
-    the same bits as the oracle's one update per step, alone and in a batch (whose phasers
-    are updated together)."""
-    print(f"[gpu] {name}: phaser read after 13 steps", flush=True)
-    fg, pg = run_raw("hip", name, 13, "ref")
-    fo, po = run_raw("oracle", name, 13, "ref")
+    """Phaser updates reach the accumulators sixteen samples at a time; a read after 21 steps
+    (one flush done, five samples pending) returns the running sums of every update: in float
+    accumulation the same bits as the oracle's one update per step, alone and in a batch
+    (whose phasers are updated together)."""
+    print(f"[gpu] {name}: phaser read after 21 steps", flush=True)
+    fg, pg = run_raw("hip", name, 21, "ref")
+    fo, po = run_raw("oracle", name, 21, "ref")
     assert pg.size and np.array_equal(pg, po) and np.array_equal(fg, fo)
     names = sorted(WITH_PHASERS)
-    mix = split_run(run_scenario("hip", ",".join(names), 13, "ref"), names)
-    bad = compare(run_scenario("oracle", name, 13, "ref"), mix[name])
+    mix = split_run(run_scenario("hip", ",".join(names), 21, "ref"), names)
+    bad = compare(run_scenario("oracle", name, 21, "ref"), mix[name])
     assert not bad, "\n".join(bad[:10])
