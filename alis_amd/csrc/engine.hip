@@ -47,6 +47,9 @@ int fail(const std::string &what)
 
 #define HIPCHECK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(#call, e_); } while (0)
 
+#ifndef TPB_TILE
+#define TPB_TILE 1
+#endif
 #ifndef K_SHUFFLE
 #define K_SHUFFLE true       // k_curl: k neighbours passed between lanes (see curl_cell.inc, KSH)
 #endif
@@ -542,6 +545,7 @@ constexpr int MAXCHAIN = 24;    // copies per launch
 constexpr int MAXPOST = 48;     // operations per launch
 constexpr int CURL_TPB = 2;     // tiles per workgroup of a batched curl launch (measured best of 1..16)
 constexpr int CURL_TPB_PAIR = 1;  // the same for entries of both parts of a Bloch world
+constexpr int CURL_TPB_TILE = TPB_TILE;  // the same for entries of 64 x 4 tiles (rows of 64 cells or more)
 
 struct PostPiece {
 	int kind;                   // OP_COPY, or OP_STRIP for an additive piece
@@ -3025,7 +3029,7 @@ int batchBuildCurl(alis_dev_batch *B)
 			if (pair) args.push_back(e.a[1]);
 			start.push_back(c.blocks);
 			c.n++;
-			const int tpb = pair ? CURL_TPB_PAIR : CURL_TPB;
+			const int tpb = pair ? CURL_TPB_PAIR : flat ? CURL_TPB : CURL_TPB_TILE;
 			c.blocks += (int)((e.tiles + tpb - 1) / tpb);
 		}
 		if (batchAlloc(B, &B->curl[h], args.size() * sizeof(CurlArgs<T>))) return -1;
@@ -3096,7 +3100,7 @@ int batchCore(alis_dev_batch *B, int h)
 		const int *start = (const int *)B->curlStart[h] + c.first;
 		const dim3 grid(c.blocks), block(64, 4);
 		hipEvent_t s0 = q == 0 ? e0 : nullptr, s1 = q + 1 == cls.size() ? e1 : nullptr;
-#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB, F, C>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
+#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, F ? CURL_TPB : CURL_TPB_TILE, F, C>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
 #define CURL_PAIR(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB_PAIR, F, C, true>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
 		if (c.pair) {
 			if (c.flat) {
