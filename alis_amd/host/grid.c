@@ -9,6 +9,7 @@
  * extents and Bloch phases, with their float/double promotions): the setup hashes of the
  * parity fixtures pin it bit for bit.
  */
+#include <stdint.h>
 #include <string.h>
 #include "internal.h"
 
@@ -58,9 +59,29 @@ static float uniformZ(world W, float k) { return W->dz*k; }
 static float gradedX(world W, float i) { return gradedCoord(W->Res.nug.x.m, W->Res.nug.x.Min, W->Res.nug.x.Max, W->dx, i); }
 static float gradedY(world W, float j) { return gradedCoord(W->Res.nug.y.m, W->Res.nug.y.Min, W->Res.nug.y.Max, W->dy, j); }
 static float gradedZ(world W, float k) { return gradedCoord(W->Res.nug.z.m, W->Res.nug.z.Min, W->Res.nug.z.Max, W->dz, k); }
-static int nodeX(world W, float x) { return nearestNode(W, W->itox, x); }
-static int nodeY(world W, float y) { return nearestNode(W, W->jtoy, y); }
-static int nodeZ(world W, float z) { return nearestNode(W, W->ktoz, z); }
+/* Nearest nodes already found, per axis: programs read the same probe points every step, and
+ * the search calls the coordinate function 19 times (80 ns a coordinate on the k-point worlds).
+ * A world's grid is fixed when it is created, so an entry holds for its lifetime.  Entries
+ * pack the coordinate's bits and the index into one 64-bit word, written and read whole, so a
+ * reader on another thread sees a whole entry or an old one (0: empty). */
+#define NODECACHE 64
+static int cachedNode(world W, int a, float x, float (*coord)(world, float))
+{
+	uint64_t (*c)[NODECACHE] = W->nodes;
+	if (!c) c = W->nodes = calloc(3 * NODECACHE, sizeof(uint64_t));
+	uint32_t b;
+	memcpy(&b, &x, sizeof b);
+	volatile uint64_t *e = &c[a][(b * 2654435761u) >> 26];
+	const uint64_t v = *e;
+	if (v && (uint32_t)(v >> 32) == b) return (int)(uint32_t)v;
+	const int i = nearestNode(W, coord, x);
+	*e = (uint64_t)b << 32 | (uint32_t)i;
+	return i;
+}
+
+static int nodeX(world W, float x) { return cachedNode(W, 0, x, W->itox); }
+static int nodeY(world W, float y) { return cachedNode(W, 1, y, W->jtoy); }
+static int nodeZ(world W, float z) { return cachedNode(W, 2, z, W->ktoz); }
 
 axis axisOf(world W, int a)
 {
@@ -378,7 +399,7 @@ world cloneWorld(world SW)
 	world W;
 	MALLOC(W, 0, 1);
 	*W = *SW;
-	W->dev = 0, W->hostStale = 0, W->parent = 0, W->probes = 0;
+	W->dev = 0, W->hostStale = 0, W->parent = 0, W->probes = 0, W->nodes = 0;
 	W->E = newFieldSet(W, 0);
 	W->H = newFieldSet(W, 1);
 	visitCurrents(W, W->E, W->CE, ARR_MAKE);
@@ -413,6 +434,7 @@ static void releaseWorld(world W, int ownCoeffs)
 	visitFieldSet(W, W->E, 0, ARR_FREE, comps);
 	visitFieldSet(W, W->H, 1, ARR_FREE, comps);
 	free(W->E), free(W->H);
+	free(W->nodes);
 	free(W);
 }
 

@@ -148,6 +148,7 @@ struct world {
 	world parent;       /* owning world of an auxiliary world                  */
 	int expandX, expandY; /* row aliasing of a TF/SF auxiliary world          */
 	void *probes;       /* the boxes get() reads (host/residency.c), or 0      */
+	void *nodes;        /* nearest nodes already found (host/grid.c), or 0     */
 };
 
 world createWorld(dom, res, sur, char*, ...);

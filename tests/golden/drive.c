@@ -686,6 +686,38 @@ int alis_drive(const char *name, int steps, const char *path)
 	return rc;
 }
 
+/* Nearest-node lookups (xtoi, ytoj, ztok) through the per-world cache: 3n coordinates --
+ * random ones, nodes and midpoints between nodes -- read once in order and again in reverse
+ * (after evictions and slot collisions), against a second world that meets each coordinate
+ * for the first time.  Returns the number of differing indices. */
+int alis_drive_nodes(int n)
+{
+	dom D = {{-200, 300}, {-100, 100}, {-50, 250}};
+	res R = {1, {3.7}, {1240}};
+	sur S = {{PML}, {PML}, {PML}, {8}};
+	world A = createWorld(D, R, S, ""), B = createWorld(D, R, S, "");
+	int (*fa[3])(world, float) = {A->xtoi, A->ytoj, A->ztok}, (*fb[3])(world, float) = {B->xtoi, B->ytoj, B->ztok};
+	float (*coord[3])(world, float) = {A->itox, A->jtoy, A->ktoz};
+	float *x = malloc(3 * (size_t)n * sizeof(float));
+	unsigned long long seed = 7;
+	for (int a = 0; a < 3; a++)
+		for (int q = 0; q < n; q++) {
+			seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+			const float u = (float)(seed >> 40) / 16777216.0f;
+			const int i = (int)(u * 60);
+			x[a * n + q] = q % 3 == 0 ? -260 + 620 * u : q % 3 == 1 ? coord[a](A, i) : 0.5f * (coord[a](A, i) + coord[a](A, i + 1));
+		}
+	int bad = 0;
+	for (int a = 0; a < 3; a++)
+		for (int q = 0; q < n; q++) (void)fa[a](A, x[a * n + q]);
+	for (int a = 0; a < 3; a++)
+		for (int q = n - 1; q >= 0; q--)
+			if (fa[a](A, x[a * n + q]) != fb[a](B, x[a * n + q])) bad++;
+	free(x);
+	deleteWorld(A), deleteWorld(B);
+	return bad;
+}
+
 /* Text outputs of a scenario with a phaser array after `steps` steps, written into dir:
  * writeRow and writeTxt lines, a row call with no numbers (which writes no file), the
  * Poynting and far-field spectra of the array (writePoyntingSpectrum with the array twice,
