@@ -181,16 +181,22 @@ template<bool NT, class T> __device__ inline void stS(T *p, T v)
 // also keep the j+1 row local, lose 5-11 %.
 
 template<class T, int CM, bool NT = true, bool NTG = false, bool MZ = false, bool PC = false>
-__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k);
+__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k, T (&coef)[6]);
+template<class T, int CM, bool NT, bool MZ>
+__device__ __forceinline__ void curlAtGiven(const CurlArgs<T> &a, int i, int j, int k, const T (&coef)[6]);
 
-template<class T, int CM, bool NT = true, bool NTG = false, int BY = 4>
-__device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, int bz)
+// Cell (bx, by, bz) of a 64 x 4 tile, of part a (PAIR: then of part *b, the imaginary part of
+// the same world).
+template<class T, int CM, bool PAIR>
+__device__ __forceinline__ void curlCell(const CurlArgs<T> &a, const CurlArgs<T> *b, int bx, int by, int bz)
 {
 	const int k = a.kMIN + bx * 64 + (int)threadIdx.x;
-	const int j = a.jMIN + by * BY + (int)threadIdx.y;
+	const int j = a.jMIN + by * 4 + (int)threadIdx.y;
 	const int i = a.iMIN + bz;
 	if (k >= a.kMIN + a.nk || j >= a.jMIN + a.nj) return;
-	curlAt<T, CM, NT, NTG>(a, i, j, k);
+	T coef[6];
+	curlAt<T, CM>(a, i, j, k, coef);
+	if (PAIR) curlAtGiven<T, CM, true, false>(*b, i, j, k, coef);
 }
 
 // The update of cell (i, j, k) in batched launches: the CPML faces reuse the curl's
@@ -198,11 +204,24 @@ __device__ __forceinline__ void curlCell(const CurlArgs<T> &a, int bx, int by, i
 // cover a slab with partial lines that neighbouring tiles complete).  Measured on 64 k-point
 // worlds: 5 % (E) and 14 % (H) faster than reloading and streaming; slower in k_curl.
 template<class T, int CM, bool NT, bool NTG, bool MZ, bool PC>
-__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k)
+__device__ __forceinline__ void curlAt(const CurlArgs<T> &a, int i, int j, int k, T (&coef)[6])
 {
 	constexpr bool SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true, NTC = NT && !PC;
 	constexpr int KSH = 0;
 #include "curl_cell.inc"
+	coef[0] = dx, coef[1] = dy, coef[2] = dz, coef[3] = cx, coef[4] = cy, coef[5] = cz;
+}
+
+// The same for the imaginary part of a Bloch world in the thread that updated the real part:
+// dt/d and 1/eps are the real part's (coef), not loaded again.
+template<class T, int CM, bool NT, bool MZ>
+__device__ __forceinline__ void curlAtGiven(const CurlArgs<T> &a, int i, int j, int k, const T (&coef)[6])
+{
+	constexpr bool NTG = false, SHARE = true, NTP = false, MERGEZ = MZ, UNI = !MZ, SEP = false, wr = true, NTC = NT;
+	constexpr int KSH = 0;
+#define CURL_COEF
+#include "curl_cell.inc"
+#undef CURL_COEF
 }
 
 template<class T, int CM, bool ROWS, bool NT = true, bool NTG = false, int BY = 4, bool ZS = false, bool KSHUF = K_SHUFFLE>
@@ -290,14 +309,17 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 			const int off = (t % per) * 256 + lid, jj = off / pitch - 1, kk = off - (jj + 1) * pitch;
 			if (jj < 0 || jj >= a.nj || kk < 1 || kk > a.nk) continue;
 			const int i = a.iMIN + t / per, j = a.jMIN + jj, k = a.kMIN - 1 + kk;
-			if (CM == 2 || (CM < 0 && a.seg)) curlAt<T, 2, true, false, true, PAIR>(a, i, j, k);
-			else if (CM == 1 || (CM < 0 && a.C0)) curlAt<T, 1, true, false, true, PAIR>(a, i, j, k);
-			else curlAt<T, 0, true, false, true, PAIR>(a, i, j, k);
+			T coef[6];
+			const int cm = CM >= 0 ? CM : a.seg ? 2 : a.C0 ? 1 : 0;
+			if (cm == 2) curlAt<T, 2, true, false, true, PAIR>(a, i, j, k, coef);
+			else if (cm == 1) curlAt<T, 1, true, false, true, PAIR>(a, i, j, k, coef);
+			else curlAt<T, 0, true, false, true, PAIR>(a, i, j, k, coef);
 			if (PAIR) {
+				// the imaginary part: the same world's dt/d and 1/eps
 				const CurlArgs<T> &a1 = args[2 * e + 1];
-				if (CM == 2 || (CM < 0 && a1.seg)) curlAt<T, 2, true, false, true>(a1, i, j, k);
-				else if (CM == 1 || (CM < 0 && a1.C0)) curlAt<T, 1, true, false, true>(a1, i, j, k);
-				else curlAt<T, 0, true, false, true>(a1, i, j, k);
+				if (cm == 2) curlAtGiven<T, 2, true, true>(a1, i, j, k, coef);
+				else if (cm == 1) curlAtGiven<T, 1, true, true>(a1, i, j, k, coef);
+				else curlAtGiven<T, 0, true, true>(a1, i, j, k, coef);
 			}
 		}
 		return;
@@ -306,15 +328,9 @@ __global__ __launch_bounds__(256) void k_curl_batch(const CurlArgs<T> *__restric
 	for (int t0 = (b - start[e]) * TPB, t1 = min(t0 + TPB, tiles); t0 < t1; t0++) {
 		const int t = a.alt < 0 ? tiles - 1 - t0 : t0;
 		const int bx = t % nkb, by = (t / nkb) % njb, bz = t / (nkb * njb);
-		if (CM == 2 || (CM < 0 && a.seg)) curlCell<T, 2>(a, bx, by, bz);
-		else if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1>(a, bx, by, bz);
-		else curlCell<T, 0>(a, bx, by, bz);
-		if (PAIR) {
-			const CurlArgs<T> &a1 = args[2 * e + 1];
-			if (CM == 2 || (CM < 0 && a1.seg)) curlCell<T, 2>(a1, bx, by, bz);
-			else if (CM == 1 || (CM < 0 && a1.C0)) curlCell<T, 1>(a1, bx, by, bz);
-			else curlCell<T, 0>(a1, bx, by, bz);
-		}
+		if (CM == 2 || (CM < 0 && a.seg)) curlCell<T, 2, PAIR>(a, args + 2 * e + 1, bx, by, bz);
+		else if (CM == 1 || (CM < 0 && a.C0)) curlCell<T, 1, PAIR>(a, args + 2 * e + 1, bx, by, bz);
+		else curlCell<T, 0, PAIR>(a, args + 2 * e + 1, bx, by, bz);
 	}
 }
 
