@@ -1027,6 +1027,16 @@ __global__ __launch_bounds__(256) void k_acc_to_double(const TA *src, double *ds
 
 inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// Launch kernel k with its start and end bound to events e0 and e1 (timing, or an event
+// another stream waits for); a plain launch, which costs the host less (64^3 dipole: about
+// 2 us a launch), when there are none.
+template<class... P, class... A>
+inline void launchEv(void (*k)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1, A... args)
+{
+	if (e0 || e1) hipExtLaunchKernelGGL(k, grid, block, lds, st, e0, e1, 0, args...);
+	else hipLaunchKernelGGL(k, grid, block, lds, st, args...);
+}
+
 // A fill kernel for buffers that must start from a value (used instead of a memset node,
 // which callers capturing the work into a graph should not meet).
 template<class T>
@@ -2070,11 +2080,11 @@ int coreHalf(alis_dev_world *D, int h)
 		CurlArgs<T> a = curlArgs<T>(D, h, m);
 		if (h == 1 && m == 0) a.nzs = zSamples<T>(D, a.zs, sampled);   // phasers sample part 0
 		hipEvent_t s0 = m == 0 ? e0 : nullptr, s1 = m == D->cplx ? e1 : nullptr;
-		if (h == 0 && D->seg) hipExtLaunchKernelGGL(k_curl<T, 2, false>, grid, block, 0, D->st, s0, s1, 0, a);
-		else if (h == 0 && D->C[0]) hipExtLaunchKernelGGL(k_curl<T, 1, false>, grid, block, 0, D->st, s0, s1, 0, a);
-		else if (h == 0) hipExtLaunchKernelGGL(k_curl<T, 0, false>, grid, block, 0, D->st, s0, s1, 0, a);
-		else if (a.nzs) hipExtLaunchKernelGGL(k_curl<T, 0, true, true, false, 4, true>, grid, block, 0, D->st, s0, s1, 0, a);
-		else hipExtLaunchKernelGGL(k_curl<T, 0, true>, grid, block, 0, D->st, s0, s1, 0, a);
+		if (h == 0 && D->seg) launchEv(k_curl<T, 2, false>, grid, block, 0, D->st, s0, s1, a);
+		else if (h == 0 && D->C[0]) launchEv(k_curl<T, 1, false>, grid, block, 0, D->st, s0, s1, a);
+		else if (h == 0) launchEv(k_curl<T, 0, false>, grid, block, 0, D->st, s0, s1, a);
+		else if (a.nzs) launchEv(k_curl<T, 0, true, true, false, 4, true>, grid, block, 0, D->st, s0, s1, a);
+		else launchEv(k_curl<T, 0, true>, grid, block, 0, D->st, s0, s1, a);
 	}
 	if (e1) {
 		D->evPairs[h].push_back(e0);
@@ -2420,7 +2430,11 @@ int postHalf(alis_dev_world *D, int h)
 		}
 		HIPCHECK(hipStreamWaitEvent(D->st, X->halfDone, 0));
 	}
-	if (!D->halfDone) HIPCHECK(hipEventCreateWithFlags(&D->halfDone, hipEventDisableTiming));
+	// the end of this program: an event the auxiliary world's stream waits for (a world with
+	// strips on another stream), or the main world's (an auxiliary world)
+	bool bind = D->isAux;
+	for (size_t s = 0; s < D->tf.size(); s++) bind = bind || sideStream(s);
+	if (bind && !D->halfDone) HIPCHECK(hipEventCreateWithFlags(&D->halfDone, hipEventDisableTiming));
 	int last = -1;
 	for (int g = 0; g < (int)D->post[h].size(); g++) if (D->post[h][g].total) last = g;
 	for (int g = 0; g <= last; g++) {
@@ -2429,11 +2443,11 @@ int postHalf(alis_dev_world *D, int h)
 		a.ops = (const PostOp<T> *)G.ops;
 		a.pieces = (const PostPiece *)G.pieces;
 		a.total = G.total;
-		if (g == last) hipExtLaunchKernelGGL(k_post<T>, dim3(blocksFor(G.total)), dim3(256), 0, D->st, nullptr, D->halfDone, 0, a);
+		if (g == last) launchEv(k_post<T>, dim3(blocksFor(G.total)), dim3(256), 0, D->st, nullptr, bind ? D->halfDone : nullptr, a);
 		else k_post<T><<<blocksFor(G.total), 256, 0, D->st>>>(a);
 		HIPCHECK(hipGetLastError());
 	}
-	if (last < 0) HIPCHECK(hipEventRecord(D->halfDone, D->st));
+	if (last < 0 && bind) HIPCHECK(hipEventRecord(D->halfDone, D->st));
 	for (size_t s = 0; s < D->tf.size(); s++)
 		if (sideStream(s)) HIPCHECK(hipStreamWaitEvent(D->tf[s].aux->st, D->halfDone, 0));
 	D->pendStrips[h] = 0;
@@ -3114,8 +3128,8 @@ int batchCore(alis_dev_batch *B, int h)
 		const int *start = (const int *)B->curlStart[h] + c.first;
 		const dim3 grid(c.blocks), block(64, 4);
 		hipEvent_t s0 = q == 0 ? e0 : nullptr, s1 = q + 1 == cls.size() ? e1 : nullptr;
-#define CURL_BATCH(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, F ? CURL_TPB : CURL_TPB_TILE, F, C>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
-#define CURL_PAIR(F, C) hipExtLaunchKernelGGL(k_curl_batch<T, CURL_TPB_PAIR, F, C, true>, grid, block, 0, B->st, s0, s1, 0, args, start, c.n)
+#define CURL_BATCH(F, C) launchEv(k_curl_batch<T, F ? CURL_TPB : CURL_TPB_TILE, F, C>, grid, block, 0, B->st, s0, s1, args, start, c.n)
+#define CURL_PAIR(F, C) launchEv(k_curl_batch<T, CURL_TPB_PAIR, F, C, true>, grid, block, 0, B->st, s0, s1, args, start, c.n)
 		if (c.pair) {
 			if (c.flat) {
 				if (c.cm == 2) CURL_PAIR(1, 2);
