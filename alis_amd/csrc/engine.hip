@@ -1027,16 +1027,6 @@ __global__ __launch_bounds__(256) void k_acc_to_double(const TA *src, double *ds
 
 inline unsigned blocksFor(long n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
-// Launch kernel k with its start and end bound to events e0 and e1 (timing, or an event
-// another stream waits for); a plain launch, which costs the host less (64^3 dipole: about
-// 2 us a launch), when there are none.
-template<class... P, class... A>
-inline void launchEv(void (*k)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1, A... args)
-{
-	if (e0 || e1) hipExtLaunchKernelGGL(k, grid, block, lds, st, e0, e1, 0, args...);
-	else hipLaunchKernelGGL(k, grid, block, lds, st, args...);
-}
-
 // A fill kernel for buffers that must start from a value (used instead of a memset node,
 // which callers capturing the work into a graph should not meet).
 template<class T>
@@ -1226,6 +1216,16 @@ static __device__ __align__(256) double scratch[FLUX_SCRATCH_DOUBLES];
 // ---------------------------------------------------------------------------------------
 // Device world
 // ---------------------------------------------------------------------------------------
+
+// Launch kernel k with its start and end bound to events e0 and e1 (timing, or an event
+// another stream waits for); a plain launch, which costs the host less, when there are none
+// (64^3 FP64 dipole, five launches a step: 0.080 -> 0.073-0.079 ms a step).
+template<class... P, class... A>
+inline void launchEv(void (*k)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1, A... args)
+{
+	if (e0 || e1) hipExtLaunchKernelGGL(k, grid, block, lds, st, e0, e1, 0, args...);
+	else hipLaunchKernelGGL(k, grid, block, lds, st, args...);
+}
 
 struct DevSlab {
 	bool on = false;
