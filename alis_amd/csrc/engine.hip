@@ -1303,7 +1303,7 @@ struct DevPhaser {
 	int preSlot = -1;           //  since the fields last changed
 };
 
-constexpr int PHS_RING = 16;    // samples per deferred flush
+constexpr int PHS_RING = 32;    // samples per deferred flush
 
 struct alis_dev_world {
 	alis_dev_world_desc W{};    // copy of the descriptor (arrays below keep its tables alive)

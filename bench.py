@@ -301,10 +301,12 @@ def serving(alis, args):
     spec_err = max(abs(x - y) for x, y in zip(a, b)) / max(abs(y) for y in b)
     # the same requests solved one at a time (all there at the start, one slot): the rate of
     # this load as single worlds, beside which the served rate stands
-    solo = serve.make([(0.0, r.cells, r.steps, r.radius, r.plasma) for r in reqs])
-    with quiet():
-        solo_res = serve.run(solo, args.serve_budget_gib * 2**30, 1, args.serve_tick, alis)
-    solo_rate = serve.summary(solo, solo_res)["mcell_updates_per_s"]
+    solo_rate = None
+    if args.serve_solo:
+        solo = serve.make([(0.0, r.cells, r.steps, r.radius, r.plasma) for r in reqs])
+        with quiet():
+            solo_res = serve.run(solo, args.serve_budget_gib * 2**30, 1, args.serve_tick, alis)
+        solo_rate = serve.summary(solo, solo_res)["mcell_updates_per_s"]
     return {
         "workload": f"{args.serve} requests, Poisson arrivals {args.serve_rate:g}/s (seed {args.serve_seed}), grid edge "
                     "{64,96,128,192} incl. 8-cell CPML, Drude-Lorentz sphere r 10-40 cells, TF/SF plane wave "
@@ -317,9 +319,10 @@ def serving(alis, args):
         "spectrum_rel_err_vs_fp64": spec_err,
         "accuracy_request": f"{reqs[q].cells}^3, {reqs[q].steps} steps, r {reqs[q].radius:g} cells",
         "value": round(s["mcell_updates_per_s"], 1), "unit": "Mcell-updates/s",
-        "one_at_a_time": {"value": round(solo_rate, 1), "unit": "Mcell-updates/s",
-                          "served_over_one_at_a_time": round(s["mcell_updates_per_s"] / solo_rate, 2),
-                          "sample": "the same requests, all arriving at the start, served with one slot"},
+        "one_at_a_time": None if solo_rate is None else {
+            "value": round(solo_rate, 1), "unit": "Mcell-updates/s",
+            "served_over_one_at_a_time": round(s["mcell_updates_per_s"] / solo_rate, 2),
+            "sample": "the same requests, all arriving at the start, served with one slot"},
         "max_batch": s["max_batch"], "max_request_gib": round(s["peak_request_bytes"] / 2**30, 3),
         "offered_gcell_updates": round(sum(r.cellSteps for r in res) / 1e9, 1),
     }
@@ -344,6 +347,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8, help="most requests on the device at once")
     ap.add_argument("--serve-tick", type=int, default=50, help="most steps between admissions")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="device memory for admitted requests")
+    ap.add_argument("--serve-solo", type=int, default=1, help="also solve the requests one at a time (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a copy of known size that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")

@@ -9,7 +9,7 @@ oracle.
   spectra agree with the float fixture to 1e-5 and with the FP64 oracle to 1e-5.
 * FP64 worlds: the final device state agrees with the FP64 oracle to 1e-12 relative.
 * The closed-form dispersion recurrence of a periodic vacuum box, FP64 on the device.
-* Phaser reads between deferred flushes (13 steps), alone and batched: bitwise the oracle.
+* Phaser reads between deferred flushes (37 steps), alone and batched: bitwise the oracle.
 * Full 256^3 nanoblock size: the GPU's state after 300 steps has the same checksum as the
   reference's own C code run on the host (oracle/_ref/wl_ref), and FP64 tracks FP32.
 """
@@ -148,10 +148,11 @@ def test_flux_device_sums_bitwise(prec):
     """The net flux (phaserPoyntingOut, src/phaser.c:228-245) takes its inner trapezoid sums
     from k_flux_sums, which adds the cells in the order the oracle's restatement does: every
     phaser's flux of every phaser scenario equals the oracle's to the bit (the float the API
-    returns, printed to round-trip), after 30 steps (six samples pending in the ring)."""
+    returns, printed to round-trip), after 38 steps (one flush done, six samples pending in the
+    ring)."""
     for name in sorted(WITH_PHASERS):
         print(f"[gpu] {name} {prec}: flux sums", flush=True)
-        g, o = run_scenario("hip", name, 30, prec), run_scenario("oracle", name, 30, prec)
+        g, o = run_scenario("hip", name, 38, prec), run_scenario("oracle", name, 38, prec)
         keys = [k for k in o if k.endswith(".flux")]
         assert keys and all(k in g for k in keys)
         assert all(g[k] == o[k] for k in keys), [(k, g[k], o[k]) for k in keys if g[k] != o[k]][:4]
@@ -174,15 +175,15 @@ def test_observers_device_sums_bitwise(prec):
 
 @pytest.mark.parametrize("name", WITH_PHASERS)
 def test_phaser_read_between_flushes(name):
-    """Phaser updates reach the accumulators sixteen samples at a time; a read after 21 steps
+    """Phaser updates reach the accumulators 32 samples at a time; a read after 37 steps
     (one flush done, five samples pending) returns the running sums of every update: in float
     accumulation the same bits as the oracle's one update per step, alone and in a batch
     (whose phasers are updated together)."""
-    print(f"[gpu] {name}: phaser read after 21 steps", flush=True)
-    fg, pg = run_raw("hip", name, 21, "ref")
-    fo, po = run_raw("oracle", name, 21, "ref")
+    print(f"[gpu] {name}: phaser read after 37 steps", flush=True)
+    fg, pg = run_raw("hip", name, 37, "ref")
+    fo, po = run_raw("oracle", name, 37, "ref")
     assert pg.size and np.array_equal(pg, po) and np.array_equal(fg, fo)
     names = sorted(WITH_PHASERS)
-    mix = split_run(run_scenario("hip", ",".join(names), 21, "ref"), names)
-    bad = compare(run_scenario("oracle", name, 21, "ref"), mix[name])
+    mix = split_run(run_scenario("hip", ",".join(names), 37, "ref"), names)
+    bad = compare(run_scenario("oracle", name, 37, "ref"), mix[name])
     assert not bad, "\n".join(bad[:10])
