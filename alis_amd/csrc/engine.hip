@@ -563,12 +563,15 @@ constexpr int CURL_TPB = 2;     // tiles per workgroup of a batched curl launch 
 constexpr int CURL_TPB_PAIR = 1;  // the same for entries of both parts of a Bloch world
 constexpr int CURL_TPB_TILE = TPB_TILE;  // the same for entries of 64 x 4 tiles (rows of 64 cells or more)
 
+constexpr int POST_WG = 64;     // threads (cells) per workgroup of k_post: one wavefront (64 k-point
+                                // worlds: H program 84 -> 79 us, E 35 -> 31 us against 256)
+
 struct PostPiece {
 	int kind;                   // OP_COPY, or OP_STRIP for an additive piece
 	int n;                      // operations in list
 	int si, sj, sk;             // copy piece: base cell - cell
 	Box b;                      // the cells
-	long off;                   // first thread in the launch (a multiple of 256)
+	long off;                   // first thread in the launch (a multiple of POST_WG)
 	unsigned char list[MAXPOST];// operation indices (copy piece: its chain, last copy first)
 };
 
@@ -615,7 +618,7 @@ __device__ __forceinline__ void postBlock(const PostArgs<T> &a, int blk)
 {
 	const PostPiece &P = a.pieces[blk];
 	const int ni = P.b.i1 - P.b.i0 + 1, nj = P.b.j1 - P.b.j0 + 1, nk = P.b.k1 - P.b.k0 + 1;
-	const int t = blk * 256 + (int)threadIdx.x - (int)P.off;
+	const int t = blk * POST_WG + (int)threadIdx.x - (int)P.off;
 	if (t >= ni * nj * nk) return;
 	const int dk = t % nk, r = t / nk, dj = r % nj, di = r / nj;
 	const int i = P.b.i0 + di, j = P.b.j0 + dj, k = P.b.k0 + dk;
@@ -649,14 +652,14 @@ __device__ __forceinline__ void postBlock(const PostArgs<T> &a, int blk)
 }
 
 template<class T>
-__global__ __launch_bounds__(256) void k_post(PostArgs<T> a)
+__global__ __launch_bounds__(POST_WG) void k_post(PostArgs<T> a)
 {
 	postBlock(a, (int)blockIdx.x);
 }
 
 // Several worlds' programs in one launch: entry e owns workgroups [start[e], start[e+1]).
 template<class T>
-__global__ __launch_bounds__(256) void k_post_batch(const PostArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
+__global__ __launch_bounds__(POST_WG) void k_post_batch(const PostArgs<T> *__restrict__ args, const int *__restrict__ start, int n)
 {
 	const int b = (int)blockIdx.x;
 	const int e = batchEntry(start, n, b);
@@ -1264,9 +1267,9 @@ struct DevSrc {
 
 struct PostSeg {
 	void *ops = nullptr;        // PostOp<T>[] in device memory
-	void *pieces = nullptr;     // PostPiece[total / 256] in device memory: each workgroup's
+	void *pieces = nullptr;     // PostPiece[total / POST_WG] in device memory: each workgroup's
 	                            //  piece
-	long total = 0;             // threads (a multiple of 256)
+	long total = 0;             // threads (a multiple of POST_WG)
 };
 
 struct DevTfsf {
@@ -1679,7 +1682,7 @@ void buildPieces(const std::vector<PostOp<T>> &v, int nadd, std::vector<PostPiec
 					for (int u = 0; u < P.n; u++) P.list[u] = (unsigned char)K[4 + u];
 					P.b = Box{b.i0 + di, b.i0 + di1, b.j0 + dj, b.j0 + dj1, b.k0 + dk, b.k0 + dk1};
 					P.off = total;
-					total += (P.b.count() + 255) / 256 * 256;
+					total += (P.b.count() + POST_WG - 1) / POST_WG * POST_WG;
 					out.push_back(P);
 				}
 	}
@@ -2370,9 +2373,9 @@ int buildPost(alis_dev_world *D)
 				}
 			}
 			if (seg.total >= (1L << 31)) return fail("end-of-half-step launch over 2^31 threads");
-			std::vector<PostPiece> perBlock(seg.total / 256);
+			std::vector<PostPiece> perBlock(seg.total / POST_WG);
 			for (const PostPiece &P : pieces)
-				for (long b = P.off / 256; b < (P.off + P.b.count() + 255) / 256; b++) perBlock[b] = P;
+				for (long b = P.off / POST_WG; b < (P.off + P.b.count() + POST_WG - 1) / POST_WG; b++) perBlock[b] = P;
 			if (devAlloc(D, &seg.ops, v.size() * sizeof(PostOp<T>))) return -1;
 			if (devAlloc(D, &seg.pieces, perBlock.size() * sizeof(PostPiece))) return -1;
 			HIPCHECK(hipMemcpyAsync(seg.ops, v.data(), v.size() * sizeof(PostOp<T>), hipMemcpyHostToDevice, D->st));
@@ -2443,8 +2446,8 @@ int postHalf(alis_dev_world *D, int h)
 		a.ops = (const PostOp<T> *)G.ops;
 		a.pieces = (const PostPiece *)G.pieces;
 		a.total = G.total;
-		if (g == last) launchEv(k_post<T>, dim3(blocksFor(G.total)), dim3(256), 0, D->st, nullptr, bind ? D->halfDone : nullptr, a);
-		else k_post<T><<<blocksFor(G.total), 256, 0, D->st>>>(a);
+		if (g == last) launchEv(k_post<T>, dim3(blocksFor(G.total, POST_WG)), dim3(POST_WG), 0, D->st, nullptr, bind ? D->halfDone : nullptr, a);
+		else k_post<T><<<blocksFor(G.total, POST_WG), POST_WG, 0, D->st>>>(a);
 		HIPCHECK(hipGetLastError());
 	}
 	if (last < 0 && bind) HIPCHECK(hipEventRecord(D->halfDone, D->st));
@@ -3086,7 +3089,7 @@ int batchBuildPost(alis_dev_batch *B)
 				if (r >= D->post[h].size() || !D->post[h][r].total) continue;
 				R.world.push_back(w), R.seg.push_back((int)r);
 				start.push_back((int)R.blocks);
-				R.blocks += D->post[h][r].total / 256;
+				R.blocks += D->post[h][r].total / POST_WG;
 			}
 			if (R.world.empty()) continue;
 			if (batchAlloc(B, &R.start, start.size() * sizeof(int))) return -1;
@@ -3187,7 +3190,7 @@ int batchPost(alis_dev_batch *B, int h)
 	fetchArgs(B->postArgs[h], B->postPinned[h], B->postBytes[h], B->st, B->postDone[h]);
 	for (alis_dev_batch::Round &R : B->rounds[h]) {
 		if (R.world.empty()) continue;
-		k_post_batch<T><<<R.blocks, 256, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
+		k_post_batch<T><<<R.blocks, POST_WG, 0, B->st>>>((const PostArgs<T> *)R.args, (const int *)R.start, (int)R.world.size());
 		HIPCHECK(hipGetLastError());
 	}
 	for (alis_dev_world *D : B->W) {
