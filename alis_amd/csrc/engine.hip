@@ -611,8 +611,8 @@ template<class T> __device__ inline T addOne(const PostArgs<T> &a, const PostOp<
 // boundaries, so the piece and its operations are the same for the whole workgroup; the
 // launch's table holds each workgroup's piece, so a workgroup reaches its cell through three
 // dependent loads (piece, operation, cell) where a search of the pieces added about seven
-// (64 k-point worlds, 12 000 workgroups of copies: 95 -> 84 us; 256^3 nanoblock H program
-// 17.5 -> 16.1 us).
+// (measured with 256-thread workgroups: 64 k-point worlds' H program 95 -> 84 us; 256^3
+// nanoblock H program 17.5 -> 16.1 us).
 template<class T>
 __device__ __forceinline__ void postBlock(const PostArgs<T> &a, int blk)
 {
